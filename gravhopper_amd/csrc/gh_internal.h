@@ -1,0 +1,120 @@
+// Internal declarations shared by the libgravhop.so translation units.
+// Everything here is C++/HIP; the exported C ABI is in include/gravhop.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/gravhop.h"
+
+namespace gh {
+
+// Reference unit scales (jbgrav.py:40-48, gravhopper.py:409,416): G*Msun/kpc^2 in
+// km/s/Myr and (km/s)*Myr in kpc, from CODATA-2018 G, IAU-2015 GM_sun, Julian year.
+constexpr double kAccelScale = 4.398600412921223e-09;
+constexpr double kDriftScale = 1.022712165045695e-03;
+
+constexpr int kBlock = 256;  // threads per workgroup for every 1-D kernel
+
+// A device buffer that only grows; reused across calls so the hot loop never allocates.
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+struct TreeBufs;  // gh_tree.hip
+
+struct Tuning {
+    int64_t n_lo = -1, n_hi = -1;  // bucket of N the choice was made for
+    int variant = -1;
+};
+
+}  // namespace gh
+
+struct gh_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // generic arenas
+    gh::DevBuf pos64, mass64, fpos64, acc64, src64, src32a, src32b, tgt32;
+    gh::DevBuf xs, vs, xh, hist_p, hist_v, scratch, counters;
+    gh::TreeBufs* tree = nullptr;
+    gh::Tuning tune_self, tune_pos;
+    int force_variant = -1;  // -1: autotune; otherwise a fixed FP32 direct variant
+    int64_t last_tree_nodes = 0;
+    int last_tree_depth = 0;
+    // kernel timing (gh_timing_enable / gh_timing_read)
+    bool timing = false;
+    struct Span {
+        hipEvent_t a, b;
+        int cls;
+    };
+    std::vector<Span> spans;
+    std::vector<hipEvent_t> event_pool;
+};
+
+namespace gh {
+
+// Error plumbing: every exported function returns 0 or a GH_E* code and leaves a
+// message in ctx->err.
+int fail(gh_ctx* c, int code, const char* fmt, ...);
+
+#define GH_HIP(ctx, call)                                                           \
+    do {                                                                            \
+        hipError_t e_ = (call);                                                     \
+        if (e_ != hipSuccess)                                                       \
+            return gh::fail((ctx), GH_EHIP, "%s failed: %s (%s:%d)", #call,         \
+                            hipGetErrorString(e_), __FILE__, __LINE__);             \
+    } while (0)
+
+#define GH_TRY(expr)            \
+    do {                        \
+        int rc_ = (expr);       \
+        if (rc_ != 0) return rc_; \
+    } while (0)
+
+int ensure(gh_ctx* c, DevBuf& b, size_t bytes);
+
+// Brackets the kernels launched between construction and destruction with HIP events
+// of class `cls` on stream s when timing is enabled on the context.
+struct TimedScope {
+    gh_ctx* c;
+    hipStream_t s;
+    int cls;
+    hipEvent_t b = nullptr;
+    TimedScope(gh_ctx* c_, hipStream_t s_, int cls_);
+    ~TimedScope();
+};
+int check_launch(gh_ctx* c, const char* what);
+int zero_dev(gh_ctx* c, double* p, int64_t n, hipStream_t s);
+inline hipStream_t pick_stream(gh_ctx* c, void* s) {
+    return s ? static_cast<hipStream_t>(s) : c->stream;
+}
+inline unsigned blocks_for(int64_t n, int per_block) {
+    return static_cast<unsigned>((n + per_block - 1) / per_block);
+}
+
+// direct sum (gh_direct.hip)
+int direct_dev(gh_ctx* c, const double* d_pos, const double* d_mass, int64_t np,
+               const double* d_fpos, int64_t nf, double eps, int precision, double* d_acc,
+               hipStream_t s);
+int pack_src32(gh_ctx* c, const double* d_pos, const double* d_mass, int64_t np,
+               float4* out, int64_t np_pad, hipStream_t s);
+int64_t pad_to(int64_t n, int64_t m);
+int direct_f32_packed(gh_ctx* c, const float4* src, int64_t np_pad, const float4* tgt,
+                      int64_t nt, bool self, double eps, double* d_acc, hipStream_t s);
+
+// tree (gh_tree.hip)
+int tree_dev(gh_ctx* c, const double* d_pos, const double* d_mass, int64_t np,
+             const double* d_fpos, int64_t nf, double eps, double theta, int precision,
+             double* d_acc, hipStream_t s);
+void tree_free(gh_ctx* c);
+
+}  // namespace gh

@@ -1,0 +1,177 @@
+// Leapfrog (drift-kick-drift) update kernels and FP64 diagnostics.
+//
+// The state of a device-resident run is kept as (n,3) float64 arrays: the half-step
+// positions xh (where the force is evaluated), the velocities v and the full-step
+// positions x.  One step of gravhopper/gravhopper.py:405-416 is
+//   force(xh) -> acc;   k_kick_drift: v' = v + (acc*A + 0)*dt,
+//                                     inc = ((0.5*v')*dt)*D,  x' = xh + inc,
+//                                     xh' = x' + inc  (next step's half drift)
+// so every step is one force evaluation plus one fused elementwise pass, which also
+// writes the snapshot slots of the history when asked to.
+#include "gh_internal.h"
+
+namespace gh {
+
+__global__ __launch_bounds__(kBlock) void k_half_drift(const double* __restrict__ x,
+                                                       const double* __restrict__ v,
+                                                       int64_t n3, double dt,
+                                                       double* __restrict__ xh)
+{
+    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (k < n3) xh[k] = x[k] + 0.5 * v[k] * dt * kDriftScale;
+}
+
+__global__ __launch_bounds__(kBlock) void k_kick_drift(const double* __restrict__ acc,
+                                                       int64_t n3, double dt,
+                                                       double* __restrict__ xh,
+                                                       double* __restrict__ v,
+                                                       double* __restrict__ x,
+                                                       double* __restrict__ hist_p,
+                                                       double* __restrict__ hist_v)
+{
+    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (k >= n3) return;
+    const double a = acc[k] * kAccelScale + 0.0;
+    const double vn = v[k] + a * dt;
+    const double inc = 0.5 * vn * dt * kDriftScale;
+    const double xn = xh[k] + inc;
+    v[k] = vn;
+    x[k] = xn;
+    xh[k] = xn + inc;
+    if (hist_p) hist_p[k] = xn;
+    if (hist_v) hist_v[k] = vn;
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_zero(double* __restrict__ p, int64_t n)
+{
+    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (k < n) p[k] = 0.0;
+}
+
+// Zero fill by kernel rather than hipMemsetAsync, so that captured step graphs hold
+// only kernel nodes.
+int zero_dev(gh_ctx* c, double* p, int64_t n, hipStream_t s)
+{
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_fill_zero, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, p, n);
+    return check_launch(c, "k_fill_zero");
+}
+
+int half_drift(gh_ctx* c, const double* x, const double* v, int64_t n, double dt, double* xh,
+               hipStream_t s)
+{
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_half_drift, dim3(blocks_for(3 * n, kBlock)), dim3(kBlock), 0, s, x,
+                       v, 3 * n, dt, xh);
+    return check_launch(c, "k_half_drift");
+}
+
+int kick_drift(gh_ctx* c, const double* acc, int64_t n, double dt, double* xh, double* v,
+               double* x, double* hp, double* hv, hipStream_t s)
+{
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_kick_drift, dim3(blocks_for(3 * n, kBlock)), dim3(kBlock), 0, s, acc,
+                       3 * n, dt, xh, v, x, hp, hv);
+    return check_launch(c, "k_kick_drift");
+}
+
+// ------------------------------------------------------------------------------
+// diagnostics: per-block partial sums, then one block adds them in block order, so
+// the result does not depend on scheduling.
+// ------------------------------------------------------------------------------
+constexpr int kDiag = 5;  // kinetic, potential, px, py, pz
+
+__device__ __forceinline__ double block_sum(double v, double* sh)
+{
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(kBlock) void k_energy_partials(const double4* __restrict__ src,
+                                                            const double* __restrict__ vel,
+                                                            int64_t n, double eps2,
+                                                            double* __restrict__ partial)
+{
+    __shared__ double4 tile[kBlock];
+    __shared__ double red[kBlock];
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const bool live = i < n;
+    double4 me = make_double4(0, 0, 0, 0);
+    if (live) me = src[i];
+    double row = 0.0;
+    for (int64_t base = 0; base < n; base += kBlock) {
+        __syncthreads();
+        if (base + threadIdx.x < n) tile[threadIdx.x] = src[base + threadIdx.x];
+        __syncthreads();
+        const int cnt = (int)((n - base) < kBlock ? (n - base) : kBlock);
+        if (!live) continue;
+        for (int jj = 0; jj < cnt; jj++) {
+            if (base + jj == i) continue;
+            const double4 s = tile[jj];
+            const double dx = s.x - me.x, dy = s.y - me.y, dz = s.z - me.z;
+            const double d = dx * dx + dy * dy + dz * dz + eps2;
+            row += s.w / sqrt(d);
+        }
+    }
+    double ke = 0.0, pe = 0.0, p0 = 0.0, p1 = 0.0, p2 = 0.0;
+    if (live) {
+        const double vx = vel[3 * i], vy = vel[3 * i + 1], vz = vel[3 * i + 2];
+        ke = 0.5 * me.w * (vx * vx + vy * vy + vz * vz);
+        pe = -0.5 * me.w * row;
+        p0 = me.w * vx;
+        p1 = me.w * vy;
+        p2 = me.w * vz;
+    }
+    double r[kDiag] = {ke, pe, p0, p1, p2};
+    for (int q = 0; q < kDiag; q++) {
+        const double t = block_sum(r[q], red);
+        if (threadIdx.x == 0) partial[(int64_t)blockIdx.x * kDiag + q] = t;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_energy_final(const double* __restrict__ partial,
+                                                         int64_t nblocks,
+                                                         double* __restrict__ out)
+{
+    if (threadIdx.x < kDiag) {
+        double s = 0.0;
+        for (int64_t b = 0; b < nblocks; b++) s += partial[b * kDiag + threadIdx.x];
+        out[threadIdx.x] = s;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_pack_src64_e(const double* __restrict__ pos,
+                                                         const double* __restrict__ mass,
+                                                         int64_t n, double4* __restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) out[i] = make_double4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], mass[i]);
+}
+
+int energy_dev(gh_ctx* c, const double* d_pos, const double* d_vel, const double* d_mass,
+               int64_t n, double eps, double* d_out5, hipStream_t s)
+{
+    const int64_t nb = blocks_for(n, kBlock);
+    GH_TRY(ensure(c, c->src64, sizeof(double4) * (size_t)(n > 0 ? n : 1)));
+    GH_TRY(ensure(c, c->counters, sizeof(double) * kDiag * (size_t)(nb + 1)));
+    double4* src = c->src64.as<double4>();
+    double* part = c->counters.as<double>() + kDiag;
+    if (n > 0) {
+        hipLaunchKernelGGL(k_pack_src64_e, dim3(nb), dim3(kBlock), 0, s, d_pos, d_mass, n, src);
+        GH_TRY(check_launch(c, "k_pack_src64_e"));
+        hipLaunchKernelGGL(k_energy_partials, dim3(nb), dim3(kBlock), 0, s, src, d_vel, n,
+                           eps * eps, part);
+        GH_TRY(check_launch(c, "k_energy_partials"));
+    }
+    hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(kBlock), 0, s, part, nb, d_out5);
+    return check_launch(c, "k_energy_final");
+}
+
+}  // namespace gh

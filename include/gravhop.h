@@ -1,0 +1,152 @@
+/*
+ * gravhop.h -- C ABI of libgravhop.so, the MI355X (gfx950) gravity engine behind
+ * gravhopper_amd's drop-in replacements of GravHopper's force entry points and
+ * Simulation.run().
+ *
+ * Conventions
+ *   - Code units as in the reference C engine: positions in kpc, masses in Msun,
+ *     G = 1; accelerations come back in G*Msun/kpc^2 (gravhopper/_jbgrav.c works in
+ *     these units and gravhopper/jbgrav.py:38-48 scales them).
+ *   - Position arrays are (n,3) row-major float64 ("AoS"), like the C-contiguous
+ *     numpy arrays the reference receives (_jbgrav.c:79-80).
+ *   - Every function returns 0 on success or a negative GH_E* code; the message of
+ *     the last failure on a context is gh_last_error(ctx).  Nothing calls exit().
+ *   - A context owns one HIP stream and device arenas that grow on demand and are
+ *     reused.  Calls on one context must not overlap; separate contexts may.
+ *   - "_dev" entry points take device pointers and a hipStream_t (passed as void*;
+ *     NULL selects the context's stream) and return without synchronising.  The
+ *     others take host pointers and return when the result is in host memory.
+ */
+#ifndef GRAVHOP_H
+#define GRAVHOP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gh_ctx gh_ctx;
+
+/* error codes */
+#define GH_OK 0
+#define GH_EARG -1        /* invalid argument (shape, size, value)                   */
+#define GH_EHIP -2        /* HIP runtime error                                       */
+#define GH_ENOMEM -3      /* device allocation failed                                */
+#define GH_ECOINCIDENT -4 /* tree: two particles at the same position                */
+#define GH_EDEPTH -5      /* tree: particles closer than the deepest supported level */
+
+/* precision of the pair arithmetic */
+#define GH_FP64 0 /* IEEE FP64 in the reference's operation order                  */
+#define GH_FP32 1 /* FP32 pair forces, FP32 tile sums, two-sum across tiles; FP64 out */
+
+/* force algorithm of gh_run */
+#define GH_DIRECT 0
+#define GH_TREE 1
+
+/* Context management (no reference counterpart: the reference keeps no state). */
+int gh_ctx_create(int device, gh_ctx **out);
+void gh_ctx_destroy(gh_ctx *ctx);
+const char *gh_last_error(const gh_ctx *ctx);
+int gh_device_count(int *count);
+int gh_sync(gh_ctx *ctx);
+
+/* Direct summation.  Replaces _jbgrav.direct_summation(pos, mass, eps)
+ * (gravhopper/_jbgrav.c:68-135, workhorse :140-193).  acc is (np,3).  Self pairs are
+ * excluded by index; with GH_FP64 the result is bit-identical to the reference. */
+int gh_direct_summation(gh_ctx *ctx, const double *pos, const double *mass, int64_t np,
+                        double eps, int precision, double *acc);
+
+/* Field at force_pos from all particles.  Replaces
+ * _jbgrav.direct_summation_position(pos, mass, force_pos, eps)
+ * (gravhopper/_jbgrav.c:200-294, workhorse :299-353).  A pair whose softened
+ * squared distance is exactly 0 contributes 0 (:327-330).  acc is (nf,3). */
+int gh_direct_summation_position(gh_ctx *ctx, const double *pos, const double *mass,
+                                 int64_t np, const double *force_pos, int64_t nf,
+                                 double eps, int precision, double *acc);
+
+/* Device-pointer form of both: force_pos == NULL means "the particles themselves"
+ * (index self-exclusion), otherwise nf targets. */
+int gh_direct_summation_dev(gh_ctx *ctx, const double *d_pos, const double *d_mass,
+                            int64_t np, const double *d_force_pos, int64_t nf, double eps,
+                            int precision, double *d_acc, void *stream);
+
+/* Barnes-Hut octree force.  Replaces _jbgrav.tree_force(pos, mass, eps, theta)
+ * (gravhopper/_jbgrav.c:564-630, workhorse :737-806).  The octree is the reference's
+ * (one particle per leaf, strict '>' octants, FP64 centre recurrence, bounding-box
+ * size rule of :764-769), built on the device; the walk opens a node unless it is a
+ * leaf or size/|centre - x| < theta (:496-502) and adds monopoles (:506-524). */
+int gh_tree_force(gh_ctx *ctx, const double *pos, const double *mass, int64_t np,
+                  double eps, double theta, int precision, double *acc);
+
+/* Replaces _jbgrav.tree_force_position(pos, mass, force_pos, eps, theta)
+ * (gravhopper/_jbgrav.c:634-727). */
+int gh_tree_force_position(gh_ctx *ctx, const double *pos, const double *mass,
+                           int64_t np, const double *force_pos, int64_t nf, double eps,
+                           double theta, int precision, double *acc);
+
+/* Device-pointer form of both (force_pos == NULL: targets are the particles). */
+int gh_tree_force_dev(gh_ctx *ctx, const double *d_pos, const double *d_mass, int64_t np,
+                      const double *d_force_pos, int64_t nf, double eps, double theta,
+                      int precision, double *d_acc, void *stream);
+
+/* Canonical serialisation of the device-built octree, depth-first pre-order with
+ * children in branch order 0..7 (the visiting order of gravoct_calc_accel,
+ * gravhopper/_jbgrav.c:530-537).  One record per node. */
+typedef struct {
+    int32_t level;   /* root = 0 */
+    int32_t branch;  /* octant within the parent (bit0 x, bit1 y, bit2 z); -1 root */
+    int64_t leaf;    /* particle index for a leaf, -1 for an internal node */
+    double center[3];
+    double size;
+    double mass;
+    double com[3];
+} gh_node_rec;
+
+/* Writes min(n_nodes, cap) records to out (host memory, may be NULL when cap == 0)
+ * and the node count to *n_nodes. */
+int gh_tree_dump(gh_ctx *ctx, const double *pos, const double *mass, int64_t np,
+                 double eps, gh_node_rec *out, int64_t cap, int64_t *n_nodes);
+
+/* Device-resident drift-kick-drift leapfrog: nsteps of Simulation.perform_timestep
+ * (gravhopper/gravhopper.py:405-416) with the force of calculate_acceleration
+ * (:419-459) and no external forces, in code units (kpc, km/s, Msun, Myr):
+ *   xh = x + ((0.5 v) dt) D;  a = F(xh) A;  v' = v + (a + 0) dt;  x' = xh + ((0.5 v') dt) D
+ * with D = 1.022712165045695e-3 kpc per km/s*Myr and A = 4.398600412921223e-9 km/s/Myr
+ * per G*Msun/kpc^2.  pos/vel (np,3) host arrays are advanced in place.  hist_pos /
+ * hist_vel (nullable) receive nsteps snapshots of (np,3) after every step. */
+int gh_run(gh_ctx *ctx, double *pos, double *vel, const double *mass, int64_t np,
+           double dt, double eps, int algorithm, double theta, int precision,
+           int64_t nsteps, double *hist_pos, double *hist_vel);
+
+/* Device-pointer form of gh_run: d_pos/d_vel (np,3) are advanced in place on the
+ * device; d_hist_pos/d_hist_vel (nullable) are device arrays of nsteps*(np,3). */
+int gh_run_dev(gh_ctx *ctx, double *d_pos, double *d_vel, const double *d_mass, int64_t np,
+               double dt, double eps, int algorithm, double theta, int precision,
+               int64_t nsteps, double *d_hist_pos, double *d_hist_vel, void *stream);
+
+/* Kernel timing (no reference counterpart).  While enabled, the library brackets its
+ * kernels with HIP events on the stream they run on; gh_timing_read synchronises and
+ * returns the summed duration [ms] and launch count of one kernel class since the
+ * last enable.  Classes: */
+#define GH_T_DIRECT_F32 0 /* FP32 direct-sum pair kernel                       */
+#define GH_T_DIRECT_F64 1 /* FP64 direct-sum pair kernel                       */
+#define GH_T_TREE_BUILD 2 /* octree build (keys, sort, linking, moments)       */
+#define GH_T_TREE_WALK 3  /* octree walk                                       */
+#define GH_T_UPDATE 4     /* fused leapfrog update                             */
+#define GH_T_COUNT 5
+int gh_timing_enable(gh_ctx *ctx, int on);
+int gh_timing_read(gh_ctx *ctx, int kernel_class, double *total_ms, int64_t *launches);
+
+/* FP64 diagnostics (no reference counterpart): kinetic energy sum 0.5 m v^2
+ * [Msun (km/s)^2], potential energy -sum_{i<j} m_i m_j / sqrt(r^2 + eps^2)
+ * [Msun^2/kpc, G = 1] and momentum sum m v [Msun km/s] (3 doubles). */
+int gh_energy(gh_ctx *ctx, const double *pos, const double *vel, const double *mass,
+              int64_t np, double eps, double *kinetic, double *potential,
+              double *momentum);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GRAVHOP_H */

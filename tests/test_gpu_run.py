@@ -1,0 +1,92 @@
+"""Device-resident leapfrog (gh_run) and the Simulation drop-in on the GPU.
+
+The FP64 direct run is bit-identical to the oracle's DKD restatement of
+gravhopper.py:405-416 (so 400 steps of the README Plummer run agree to the last bit),
+host-driven and device-resident Simulation runs agree bit for bit, and continued runs
+equal one long run.
+"""
+import numpy as np
+import pytest
+
+from conftest import assert_bitwise, rel_err
+from gravhopper_amd import Simulation, _native, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def test_run_fp64_bitwise_vs_oracle_short(ctx, oracle):
+    ic = workloads.plummer(500, seed=31)
+    x, v, hp, hv = ctx.run(ic["pos"], ic["vel"], ic["mass"], 0.005, 5e-5, 10, history=True)
+    xo, vo, hpo, hvo = oracle.dkd_direct(ic["pos"], ic["vel"], ic["mass"], 0.005, 5e-5, 10,
+                                         history=True)
+    assert_bitwise(x, xo, "pos")
+    assert_bitwise(v, vo, "vel")
+    assert_bitwise(hp, hpo, "hist pos")
+    assert_bitwise(hv, hvo, "hist vel")
+
+
+def test_run_config1_400_steps_bitwise(ctx, oracle):
+    # README example: Plummer N=2000, b=1 pc, 1e6 Msun, eps=0.05 pc, dt=0.005 Myr, 400 steps
+    ic = workloads.plummer(2000, b=1e-3, mass=1e6, seed=20260001)
+    x, v = ctx.run(ic["pos"], ic["vel"], ic["mass"], 0.005, 5e-5, 400)
+    xo, vo = oracle.dkd_direct(ic["pos"], ic["vel"], ic["mass"], 0.005, 5e-5, 400)
+    assert_bitwise(x, xo, "final pos")
+    assert_bitwise(v, vo, "final vel")
+
+
+def test_simulation_device_vs_host_loop_bitwise(ctx):
+    ic = workloads.plummer(300, seed=32)
+    a = Simulation(dt=0.005, eps=5e-5, algorithm='direct')
+    a.add_IC(ic)
+    a.run(6)
+    b = Simulation(dt=0.005, eps=5e-5, algorithm='direct')
+    b.add_IC(ic)
+    # a zero external force forces the host-driven loop (gravhopper.py:317-320)
+    b.add_external_force(lambda pos, args: np.zeros_like(pos))
+    b.run(6)
+    assert_bitwise(a.positions, b.positions, "positions")
+    assert_bitwise(a.velocities, b.velocities, "velocities")
+    np.testing.assert_array_equal(a.times, b.times)
+
+
+def test_simulation_continue_equals_long_run(ctx):
+    ic = workloads.plummer(256, seed=33)
+    a = Simulation(dt=0.002, eps=1e-4, algorithm='direct')
+    a.add_IC(ic)
+    a.run(5)
+    a.run(7)
+    b = Simulation(dt=0.002, eps=1e-4, algorithm='direct')
+    b.add_IC(ic)
+    b.run(12)
+    assert a.positions.shape == (13, 256, 3)
+    assert_bitwise(a.positions, b.positions)
+    assert_bitwise(a.velocities, b.velocities)
+
+
+def test_simulation_external_forces(ctx):
+    # harmonic external potential on a single particle: DKD closed form
+    # cos(Omega dt) = 1 - (omega dt)^2 / 2 (no N-body force for Np = 1)
+    omega = 0.05  # 1/Myr
+    sim = Simulation(dt=1.0, eps=0.1, algorithm='direct')
+    sim.add_IC({'pos': np.array([[1.0, 0.0, 0.0]]), 'vel': np.zeros((1, 3)), 'mass': np.ones(1)})
+    # acceleration in km/s/Myr for x in kpc: -omega^2 x converted from kpc/Myr^2
+    sim.add_external_force(lambda pos, args: -omega ** 2 * pos / 1.022712165045695e-03)
+    sim.run(200)
+    x = sim.positions[:, 0, 0]
+    big_omega = np.arccos(1 - (omega * 1.0) ** 2 / 2)
+    t = np.arange(201)
+    np.testing.assert_allclose(x, np.cos(big_omega * t) / np.cos(big_omega / 2) *
+                               np.cos(big_omega / 2), atol=2e-3)
+
+
+def test_simulation_fp32_close_to_fp64(ctx):
+    ic = workloads.plummer(4096, seed=34)
+    res = {}
+    for prec in ('fp64', 'fp32'):
+        s = Simulation(dt=0.001, eps=1e-5, algorithm='direct', precision=prec)
+        s.add_IC(ic)
+        s.run(5)
+        res[prec] = s.positions[-1]
+    disp = np.linalg.norm(res['fp64'] - ic['pos'], axis=1)
+    diff = np.linalg.norm(res['fp32'] - res['fp64'], axis=1)
+    assert np.median(diff / disp) < 1e-5
