@@ -43,6 +43,7 @@ SIGNATURES = {
     "gh_tree_force_position": ([_P, _P, _P, _I64, _P, _I64, _D, _D, _I, _P], _I),
     "gh_tree_force_dev": ([_P, _P, _P, _I64, _P, _I64, _D, _D, _I, _P, _P], _I),
     "gh_tree_dump": ([_P, _P, _P, _I64, _D, _P, _I64, ctypes.POINTER(_I64)], _I),
+    "gh_tree_interactions": ([_P, _P, _P, _I64, _P, _I64, _D, _D, _P, _P], _I),
     "gh_run": ([_P, _P, _P, _P, _I64, _D, _D, _I, _D, _I, _I64, _P, _P], _I),
     "gh_run_dev": ([_P, _P, _P, _P, _I64, _D, _D, _I, _D, _I, _I64, _P, _P, _P], _I),
     "gh_timing_enable": ([_P, _I], _I),
@@ -175,6 +176,19 @@ class Context:
         self.check(self.lib.gh_tree_dump(self.handle, _ptr(pos), _ptr(mass), len(mass),
                                          float(eps), _ptr(out), n.value, ctypes.byref(n)))
         return out
+
+    def tree_interactions(self, pos, mass, eps, theta=0.7, force_pos=None):
+        """FP64 tree force and accepted-node count per target."""
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        mass = np.ascontiguousarray(mass, dtype=np.float64)
+        fpos = None if force_pos is None else np.ascontiguousarray(force_pos, dtype=np.float64)
+        nt = len(mass) if fpos is None else len(fpos)
+        out = np.empty((nt, 3))
+        cnt = np.empty(nt, dtype=np.int64)
+        self.check(self.lib.gh_tree_interactions(
+            self.handle, _ptr(pos), _ptr(mass), len(mass), _ptr(fpos), nt if fpos is not None
+            else 0, float(eps), float(theta), _ptr(out), _ptr(cnt)))
+        return out, cnt
 
     # -- device-resident leapfrog ----------------------------------------------
     def run(self, pos, vel, mass, dt, eps, nsteps, algorithm=GH_DIRECT, theta=0.7,

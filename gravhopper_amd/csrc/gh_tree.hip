@@ -1,26 +1,986 @@
-// Barnes-Hut octree on the device (placeholder until the build and walk land).
+// Barnes-Hut octree on the device, with the reference's exact topology.
+//
+// The reference (gravhopper/_jbgrav.c:360-462, 737-786) inserts particles one by one
+// into a pointer octree: one particle per leaf, octant bit k = (x_k > c_k) strictly,
+// branch = bit0 + 2 bit1 + 4 bit2, child centre c + (s * 0.5) * halfsize (FP64),
+// child size = parent halfsize, root box from the particle extent with the size rule
+// of :764-769.  The resulting tree does not depend on insertion order: it is the
+// minimal octree in which every particle sits alone in a leaf, each particle's octant
+// path being fixed by its own position and the FP64 centre recurrence.
+//
+// Device construction of that same tree:
+//   1. k_bbox_*       extent -> root centre and size (the reference's arithmetic)
+//   2. k_keys         per particle, octant digits of W*21 levels of the recurrence,
+//                     21 levels (63 bits) per 64-bit word, level 0 most significant
+//   3. radix sort     stable LSD sort of (key words, index), 8-bit digits
+//                     (k_rs_hist / k_rs_scan / k_rs_scatter, wave-ballot ranking)
+//   4. k_delta        delta_i = number of leading levels shared by sorted keys i, i+1;
+//                     keys equal over all W words -> more words (or coincident error)
+//   5. nodes          in depth-first pre-order (children in branch order) the nodes
+//                     whose particle range starts at sorted position i are the internal
+//                     nodes of levels (delta_{i-1}, delta_i] followed by the leaf of i at
+//                     level max(delta_{i-1}, delta_i) + 1, so one scan of
+//                     max(0, delta_i - delta_{i-1}) + 1 numbers the whole tree; a node's
+//                     range ends at the first j >= i with delta_j < level (sparse-table
+//                     query) and its skip pointer is the first node of position end+1
+//   6. moments        node mass and first moment as differences of double-double prefix
+//                     sums over the sorted particles (error far below one FP64 ulp)
+//   7. walk           stackless pre-order walk with skip pointers.  A wavefront walks 64
+//                     targets in lockstep: every lane keeps its own next node, the wave
+//                     visits the smallest of them (next = n+1 if any active lane opened
+//                     n, else skip[n]), so each lane accepts exactly the nodes the
+//                     reference's recursive walk (:487-541) accepts, while the node index
+//                     stays wave-uniform and node records arrive through scalar loads.
+#include <algorithm>
+
 #include "gh_internal.h"
 
 namespace gh {
 
-struct TreeBufs {};
+typedef unsigned long long u64;
+typedef unsigned int u32;
 
-int tree_dev(gh_ctx* c, const double*, const double*, int64_t, const double*, int64_t, double,
-             double, int, double*, hipStream_t)
+constexpr int kLevelsPerWord = 21;
+constexpr int kMaxWords = 8;  // 168 levels
+constexpr int kRsItems = 16;
+constexpr int kRsChunk = kBlock * kRsItems;  // elements per radix-sort workgroup
+
+// FP64 node record of the walk and the dump
+struct NodeD {
+    double cx, cy, cz, size;
+    double mx, my, mz, mass;  // centre of mass, mass
+};
+// FP32 walk record: geo = (centre, (size/theta)^2 or -1 for a leaf), mon = (com, mass)
+struct NodeF {
+    float4 geo;
+    float4 mon;
+};
+
+struct DD4 {
+    double h[4], l[4];
+};
+
+struct TreeBufs {
+    DevBuf keys[2];  // W words per element, word-major: keys[w * n + i]
+    DevBuf idx[2];
+    DevBuf hist, bbox_part, box, flags, delta, off, sparse, prefix, block_sums;
+    DevBuf nodes64, nodes32, skip, level, branch, leaf;
+    DevBuf tgt_order;
+    int64_t n_nodes = 0;
+    int words = 0;
+    int key_sel = 0;  // which of keys[0]/keys[1] holds the sorted keys
+};
+
+// ------------------------------------------------------------------------------
+// 1. bounding box
+// ------------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_bbox_partial(const double* __restrict__ pos,
+                                                         int64_t n, double* __restrict__ part)
 {
-    return fail(c, GH_EARG, "tree force not available in this build");
+    __shared__ double smin[3][kBlock], smax[3][kBlock];
+    double mn[3], mx[3];
+    for (int k = 0; k < 3; k++) mn[k] = mx[k] = pos[k];  // particle 0 seeds, as :747-750
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kBlock) {
+        for (int k = 0; k < 3; k++) {
+            const double q = pos[3 * i + k];
+            if (q < mn[k]) mn[k] = q;
+            if (q > mx[k]) mx[k] = q;
+        }
+    }
+    for (int k = 0; k < 3; k++) {
+        smin[k][threadIdx.x] = mn[k];
+        smax[k][threadIdx.x] = mx[k];
+    }
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+            for (int k = 0; k < 3; k++) {
+                const double a = smin[k][threadIdx.x + w], b = smax[k][threadIdx.x + w];
+                if (a < smin[k][threadIdx.x]) smin[k][threadIdx.x] = a;
+                if (b > smax[k][threadIdx.x]) smax[k][threadIdx.x] = b;
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3; k++) {
+            part[6 * blockIdx.x + k] = smin[k][0];
+            part[6 * blockIdx.x + 3 + k] = smax[k][0];
+        }
+}
+
+// box[0..2] = centre, box[3] = size (gravhopper/_jbgrav.c:764-772)
+__global__ void k_bbox_final(const double* __restrict__ part, int nparts,
+                             double eps, double* __restrict__ box)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double mn[3], mx[3];
+    for (int k = 0; k < 3; k++) {
+        mn[k] = part[k];
+        mx[k] = part[3 + k];
+    }
+    for (int b = 1; b < nparts; b++)
+        for (int k = 0; k < 3; k++) {
+            if (part[6 * b + k] < mn[k]) mn[k] = part[6 * b + k];
+            if (part[6 * b + 3 + k] > mx[k]) mx[k] = part[6 * b + 3 + k];
+        }
+    double bs = mx[0] - mn[0] + eps;
+    for (int k = 1; k < 3; k++)
+        if ((mx[k] - mn[k]) > bs) bs = mx[k] - mn[k] + eps;
+    for (int k = 0; k < 3; k++) box[k] = 0.5 * (mn[k] + mx[k]);
+    box[3] = bs;
+}
+
+// ------------------------------------------------------------------------------
+// 2. keys from the FP64 centre recurrence
+// ------------------------------------------------------------------------------
+// One level: octant digit of x against centre c, then the child centre
+// c + (s * 0.5) * h with s = +-1 and h the current node's halfsize (:406, :426, :456).
+__device__ __forceinline__ int descend(double x, double y, double z, double& cx, double& cy,
+                                       double& cz, double h)
+{
+    const bool gx = x > cx, gy = y > cy, gz = z > cz;
+    cx = cx + (gx ? 0.5 : -0.5) * h;
+    cy = cy + (gy ? 0.5 : -0.5) * h;
+    cz = cz + (gz ? 0.5 : -0.5) * h;
+    return (int)gx | ((int)gy << 1) | ((int)gz << 2);
+}
+
+__global__ __launch_bounds__(kBlock) void k_keys(const double* __restrict__ pos, int64_t n,
+                                                 const double* __restrict__ box, int words,
+                                                 u64* __restrict__ keys, u32* __restrict__ idx)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
+    double cx = box[0], cy = box[1], cz = box[2];
+    double size = box[3];
+    for (int w = 0; w < words; w++) {
+        u64 k = 0;
+        for (int l = 0; l < kLevelsPerWord; l++) {
+            const double h = 0.5 * size;
+            const int d = descend(x, y, z, cx, cy, cz, h);
+            k = (k << 3) | (u64)d;
+            size = h;
+        }
+        keys[(int64_t)w * n + i] = k;
+    }
+    idx[i] = (u32)i;
+}
+
+// ------------------------------------------------------------------------------
+// 3. LSD radix sort, 8-bit digits, stable
+// ------------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_rs_hist(const u64* __restrict__ key, int64_t n,
+                                                    int shift, u32* __restrict__ hist,
+                                                    int nblocks)
+{
+    __shared__ u32 h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * kRsChunk;
+    for (int it = 0; it < kRsItems; it++) {
+        const int64_t i = base + (int64_t)it * kBlock + threadIdx.x;
+        if (i < n) atomicAdd(&h[(key[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+// exclusive scan of len u32 values by one workgroup of 1024 threads
+__global__ __launch_bounds__(1024) void k_rs_scan(u32* __restrict__ v, int64_t len)
+{
+    __shared__ u32 part[1024];
+    const int64_t per = (len + 1023) / 1024;
+    const int64_t b = (int64_t)threadIdx.x * per;
+    const int64_t e = min(len, b + per);
+    u32 s = 0;
+    for (int64_t k = b; k < e; k++) s += v[k];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const u32 t = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += t;
+        __syncthreads();
+    }
+    u32 run = threadIdx.x ? part[threadIdx.x - 1] : 0u;
+    for (int64_t k = b; k < e; k++) {
+        const u32 t = v[k];
+        v[k] = run;
+        run += t;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_rs_scatter(
+    const u64* __restrict__ kin, u64* __restrict__ kout, const u32* __restrict__ iin,
+    u32* __restrict__ iout, int64_t n, int words, int wsel, int shift,
+    const u32* __restrict__ hist, int nblocks)
+{
+    __shared__ u32 gbase[256];
+    __shared__ u32 wcount[kBlock / 64][256];
+    __shared__ u32 woff[kBlock / 64][256];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    gbase[threadIdx.x] = hist[(int64_t)threadIdx.x * nblocks + blockIdx.x];
+    for (int w = 0; w < kBlock / 64; w++) wcount[w][threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * kRsChunk;
+    const u64 lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int it = 0; it < kRsItems; it++) {
+        const int64_t i = base + (int64_t)it * kBlock + threadIdx.x;
+        const bool valid = i < n;
+        const u32 d = valid ? (u32)((kin[(int64_t)wsel * n + i] >> shift) & 255u) : 0u;
+        u64 peers = __ballot(valid);
+        for (int b = 0; b < 8; b++) {
+            const u64 m = __ballot((d >> b) & 1u);
+            peers &= ((d >> b) & 1u) ? m : ~m;
+        }
+        const u32 rank = (u32)__popcll(peers & lt);
+        if (valid && rank == 0) wcount[wave][d] = (u32)__popcll(peers);
+        __syncthreads();
+        {
+            const int dd = threadIdx.x;
+            u32 run = gbase[dd];
+            for (int w = 0; w < kBlock / 64; w++) {
+                woff[w][dd] = run;
+                run += wcount[w][dd];
+                wcount[w][dd] = 0;
+            }
+            gbase[dd] = run;
+        }
+        __syncthreads();
+        if (valid) {
+            const u32 dst = woff[wave][d] + rank;
+            for (int w = 0; w < words; w++) kout[(int64_t)w * n + dst] = kin[(int64_t)w * n + i];
+            iout[dst] = iin[i];
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------------------
+// 4. shared levels of adjacent sorted keys
+// ------------------------------------------------------------------------------
+// dp[i] = delta_i + 1 (delta_i = shared leading levels of sorted keys i and i+1), with
+// dp[n-1] = 0; flags[0] counts pairs equal over all words, flags[1] counts those at
+// identical positions, flags[2] = max delta + 1.
+__global__ __launch_bounds__(kBlock) void k_delta(const u64* __restrict__ keys, int64_t n,
+                                                  int words, const u32* __restrict__ perm,
+                                                  const double* __restrict__ pos,
+                                                  unsigned char* __restrict__ dp,
+                                                  int* __restrict__ flags)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    if (i == n - 1) {
+        dp[i] = 0;
+        return;
+    }
+    int delta = words * kLevelsPerWord;
+    for (int w = 0; w < words; w++) {
+        const u64 a = keys[(int64_t)w * n + i], b = keys[(int64_t)w * n + i + 1];
+        if (a != b) {
+            const int lz = __clzll(a ^ b);  // >= 1: bit 63 is never used
+            delta = w * kLevelsPerWord + (lz - 1) / 3;
+            break;
+        }
+    }
+    if (delta == words * kLevelsPerWord) {
+        atomicAdd(&flags[0], 1);
+        const u32 p = perm[i], q = perm[i + 1];
+        if (pos[3 * p] == pos[3 * q] && pos[3 * p + 1] == pos[3 * q + 1] &&
+            pos[3 * p + 2] == pos[3 * q + 2])
+            atomicAdd(&flags[1], 1);
+    }
+    dp[i] = (unsigned char)(delta + 1);
+    atomicMax(&flags[2], delta + 1);
+}
+
+// cnt[i] = internal nodes starting at sorted position i, plus its leaf
+__global__ __launch_bounds__(kBlock) void k_node_counts(const unsigned char* __restrict__ dp,
+                                                        int64_t n, long long* __restrict__ cnt)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int lprev = (i == 0) ? -1 : (int)dp[i - 1] - 1;
+    const int lcur = (int)dp[i] - 1;
+    cnt[i] = (long long)(lcur > lprev ? lcur - lprev : 0) + 1;
+}
+
+// ------------------------------------------------------------------------------
+// generic three-phase scans (int64 sums and double-double 4-vectors)
+// ------------------------------------------------------------------------------
+__device__ __forceinline__ void dd_add(double& h, double& l, double bh, double bl)
+{
+    const double s = h + bh;
+    const double bp = s - h;
+    double e = (h - (s - bp)) + (bh - bp);
+    e += l + bl;
+    h = s + e;
+    l = e - (h - s);
+}
+
+struct OpI64 {
+    typedef long long T;
+    __device__ static T zero() { return 0; }
+    __device__ static T add(T a, T b) { return a + b; }
+};
+struct OpDD4 {
+    typedef DD4 T;
+    __device__ static T zero()
+    {
+        DD4 z;
+        for (int k = 0; k < 4; k++) z.h[k] = z.l[k] = 0.0;
+        return z;
+    }
+    __device__ static T add(T a, const T& b)
+    {
+        for (int k = 0; k < 4; k++) dd_add(a.h[k], a.l[k], b.h[k], b.l[k]);
+        return a;
+    }
+};
+
+constexpr int kScanItems = 4;
+constexpr int kScanChunk = kBlock * kScanItems;
+
+// in[0..n) -> out[0..n] exclusive (out[0] = 0, out[n] = total); phase 1 writes the
+// block-local exclusive prefix and the block totals.
+template <class Op>
+__global__ __launch_bounds__(kBlock) void k_scan_local(const typename Op::T* __restrict__ in,
+                                                       int64_t n, typename Op::T* __restrict__ out,
+                                                       typename Op::T* __restrict__ sums)
+{
+    typedef typename Op::T T;
+    __shared__ T sh[kBlock];
+    const int64_t b = (int64_t)blockIdx.x * kScanChunk + (int64_t)threadIdx.x * kScanItems;
+    T v[kScanItems];
+    T run = Op::zero();
+    for (int k = 0; k < kScanItems; k++) {
+        v[k] = run;
+        if (b + k < n) run = Op::add(run, in[b + k]);
+    }
+    sh[threadIdx.x] = run;
+    __syncthreads();
+    for (int off = 1; off < kBlock; off <<= 1) {
+        T t = Op::zero();
+        const bool take = threadIdx.x >= off;
+        if (take) t = sh[threadIdx.x - off];
+        __syncthreads();
+        if (take) sh[threadIdx.x] = Op::add(t, sh[threadIdx.x]);
+        __syncthreads();
+    }
+    const T before = threadIdx.x ? sh[threadIdx.x - 1] : Op::zero();
+    for (int k = 0; k < kScanItems; k++)
+        if (b + k < n) out[b + k] = Op::add(before, v[k]);
+    if (threadIdx.x == kBlock - 1) sums[blockIdx.x] = sh[kBlock - 1];
+}
+
+template <class Op>
+__global__ __launch_bounds__(kBlock) void k_scan_sums(typename Op::T* __restrict__ sums,
+                                                      int64_t nb, typename Op::T* __restrict__ total)
+{
+    if (threadIdx.x != 0) return;
+    typename Op::T run = Op::zero();
+    for (int64_t k = 0; k < nb; k++) {
+        const typename Op::T t = sums[k];
+        sums[k] = run;
+        run = Op::add(run, t);
+    }
+    *total = run;
+}
+
+template <class Op>
+__global__ __launch_bounds__(kBlock) void k_scan_add(typename Op::T* __restrict__ out, int64_t n,
+                                                     const typename Op::T* __restrict__ sums)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) out[i] = Op::add(sums[i / kScanChunk], out[i]);
+}
+
+// out must hold n+1 elements and tmp at least blocks+1 elements
+template <class Op>
+static int scan_exclusive(gh_ctx* c, const typename Op::T* in, int64_t n, typename Op::T* out,
+                          typename Op::T* tmp, hipStream_t s)
+{
+    const int64_t nb = (n + kScanChunk - 1) / kScanChunk;
+    if (nb > 0) {
+        hipLaunchKernelGGL(k_scan_local<Op>, dim3((unsigned)nb), dim3(kBlock), 0, s, in, n, out,
+                           tmp);
+        GH_TRY(check_launch(c, "k_scan_local"));
+    }
+    hipLaunchKernelGGL(k_scan_sums<Op>, dim3(1), dim3(kBlock), 0, s, tmp, nb, out + n);
+    GH_TRY(check_launch(c, "k_scan_sums"));
+    if (n > 0) {
+        hipLaunchKernelGGL(k_scan_add<Op>, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, out,
+                           n, tmp);
+        GH_TRY(check_launch(c, "k_scan_add"));
+    }
+    return 0;
+}
+
+// ------------------------------------------------------------------------------
+// 5./6. sparse table, moments, node records
+// ------------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_sparse_level(unsigned char* __restrict__ st,
+                                                         int64_t n, int k)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const unsigned char* prev = st + (int64_t)(k - 1) * n;
+    const int64_t half = (int64_t)1 << (k - 1);
+    unsigned char v = prev[i];
+    if (i + half < n && prev[i + half] < v) v = prev[i + half];
+    st[(int64_t)k * n + i] = v;
+}
+
+// first j >= s with dp[j] < lv1 (dp[n-1] = 0 guarantees one exists for lv1 >= 1)
+__device__ __forceinline__ int64_t next_below(const unsigned char* __restrict__ st, int64_t n,
+                                              int K, int64_t s, int lv1)
+{
+    int64_t p = s;
+    for (int k = K - 1; k >= 0; k--) {
+        if (p + ((int64_t)1 << k) <= n && st[(int64_t)k * n + p] >= lv1) p += (int64_t)1 << k;
+    }
+    return p;
+}
+
+__global__ __launch_bounds__(kBlock) void k_moment_inputs(const double* __restrict__ pos,
+                                                          const double* __restrict__ mass,
+                                                          const u32* __restrict__ perm,
+                                                          int64_t n, DD4* __restrict__ v)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const u32 p = perm[i];
+    const double m = mass[p];
+    DD4 r;
+    r.h[0] = m;
+    r.h[1] = m * pos[3 * p];
+    r.h[2] = m * pos[3 * p + 1];
+    r.h[3] = m * pos[3 * p + 2];
+    for (int k = 0; k < 4; k++) r.l[k] = 0.0;
+    v[i] = r;
+}
+
+__device__ __forceinline__ double dd_diff(const DD4& a, const DD4& b, int k)
+{
+    return (a.h[k] - b.h[k]) + (a.l[k] - b.l[k]);
+}
+
+struct FillArgs {
+    const double* pos;
+    const double* mass;
+    const u32* perm;
+    const unsigned char* dp;
+    const long long* off;  // n+1 entries
+    const unsigned char* st;
+    int K;
+    const double* box;
+    const DD4* pref;  // n+1 entries
+    double theta;
+    NodeD* nodes64;
+    NodeF* nodes32;
+    int* skip;
+    short* level;
+    signed char* branch;
+    int* leaf;  // original particle index, -1 for internal nodes
+};
+
+__device__ void emit_node(const FillArgs& a, int64_t n, int64_t id, int lv, int br, double cx,
+                          double cy, double cz, double size, int64_t s, int64_t e, bool is_leaf,
+                          u32 p)
+{
+    NodeD d;
+    d.cx = cx;
+    d.cy = cy;
+    d.cz = cz;
+    d.size = size;
+    if (is_leaf) {
+        d.mass = a.mass[p];
+        d.mx = a.pos[3 * p];
+        d.my = a.pos[3 * p + 1];
+        d.mz = a.pos[3 * p + 2];
+    } else {
+        const DD4 hi = a.pref[e + 1], lo = a.pref[s];
+        const double m = dd_diff(hi, lo, 0);
+        d.mass = m;
+        d.mx = dd_diff(hi, lo, 1) / m;
+        d.my = dd_diff(hi, lo, 2) / m;
+        d.mz = dd_diff(hi, lo, 3) / m;
+    }
+    a.nodes64[id] = d;
+    NodeF f;
+    float open2;
+    if (is_leaf) {
+        open2 = -1.0f;
+    } else if (a.theta > 0.0) {
+        const double r = size / a.theta;
+        open2 = (float)(r * r);
+    } else {
+        open2 = __int_as_float(0x7f800000);  // +inf: never accepted (size/dist < theta <= 0)
+    }
+    f.geo = make_float4((float)cx, (float)cy, (float)cz, open2);
+    f.mon = make_float4((float)d.mx, (float)d.my, (float)d.mz, (float)d.mass);
+    a.nodes32[id] = f;
+    a.skip[id] = (int)a.off[e + 1];
+    a.level[id] = (short)lv;
+    a.branch[id] = (signed char)br;
+    a.leaf[id] = is_leaf ? (int)p : -1;
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_nodes(FillArgs a, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const u32 p = a.perm[i];
+    const double x = a.pos[3 * p], y = a.pos[3 * p + 1], z = a.pos[3 * p + 2];
+    const int lprev = (i == 0) ? -1 : (int)a.dp[i - 1] - 1;
+    const int lcur = (int)a.dp[i] - 1;
+    const int leaf_lv = (lprev > lcur ? lprev : lcur) + 1;
+    const int64_t base = a.off[i];
+    double cx = a.box[0], cy = a.box[1], cz = a.box[2], size = a.box[3];
+    int br = -1;
+    for (int lv = 0; lv <= leaf_lv; lv++) {
+        if (lv > lprev && lv <= lcur) {
+            const int64_t e = next_below(a.st, n, a.K, i, lv + 1);
+            emit_node(a, n, base + (lv - lprev - 1), lv, br, cx, cy, cz, size, i, e, false, p);
+        }
+        if (lv == leaf_lv) {
+            const int64_t id = base + (lcur > lprev ? lcur - lprev : 0);
+            emit_node(a, n, id, lv, br, cx, cy, cz, size, i, i, true, p);
+            break;
+        }
+        const double h = 0.5 * size;
+        br = descend(x, y, z, cx, cy, cz, h);
+        size = h;
+    }
+}
+
+// ------------------------------------------------------------------------------
+// 7. walks
+// ------------------------------------------------------------------------------
+// FP32 monopoles, FP64 accumulation.  Targets: tgt (AoS f64) in the order given by
+// order[] (nullable), results written to acc[order[k]].
+template <bool GUARD>
+__global__ __launch_bounds__(kBlock) void k_walk_f32(const NodeF* __restrict__ nodes,
+                                                     const int* __restrict__ skip,
+                                                     int64_t n_nodes,
+                                                     const double* __restrict__ tgt,
+                                                     const u32* __restrict__ order, int64_t nt,
+                                                     float eps2, double* __restrict__ acc)
+{
+    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const bool live = k < nt;
+    int64_t t = 0;
+    float x = 0.f, y = 0.f, z = 0.f;
+    if (live) {
+        t = order ? (int64_t)order[k] : k;
+        x = (float)tgt[3 * t];
+        y = (float)tgt[3 * t + 1];
+        z = (float)tgt[3 * t + 2];
+    }
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    int my_next = live ? 0 : 0x7fffffff;
+    int node = 0;
+    while (node < n_nodes) {
+        const NodeF nd = nodes[node];
+        const int sk = skip[node];
+        const bool active = (my_next == node);
+        bool open = false;
+        if (active) {
+            const float cdx = nd.geo.x - x, cdy = nd.geo.y - y, cdz = nd.geo.z - z;
+            const float d2 = __builtin_fmaf(cdx, cdx, __builtin_fmaf(cdy, cdy, cdz * cdz));
+            open = !(d2 > nd.geo.w);
+            if (!open) {
+                const float dx = nd.mon.x - x, dy = nd.mon.y - y, dz = nd.mon.z - z;
+                float r2 = __builtin_fmaf(dx, dx, eps2);
+                r2 = __builtin_fmaf(dy, dy, r2);
+                r2 = __builtin_fmaf(dz, dz, r2);
+                const float ri = __builtin_amdgcn_rsqf(r2);
+                float sc = (nd.mon.w * ri) * (ri * ri);
+                if (GUARD) sc = (r2 > 0.f) ? sc : 0.f;
+                ax += (double)(dx * sc);
+                ay += (double)(dy * sc);
+                az += (double)(dz * sc);
+                my_next = sk;
+            } else {
+                my_next = node + 1;
+            }
+        }
+        const bool any_open = __ballot(active && open) != 0ull;
+        node = __builtin_amdgcn_readfirstlane(any_open ? node + 1 : sk);
+    }
+    if (live) {
+        acc[3 * t] = ax;
+        acc[3 * t + 1] = ay;
+        acc[3 * t + 2] = az;
+    }
+}
+
+// FP64 in the reference's operation order for the opening test and the monopole
+// (gravhopper/_jbgrav.c:496-524), so the accepted node set equals the reference's.
+__global__ __launch_bounds__(kBlock) void k_walk_f64(const NodeD* __restrict__ nodes,
+                                                     const int* __restrict__ skip,
+                                                     const int* __restrict__ leaf,
+                                                     int64_t n_nodes,
+                                                     const double* __restrict__ tgt,
+                                                     const u32* __restrict__ order, int64_t nt,
+                                                     double eps, double theta,
+                                                     double* __restrict__ acc,
+                                                     int* __restrict__ counts)
+{
+    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const bool live = k < nt;
+    int64_t t = 0;
+    double x = 0.0, y = 0.0, z = 0.0;
+    if (live) {
+        t = order ? (int64_t)order[k] : k;
+        x = tgt[3 * t];
+        y = tgt[3 * t + 1];
+        z = tgt[3 * t + 2];
+    }
+    const double eps2 = eps * eps;
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    int cnt = 0;
+    int my_next = live ? 0 : 0x7fffffff;
+    int node = 0;
+    while (node < n_nodes) {
+        const NodeD nd = nodes[node];
+        const int sk = skip[node];
+        const bool is_leaf = leaf[node] >= 0;
+        const bool active = (my_next == node);
+        bool open = false;
+        if (active) {
+            double dist = 0.0;
+            dist = dist + (nd.cx - x) * (nd.cx - x);
+            dist = dist + (nd.cy - y) * (nd.cy - y);
+            dist = dist + (nd.cz - z) * (nd.cz - z);
+            dist = sqrt(dist);
+            open = !(is_leaf || ((nd.size / dist) < theta));
+            if (!open) {
+                const double dx = nd.mx - x, dy = nd.my - y, dz = nd.mz - z;
+                double d2 = dx * dx;
+                d2 = d2 + dy * dy;
+                d2 = d2 + dz * dz;
+                const double d = d2 + eps2;
+                const double inv = (d == 0.0) ? 0.0 : 1.0 / d / sqrt(d);
+                ax = ax + dx * nd.mass * inv;
+                ay = ay + dy * nd.mass * inv;
+                az = az + dz * nd.mass * inv;
+                cnt++;
+                my_next = sk;
+            } else {
+                my_next = node + 1;
+            }
+        }
+        const bool any_open = __ballot(active && open) != 0ull;
+        node = __builtin_amdgcn_readfirstlane(any_open ? node + 1 : sk);
+    }
+    if (live) {
+        acc[3 * t] = ax;
+        acc[3 * t + 1] = ay;
+        acc[3 * t + 2] = az;
+        if (counts) counts[t] = cnt;
+    }
+}
+
+// ------------------------------------------------------------------------------
+// host orchestration
+// ------------------------------------------------------------------------------
+static int radix_sort(gh_ctx* c, TreeBufs* tb, int64_t n, int words, hipStream_t s)
+{
+    const int nblocks = (int)((n + kRsChunk - 1) / kRsChunk);
+    GH_TRY(ensure(c, tb->hist, sizeof(u32) * 256 * (size_t)nblocks));
+    int sel = 0;
+    for (int w = words - 1; w >= 0; w--) {
+        for (int shift = 0; shift < 64; shift += 8) {
+            const u64* kin = tb->keys[sel].as<u64>();
+            hipLaunchKernelGGL(k_rs_hist, dim3(nblocks), dim3(kBlock), 0, s, kin + (int64_t)w * n,
+                               n, shift, tb->hist.as<u32>(), nblocks);
+            GH_TRY(check_launch(c, "k_rs_hist"));
+            hipLaunchKernelGGL(k_rs_scan, dim3(1), dim3(1024), 0, s, tb->hist.as<u32>(),
+                               (int64_t)256 * nblocks);
+            GH_TRY(check_launch(c, "k_rs_scan"));
+            hipLaunchKernelGGL(k_rs_scatter, dim3(nblocks), dim3(kBlock), 0, s, kin,
+                               tb->keys[sel ^ 1].as<u64>(), tb->idx[sel].as<u32>(),
+                               tb->idx[sel ^ 1].as<u32>(), n, words, w, shift,
+                               tb->hist.as<u32>(), nblocks);
+            GH_TRY(check_launch(c, "k_rs_scatter"));
+            sel ^= 1;
+        }
+    }
+    tb->key_sel = sel;
+    return 0;
+}
+
+// Builds the tree of (d_pos, d_mass) into c->tree.  One host synchronisation reads the
+// node count and the key checks.
+static int build_tree(gh_ctx* c, const double* d_pos, const double* d_mass, int64_t n,
+                      double eps, double theta, hipStream_t s)
+{
+    if (!c->tree) c->tree = new TreeBufs();
+    TreeBufs* tb = c->tree;
+    TimedScope ts(c, s, GH_T_TREE_BUILD);
+
+    // 1. box
+    const int nparts = (int)std::min<int64_t>(1024, (n + kBlock - 1) / kBlock);
+    GH_TRY(ensure(c, tb->bbox_part, sizeof(double) * 6 * (size_t)nparts));
+    GH_TRY(ensure(c, tb->box, sizeof(double) * 8));
+    hipLaunchKernelGGL(k_bbox_partial, dim3(nparts), dim3(kBlock), 0, s, d_pos, n,
+                       tb->bbox_part.as<double>());
+    GH_TRY(check_launch(c, "k_bbox_partial"));
+    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, s, tb->bbox_part.as<double>(), nparts,
+                       eps, tb->box.as<double>());
+    GH_TRY(check_launch(c, "k_bbox_final"));
+
+    GH_TRY(ensure(c, tb->idx[0], sizeof(u32) * (size_t)n));
+    GH_TRY(ensure(c, tb->idx[1], sizeof(u32) * (size_t)n));
+    GH_TRY(ensure(c, tb->delta, (size_t)n));
+    GH_TRY(ensure(c, tb->flags, sizeof(int) * 4));
+    int hflags[4] = {0, 0, 0, 0};
+    int words = 1;
+    for (;;) {
+        GH_TRY(ensure(c, tb->keys[0], sizeof(u64) * (size_t)n * words));
+        GH_TRY(ensure(c, tb->keys[1], sizeof(u64) * (size_t)n * words));
+        hipLaunchKernelGGL(k_keys, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, d_pos, n,
+                           tb->box.as<double>(), words, tb->keys[0].as<u64>(),
+                           tb->idx[0].as<u32>());
+        GH_TRY(check_launch(c, "k_keys"));
+        GH_TRY(radix_sort(c, tb, n, words, s));
+        GH_HIP(c, hipMemcpyAsync(tb->flags.p, hflags, sizeof(hflags), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_delta, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s,
+                           tb->keys[tb->key_sel].as<u64>(), n, words,
+                           tb->idx[tb->key_sel].as<u32>(), d_pos, tb->delta.as<unsigned char>(),
+                           tb->flags.as<int>());
+        GH_TRY(check_launch(c, "k_delta"));
+        int got[4];
+        GH_HIP(c, hipMemcpyAsync(got, tb->flags.p, sizeof(got), hipMemcpyDeviceToHost, s));
+        GH_HIP(c, hipStreamSynchronize(s));
+        if (got[1] > 0)
+            return fail(c, GH_ECOINCIDENT,
+                        "%d pair(s) of particles share a position; the octree cannot "
+                        "separate them", got[1]);
+        if (got[0] == 0) {
+            c->last_tree_depth = got[2];
+            break;
+        }
+        if (++words > kMaxWords)
+            return fail(c, GH_EDEPTH,
+                        "particles closer than %d octree levels below the root box",
+                        kMaxWords * kLevelsPerWord);
+    }
+    tb->words = words;
+    const u32* perm = tb->idx[tb->key_sel].as<u32>();
+
+    // 5. numbering
+    const int64_t nb = (n + kScanChunk - 1) / kScanChunk;
+    GH_TRY(ensure(c, tb->off, sizeof(long long) * (size_t)(2 * n + 2)));
+    GH_TRY(ensure(c, tb->block_sums, sizeof(DD4) * (size_t)(nb + 1)));
+    long long* cnt = tb->off.as<long long>() + (n + 1);
+    long long* off = tb->off.as<long long>();
+    hipLaunchKernelGGL(k_node_counts, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s,
+                       tb->delta.as<unsigned char>(), n, cnt);
+    GH_TRY(check_launch(c, "k_node_counts"));
+    GH_TRY(scan_exclusive<OpI64>(c, cnt, n, off, tb->block_sums.as<long long>(), s));
+    long long total = 0;
+    GH_HIP(c, hipMemcpyAsync(&total, off + n, sizeof(total), hipMemcpyDeviceToHost, s));
+
+    // sparse table over dp (levels of shared prefix)
+    int K = 1;  // windows of up to 2^(K-1) cover any distance < n <= 2^K
+    while (((int64_t)1 << K) < n) K++;
+    GH_TRY(ensure(c, tb->sparse, (size_t)n * K));
+    GH_HIP(c, hipMemcpyAsync(tb->sparse.p, tb->delta.p, (size_t)n, hipMemcpyDeviceToDevice, s));
+    for (int k = 1; k < K; k++) {
+        hipLaunchKernelGGL(k_sparse_level, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s,
+                           tb->sparse.as<unsigned char>(), n, k);
+        GH_TRY(check_launch(c, "k_sparse_level"));
+    }
+
+    // 6. moments: exclusive double-double prefix sums of (m, m x, m y, m z)
+    GH_TRY(ensure(c, tb->prefix, sizeof(DD4) * (size_t)(2 * n + 1)));
+    DD4* vin = tb->prefix.as<DD4>() + (n + 1);
+    DD4* pref = tb->prefix.as<DD4>();
+    hipLaunchKernelGGL(k_moment_inputs, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, d_pos,
+                       d_mass, perm, n, vin);
+    GH_TRY(check_launch(c, "k_moment_inputs"));
+    GH_TRY(scan_exclusive<OpDD4>(c, vin, n, pref, tb->block_sums.as<DD4>(), s));
+
+    GH_HIP(c, hipStreamSynchronize(s));
+    if (total <= 0 || total >= ((long long)1 << 31))
+        return fail(c, GH_EDEPTH, "octree of %lld nodes is out of range", total);
+    tb->n_nodes = total;
+    const size_t T = (size_t)total;
+    GH_TRY(ensure(c, tb->nodes64, sizeof(NodeD) * T));
+    GH_TRY(ensure(c, tb->nodes32, sizeof(NodeF) * T));
+    GH_TRY(ensure(c, tb->skip, sizeof(int) * T));
+    GH_TRY(ensure(c, tb->level, sizeof(short) * T));
+    GH_TRY(ensure(c, tb->branch, T));
+    GH_TRY(ensure(c, tb->leaf, sizeof(int) * T));
+
+    FillArgs fa;
+    fa.pos = d_pos;
+    fa.mass = d_mass;
+    fa.perm = perm;
+    fa.dp = tb->sparse.as<unsigned char>();
+    fa.off = off;
+    fa.st = tb->sparse.as<unsigned char>();
+    fa.K = K;
+    fa.box = tb->box.as<double>();
+    fa.pref = pref;
+    fa.theta = theta;
+    fa.nodes64 = tb->nodes64.as<NodeD>();
+    fa.nodes32 = tb->nodes32.as<NodeF>();
+    fa.skip = tb->skip.as<int>();
+    fa.level = tb->level.as<short>();
+    fa.branch = tb->branch.as<signed char>();
+    fa.leaf = tb->leaf.as<int>();
+    hipLaunchKernelGGL(k_fill_nodes, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, fa, n);
+    GH_TRY(check_launch(c, "k_fill_nodes"));
+    c->last_tree_nodes = total;
+    return 0;
+}
+
+static int walk(gh_ctx* c, const double* tgt, const u32* order, int64_t nt, double eps,
+                double theta, int precision, double* d_acc, int* counts, hipStream_t s)
+{
+    TreeBufs* tb = c->tree;
+    if (nt <= 0) return 0;
+    TimedScope ts(c, s, GH_T_TREE_WALK);
+    if (precision == GH_FP64) {
+        hipLaunchKernelGGL(k_walk_f64, dim3(blocks_for(nt, kBlock)), dim3(kBlock), 0, s,
+                           tb->nodes64.as<NodeD>(), tb->skip.as<int>(), tb->leaf.as<int>(),
+                           tb->n_nodes, tgt, order, nt, eps, theta, d_acc, counts);
+        return check_launch(c, "k_walk_f64");
+    }
+    const float eps2 = (float)(eps * eps);
+    const bool guard = !(eps2 >= 1e-30f);
+    if (guard)
+        hipLaunchKernelGGL(k_walk_f32<true>, dim3(blocks_for(nt, kBlock)), dim3(kBlock), 0, s,
+                           tb->nodes32.as<NodeF>(), tb->skip.as<int>(), tb->n_nodes, tgt, order,
+                           nt, eps2, d_acc);
+    else
+        hipLaunchKernelGGL(k_walk_f32<false>, dim3(blocks_for(nt, kBlock)), dim3(kBlock), 0, s,
+                           tb->nodes32.as<NodeF>(), tb->skip.as<int>(), tb->n_nodes, tgt, order,
+                           nt, eps2, d_acc);
+    return check_launch(c, "k_walk_f32");
+}
+
+int tree_dev(gh_ctx* c, const double* d_pos, const double* d_mass, int64_t np,
+             const double* d_fpos, int64_t nf, double eps, double theta, int precision,
+             double* d_acc, hipStream_t s)
+{
+    const bool self = (d_fpos == nullptr);
+    const int64_t nt = self ? np : nf;
+    if (nt == 0) return 0;
+    if (np == 0) return zero_dev(c, d_acc, 3 * nt, s);
+    if (precision != GH_FP64 && precision != GH_FP32)
+        return fail(c, GH_EARG, "unknown precision %d", precision);
+    GH_TRY(build_tree(c, d_pos, d_mass, np, eps, theta, s));
+    // self targets are walked in sorted (octant-path) order: neighbouring lanes take
+    // neighbouring particles, so a wavefront's paths through the tree nearly coincide
+    const u32* order = self ? c->tree->idx[c->tree->key_sel].as<u32>() : nullptr;
+    return walk(c, self ? d_pos : d_fpos, order, nt, eps, theta, precision, d_acc, nullptr, s);
 }
 
 void tree_free(gh_ctx* c)
 {
-    delete c->tree;
+    if (!c->tree) return;
+    TreeBufs* tb = c->tree;
+    DevBuf* bufs[] = {&tb->keys[0], &tb->keys[1], &tb->idx[0],    &tb->idx[1],  &tb->hist,
+                      &tb->bbox_part, &tb->box,   &tb->flags,     &tb->delta,   &tb->off,
+                      &tb->sparse,  &tb->prefix,  &tb->block_sums, &tb->nodes64, &tb->nodes32,
+                      &tb->skip,    &tb->level,   &tb->branch,    &tb->leaf,    &tb->tgt_order};
+    for (DevBuf* b : bufs)
+        if (b->p) (void)hipFree(b->p);
+    delete tb;
     c->tree = nullptr;
 }
 
 }  // namespace gh
 
-extern "C" int gh_tree_dump(gh_ctx* c, const double*, const double*, int64_t, double,
-                            gh_node_rec*, int64_t, int64_t*)
+using namespace gh;
+
+extern "C" int gh_tree_dump(gh_ctx* c, const double* pos, const double* mass, int64_t np,
+                            double eps, gh_node_rec* out, int64_t cap, int64_t* n_nodes)
 {
-    return gh::fail(c, GH_EARG, "tree dump not available in this build");
+    if (!c) return GH_EARG;
+    if (np < 1) return fail(c, GH_EARG, "the tree needs at least one particle");
+    GH_HIP(c, hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    GH_TRY(ensure(c, c->pos64, sizeof(double) * 3 * (size_t)np));
+    GH_TRY(ensure(c, c->mass64, sizeof(double) * (size_t)np));
+    GH_HIP(c, hipMemcpyAsync(c->pos64.p, pos, sizeof(double) * 3 * (size_t)np,
+                             hipMemcpyHostToDevice, s));
+    GH_HIP(c, hipMemcpyAsync(c->mass64.p, mass, sizeof(double) * (size_t)np,
+                             hipMemcpyHostToDevice, s));
+    GH_TRY(build_tree(c, c->pos64.as<double>(), c->mass64.as<double>(), np, eps, 0.7, s));
+    TreeBufs* tb = c->tree;
+    const int64_t T = tb->n_nodes;
+    if (n_nodes) *n_nodes = T;
+    const int64_t m = cap < T ? cap : T;
+    if (m <= 0 || !out) return 0;
+    std::vector<NodeD> nd(m);
+    std::vector<short> lv(m);
+    std::vector<signed char> br(m);
+    std::vector<int> lf(m);
+    GH_HIP(c, hipMemcpyAsync(nd.data(), tb->nodes64.p, sizeof(NodeD) * m, hipMemcpyDeviceToHost, s));
+    GH_HIP(c, hipMemcpyAsync(lv.data(), tb->level.p, sizeof(short) * m, hipMemcpyDeviceToHost, s));
+    GH_HIP(c, hipMemcpyAsync(br.data(), tb->branch.p, (size_t)m, hipMemcpyDeviceToHost, s));
+    GH_HIP(c, hipMemcpyAsync(lf.data(), tb->leaf.p, sizeof(int) * m, hipMemcpyDeviceToHost, s));
+    GH_HIP(c, hipStreamSynchronize(s));
+    for (int64_t k = 0; k < m; k++) {
+        gh_node_rec& r = out[k];
+        r.level = lv[k];
+        r.branch = br[k];
+        r.leaf = lf[k];
+        r.center[0] = nd[k].cx;
+        r.center[1] = nd[k].cy;
+        r.center[2] = nd[k].cz;
+        r.size = nd[k].size;
+        r.mass = nd[k].mass;
+        r.com[0] = nd[k].mx;
+        r.com[1] = nd[k].my;
+        r.com[2] = nd[k].mz;
+    }
+    return 0;
+}
+
+// Accepted-node counts of the FP64 walk, for comparing the walk with the reference's.
+extern "C" int gh_tree_interactions(gh_ctx* c, const double* pos, const double* mass,
+                                    int64_t np, const double* force_pos, int64_t nf, double eps,
+                                    double theta, double* acc, int64_t* counts)
+{
+    if (!c) return GH_EARG;
+    if (np < 1) return fail(c, GH_EARG, "the tree needs at least one particle");
+    const bool self = force_pos == nullptr;
+    const int64_t nt = self ? np : nf;
+    GH_HIP(c, hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    GH_TRY(ensure(c, c->pos64, sizeof(double) * 3 * (size_t)np));
+    GH_TRY(ensure(c, c->mass64, sizeof(double) * (size_t)np));
+    GH_TRY(ensure(c, c->acc64, sizeof(double) * 3 * (size_t)(nt > 0 ? nt : 1)));
+    GH_TRY(ensure(c, c->scratch, sizeof(int) * (size_t)(nt > 0 ? nt : 1)));
+    GH_HIP(c, hipMemcpyAsync(c->pos64.p, pos, sizeof(double) * 3 * (size_t)np,
+                             hipMemcpyHostToDevice, s));
+    GH_HIP(c, hipMemcpyAsync(c->mass64.p, mass, sizeof(double) * (size_t)np,
+                             hipMemcpyHostToDevice, s));
+    const double* tgt = c->pos64.as<double>();
+    if (!self) {
+        GH_TRY(ensure(c, c->fpos64, sizeof(double) * 3 * (size_t)(nt > 0 ? nt : 1)));
+        GH_HIP(c, hipMemcpyAsync(c->fpos64.p, force_pos, sizeof(double) * 3 * (size_t)nt,
+                                 hipMemcpyHostToDevice, s));
+        tgt = c->fpos64.as<double>();
+    }
+    GH_TRY(build_tree(c, c->pos64.as<double>(), c->mass64.as<double>(), np, eps, theta, s));
+    const u32* order = self ? c->tree->idx[c->tree->key_sel].as<u32>() : nullptr;
+    GH_TRY(walk(c, tgt, order, nt, eps, theta, GH_FP64, c->acc64.as<double>(),
+                c->scratch.as<int>(), s));
+    std::vector<int> cnt((size_t)(nt > 0 ? nt : 1));
+    GH_HIP(c, hipMemcpyAsync(acc, c->acc64.p, sizeof(double) * 3 * (size_t)nt,
+                             hipMemcpyDeviceToHost, s));
+    GH_HIP(c, hipMemcpyAsync(cnt.data(), c->scratch.p, sizeof(int) * (size_t)nt,
+                             hipMemcpyDeviceToHost, s));
+    GH_HIP(c, hipStreamSynchronize(s));
+    for (int64_t k = 0; k < nt; k++) counts[k] = cnt[k];
+    return 0;
 }

@@ -108,6 +108,14 @@ typedef struct {
 int gh_tree_dump(gh_ctx *ctx, const double *pos, const double *mass, int64_t np,
                  double eps, gh_node_rec *out, int64_t cap, int64_t *n_nodes);
 
+/* FP64 tree force together with the number of nodes each target accepts (leaves
+ * included): the walk's interaction count, comparable with the reference walk
+ * gravoct_calc_accel (gravhopper/_jbgrav.c:487-541).  force_pos == NULL: targets are
+ * the particles.  acc (nt,3), counts (nt). */
+int gh_tree_interactions(gh_ctx *ctx, const double *pos, const double *mass, int64_t np,
+                         const double *force_pos, int64_t nf, double eps, double theta,
+                         double *acc, int64_t *counts);
+
 /* Device-resident drift-kick-drift leapfrog: nsteps of Simulation.perform_timestep
  * (gravhopper/gravhopper.py:405-416) with the force of calculate_acceleration
  * (:419-459) and no external forces, in code units (kpc, km/s, Msun, Myr):
