@@ -61,23 +61,97 @@ def cpu_baseline(budget_s=30.0):
     except (AttributeError, OSError):
         pass
     ref = O.ref_module()
-    n = 8192 if ref is not None else 16384
+    n = 16384 if ref is not None else 32768
     ic = workloads.plummer(n, b=1e-3, mass=1e6, seed=SEED_DIRECT)
-    t0 = time.perf_counter()
+    times = []
+    spent = 0.0
+    while len(times) < 3 and spent < budget_s:
+        t0 = time.perf_counter()
+        if ref is not None:
+            ref.direct_summation(ic["pos"], ic["mass"], EPS_DIRECT)
+        else:
+            O.direct(ic["pos"], ic["mass"], EPS_DIRECT, nthreads=1)
+        times.append(time.perf_counter() - t0)
+        spent += times[-1]
     if ref is not None:
-        ref.direct_summation(ic["pos"], ic["mass"], EPS_DIRECT)
         kind = "reference"
         what = "gravhopper/_jbgrav.c direct_summation (built from the reference source)"
     else:
-        O.direct(ic["pos"], ic["mass"], EPS_DIRECT, nthreads=1)
         kind = "port"
         what = "oracle/jbgrav_oracle.c orc_direct (restatement, same FP64 op order)"
-    dt = time.perf_counter() - t0
+    dt = float(np.median(times))
     return {"value": n * (n - 1) / dt, "unit": "pair interactions/s", "cores": 1,
             "kind": kind,
-            "sample": "%s, Plummer N=%d, one FP64 force evaluation (%.1f s); the reference "
-                      "cannot run N=262144 (N^2 scratch of 2.2 TB, 32-bit index overflow "
-                      "above N=26754)" % (what, n, dt)}
+            "sample": "%s, Plummer N=%d, median of %d FP64 force evaluations (%.1f s each, "
+                      "%.1f s total); the reference cannot run N=262144 (N^2 scratch of "
+                      "2.2 TB, 32-bit index overflow above N=26754)" % (
+                          what, n, len(times), dt, spent)}
+
+
+def bench_tree(ctx, torch, steps, warmup):
+    """Barnes-Hut step at N = 1048576: Hernquist halo + exponential disk, theta = 0.7,
+    eps = 0.05 kpc, dt = 0.1 Myr, FP32 monopoles, FP64 state."""
+    from gravhopper_amd import _native, workloads
+    g = workloads.galaxy(seed=3)
+    n = len(g["mass"])
+    eps, dt, theta = 0.05, 0.1, 0.7
+    dev = torch.device("cuda", ctx.device)
+    x = torch.tensor(g["pos"], dtype=torch.float64, device=dev).contiguous()
+    v = torch.tensor(g["vel"], dtype=torch.float64, device=dev).contiguous()
+    m = torch.tensor(g["mass"], dtype=torch.float64, device=dev).contiguous()
+    FP32, FP64, TREE = _native.GH_FP32, _native.GH_FP64, _native.GH_TREE
+    ctx.run_dev(x.data_ptr(), v.data_ptr(), m.data_ptr(), n, dt, eps, warmup,
+                algorithm=TREE, theta=theta, precision=FP32)
+    ctx.sync()
+    torch.cuda.synchronize()
+    ctx.timing_enable(True)
+    t0 = time.perf_counter()
+    ctx.run_dev(x.data_ptr(), v.data_ptr(), m.data_ptr(), n, dt, eps, steps,
+                algorithm=TREE, theta=theta, precision=FP32)
+    ctx.sync()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    b_ms, b_n = ctx.timing_read(_native.T_TREE_BUILD)
+    w_ms, w_n = ctx.timing_read(_native.T_TREE_WALK)
+    u_ms, u_n = ctx.timing_read(_native.T_UPDATE)
+    ctx.timing_enable(False)
+    out = {"workload": "Hernquist halo 524288 (a=10 kpc, 1e12 Msun) + exp disk 524288 "
+                       "(Rd=3 kpc, z0=0.3 kpc, 5e10 Msun), theta=0.7, eps=0.05 kpc, "
+                       "dt=0.1 Myr, FP32 monopoles, FP64 state",
+           "n_particles": n, "steps": steps, "steps_per_s": steps / el,
+           "ms_per_step": el / steps * 1e3,
+           "build_ms": b_ms / max(b_n, 1), "walk_ms": w_ms / max(w_n, 1),
+           "update_ms": u_ms / max(u_n, 1), "nodes": None}
+    # force accuracy on a 16384-particle subsample against the FP64 direct sum
+    pos = x.cpu().numpy()
+    mass = g["mass"]
+    rng = np.random.default_rng(16384)
+    sub = rng.choice(n, 16384, replace=False)
+    acc_t = torch.empty((n, 3), dtype=torch.float64, device=dev)
+    ctx.tree_dev(x.data_ptr(), m.data_ptr(), n, eps, theta, FP32, acc_t.data_ptr())
+    tsub = torch.tensor(pos[sub], dtype=torch.float64, device=dev).contiguous()
+    acc_d = torch.empty((len(sub), 3), dtype=torch.float64, device=dev)
+    ctx.direct_dev(x.data_ptr(), m.data_ptr(), n, eps, FP64, acc_d.data_ptr(),
+                   d_force_pos=tsub.data_ptr(), nf=len(sub))
+    ctx.sync()
+    at = acc_t.cpu().numpy()[sub]
+    ad = acc_d.cpu().numpy()
+    e = np.linalg.norm(at - ad, axis=1) / np.linalg.norm(ad, axis=1)
+    out["force_rel_err_median"] = float(np.median(e))
+    out["force_rel_err_p99"] = float(np.percentile(e, 99))
+    # interaction count of the reference walk (FP64 walk, same accepted nodes)
+    _, cnt = ctx.tree_interactions(pos, mass, eps, theta)
+    out["interactions_per_target"] = float(cnt.mean())
+    out["walk_achieved_tflops"] = 20.0 * cnt.sum() / (out["walk_ms"] * 1e-3) / 1e12
+    # the FP32 direct sum at the same N, one evaluation
+    ctx.timing_enable(True)
+    ctx.direct_dev(x.data_ptr(), m.data_ptr(), n, eps, FP32, acc_t.data_ptr())
+    ctx.sync()
+    d_ms, _ = ctx.timing_read(_native.T_DIRECT_F32)
+    ctx.timing_enable(False)
+    out["direct_fp32_ms_same_n"] = d_ms
+    out["tree_speedup_over_direct"] = d_ms / out["ms_per_step"]
+    return out
 
 
 def main():
@@ -88,6 +162,8 @@ def main():
     ap.add_argument("--n", type=int, default=N_DIRECT)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--no-tree", action="store_true")
+    ap.add_argument("--tree-steps", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this engine runs one process on one MI355X "
@@ -182,6 +258,9 @@ def main():
             "energy_rel_drift": abs(e1 - e0) / abs(e0),
             "energy_drift_steps": args.warmup + args.steps,
         })
+    if not args.no_extras and not args.no_tree:
+        log("bench: tree step at N=1M")
+        extras["tree"] = bench_tree(ctx, torch, args.tree_steps, 1)
     result["extras"] = extras
     if not args.no_cpu_baseline:
         log("bench: cpu baseline")

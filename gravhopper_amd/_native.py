@@ -47,9 +47,13 @@ SIGNATURES = {
     "gh_run": ([_P, _P, _P, _P, _I64, _D, _D, _I, _D, _I, _I64, _P, _P], _I),
     "gh_run_dev": ([_P, _P, _P, _P, _I64, _D, _D, _I, _D, _I, _I64, _P, _P, _P], _I),
     "gh_timing_enable": ([_P, _I], _I),
+    "gh_set_tuning": ([_P, _I, _I], _I),
     "gh_timing_read": ([_P, _I, ctypes.POINTER(_D), ctypes.POINTER(_I64)], _I),
     "gh_energy": ([_P, _P, _P, _P, _I64, _D, _P, _P, _P], _I),
 }
+
+# gh_set_tuning keys
+TUNE_DIRECT_SELF, TUNE_DIRECT_IPT = 0, 1
 
 # kernel classes of gh_timing_read
 T_DIRECT_F32, T_DIRECT_F64, T_TREE_BUILD, T_TREE_WALK, T_UPDATE = range(5)
@@ -228,6 +232,9 @@ class Context:
             self.handle, ctypes.c_void_p(d_pos), ctypes.c_void_p(d_mass), int(n),
             ctypes.c_void_p(d_force_pos or None), int(nf), float(eps), float(theta),
             int(precision), ctypes.c_void_p(d_acc), ctypes.c_void_p(stream or None)))
+
+    def set_tuning(self, key, value):
+        self.check(self.lib.gh_set_tuning(self.handle, int(key), int(value)))
 
     def timing_enable(self, on=True):
         self.check(self.lib.gh_timing_enable(self.handle, 1 if on else 0))

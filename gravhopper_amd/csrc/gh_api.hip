@@ -201,7 +201,8 @@ void gh_ctx_destroy(gh_ctx* c)
     tree_free(c);
     DevBuf* bufs[] = {&c->pos64, &c->mass64, &c->fpos64, &c->acc64, &c->src64,
                       &c->src32a, &c->src32b, &c->tgt32,  &c->xs,    &c->vs,
-                      &c->xh,     &c->hist_p, &c->hist_v, &c->scratch, &c->counters};
+                      &c->xh,     &c->hist_p, &c->hist_v, &c->scratch, &c->counters,
+                      &c->work};
     for (DevBuf* b : bufs) free_buf(*b);
     release_spans(c);
     for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
@@ -359,6 +360,14 @@ int gh_run(gh_ctx* c, double* pos, double* vel, const double* mass, int64_t np, 
     GH_HIP(c, hipMemcpyAsync(pos, x, sizeof(double) * (size_t)n3, hipMemcpyDeviceToHost, s));
     GH_HIP(c, hipMemcpyAsync(vel, v, sizeof(double) * (size_t)n3, hipMemcpyDeviceToHost, s));
     GH_HIP(c, hipStreamSynchronize(s));
+    return 0;
+}
+
+int gh_set_tuning(gh_ctx* c, int key, int value)
+{
+    if (!c) return GH_EARG;
+    if (key < 0 || key >= GH_TUNE_COUNT) return fail(c, GH_EARG, "unknown tuning key %d", key);
+    c->tune[key] = value;
     return 0;
 }
 

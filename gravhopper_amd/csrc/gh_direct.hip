@@ -195,6 +195,149 @@ __global__ __launch_bounds__(kBlock) void k_direct_f32(const float4* __restrict_
     }
 }
 
+// ------------------------------------------------------------------------------
+// FP32 pairs, each unordered pair computed once (self field only)
+// ------------------------------------------------------------------------------
+// Particles are cut into blocks of BLK = 256*IPT.  A workgroup takes one block pair
+// (I, J), I <= J.  Its 4 wavefronts hold the 256*IPT i-particles of block I in
+// registers (IPT per lane); block J is staged in LDS.  For I < J, block J is walked in
+// chunks of 64: at step s lane l pairs its i-particles with j = (l + s) mod 64 of the
+// chunk and accumulates both the force on its i-particles and the force on that j in
+// a register that moves to lane l-1 after every step (DPP wavefront rotation), so
+// after 64 steps lane l holds the whole wavefront's force on j = l.  The four
+// wavefronts' j-partials are added in LDS.  A diagonal pair I = J is done one-sided.
+//
+// Results are partial fields part[K][p] (FP32, 3 per particle): particle p of block B
+// receives one partial from every block K (K = B for its diagonal); k_reduce_sym adds
+// them in block order in FP64.
+constexpr int kRotFromNext = 0x134;  // DPP wave_rol:1 -- lane l reads lane l+1 (mod 64)
+
+__device__ __forceinline__ float rot_from_next(float v)
+{
+    // every lane has a source lane, so no "old" value is needed
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), kRotFromNext, 0xf, 0xf,
+                                                   false));
+}
+
+template <int IPT, bool GUARD>
+__global__ __launch_bounds__(kBlock) void k_direct_sym_f32(const float4* __restrict__ src,
+                                                           int64_t np_pad,
+                                                           const int2* __restrict__ work,
+                                                           float eps2,
+                                                           float* __restrict__ part)
+{
+    constexpr int BLK = kBlock * IPT;
+    constexpr int NCH = BLK / 64;
+    // each 64-particle chunk of block J is stored twice in a row, so that lane l's
+    // rotating index l + s needs no wrap-around
+    __shared__ float4 tile[NCH * 128];
+    __shared__ float jp[kBlock / 64][3][64];
+    const int I = work[blockIdx.x].x, J = work[blockIdx.x].y;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t ibase = (int64_t)I * BLK + wave * 64 * IPT + lane;
+    const int64_t jbase = (int64_t)J * BLK;
+
+    float xi[IPT], yi[IPT], zi[IPT], mi[IPT], ax[IPT], ay[IPT], az[IPT];
+#pragma unroll
+    for (int k = 0; k < IPT; k++) {
+        const float4 t = src[ibase + k * 64];
+        xi[k] = t.x;
+        yi[k] = t.y;
+        zi[k] = t.z;
+        mi[k] = t.w;
+        ax[k] = ay[k] = az[k] = 0.f;
+    }
+    for (int q = threadIdx.x; q < NCH * 128; q += kBlock)
+        tile[q] = src[jbase + (q >> 7) * 64 + (q & 63)];
+    __syncthreads();
+
+    if (I == J) {
+        for (int c = 0; c < NCH; c++) {
+#pragma unroll 8
+            for (int q = 0; q < 64; q++) {
+                const float4 s = tile[c * 128 + q];
+#pragma unroll
+                for (int k = 0; k < IPT; k++) {
+                    const float dx = s.x - xi[k], dy = s.y - yi[k], dz = s.z - zi[k];
+                    float r2 = __builtin_fmaf(dx, dx, eps2);
+                    r2 = __builtin_fmaf(dy, dy, r2);
+                    r2 = __builtin_fmaf(dz, dz, r2);
+                    const float ri = __builtin_amdgcn_rsqf(r2);
+                    float sc = (s.w * ri) * (ri * ri);
+                    if (GUARD) sc = (r2 > 0.f) ? sc : 0.f;
+                    ax[k] = __builtin_fmaf(dx, sc, ax[k]);
+                    ay[k] = __builtin_fmaf(dy, sc, ay[k]);
+                    az[k] = __builtin_fmaf(dz, sc, az[k]);
+                }
+            }
+        }
+    } else {
+        for (int c = 0; c < NCH; c++) {
+            const float4* row = &tile[c * 128 + lane];
+            float fx = 0.f, fy = 0.f, fz = 0.f;
+            for (int s0 = 0; s0 < 64; s0 += 8) {
+#pragma unroll
+                for (int s = 0; s < 8; s++) {
+                    const float4 sj = row[s0 + s];
+#pragma unroll
+                    for (int k = 0; k < IPT; k++) {
+                        const float dx = sj.x - xi[k], dy = sj.y - yi[k], dz = sj.z - zi[k];
+                        float r2 = __builtin_fmaf(dx, dx, eps2);
+                        r2 = __builtin_fmaf(dy, dy, r2);
+                        r2 = __builtin_fmaf(dz, dz, r2);
+                        const float ri = __builtin_amdgcn_rsqf(r2);
+                        float ri3 = ri * ri * ri;
+                        if (GUARD) ri3 = (r2 > 0.f) ? ri3 : 0.f;
+                        const float si = sj.w * ri3;
+                        ax[k] = __builtin_fmaf(dx, si, ax[k]);
+                        ay[k] = __builtin_fmaf(dy, si, ay[k]);
+                        az[k] = __builtin_fmaf(dz, si, az[k]);
+                        const float sm = mi[k] * ri3;
+                        fx = __builtin_fmaf(dx, sm, fx);
+                        fy = __builtin_fmaf(dy, sm, fy);
+                        fz = __builtin_fmaf(dz, sm, fz);
+                    }
+                    fx = rot_from_next(fx);
+                    fy = rot_from_next(fy);
+                    fz = rot_from_next(fz);
+                }
+            }
+            // lane l now holds this wavefront's pull of its i-particles by j = l; the
+            // force on j is the opposite
+            jp[wave][0][lane] = fx;
+            jp[wave][1][lane] = fy;
+            jp[wave][2][lane] = fz;
+            __syncthreads();
+            if (threadIdx.x < 3 * 64) {
+                const int comp = threadIdx.x >> 6, t = threadIdx.x & 63;
+                float sacc = jp[0][comp][t];
+#pragma unroll
+                for (int w = 1; w < kBlock / 64; w++) sacc += jp[w][comp][t];
+                part[((int64_t)I * np_pad + jbase + c * 64 + t) * 3 + comp] = -sacc;
+            }
+            __syncthreads();
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < IPT; k++) {
+        float* o = part + ((int64_t)J * np_pad + ibase + k * 64) * 3;
+        o[0] = ax[k];
+        o[1] = ay[k];
+        o[2] = az[k];
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_reduce_sym(const float* __restrict__ part, int nb,
+                                                       int64_t np_pad, int64_t n3,
+                                                       double* __restrict__ acc)
+{
+    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (k >= n3) return;
+    double s = 0.0;
+    for (int b = 0; b < nb; b++) s += (double)part[(int64_t)b * np_pad * 3 + k];
+    acc[k] = s;
+}
+
 // Sum of per-split partial fields, in split order.
 __global__ __launch_bounds__(kBlock) void k_reduce_splits(const double* __restrict__ part,
                                                           int splits, int64_t n3,
@@ -260,6 +403,73 @@ void launch_f32(bool guard, dim3 grid, hipStream_t s, const float4* src, int64_t
 
 }  // namespace
 
+// Symmetric (pair-once) kernel for the self field: particles per lane, or 0 for the
+// one-sided kernel.  Block pairs must number a few thousand to fill 256 CUs.
+int sym_ipt(gh_ctx* c, int64_t np)
+{
+    const int forced = c->tune[GH_TUNE_DIRECT_SELF];
+    if (forced < 0) return 0;
+    if (forced == 1 || forced == 2 || forced == 4) return forced;
+    if (np >= 65536) return 4;
+    if (np >= 32768) return 2;
+    if (np >= 16384) return 1;
+    return 0;
+}
+
+template <int IPT>
+static void launch_sym(bool guard, unsigned items, hipStream_t s, const float4* src,
+                       int64_t np_pad, const int2* work, float eps2, float* part)
+{
+    if (guard)
+        hipLaunchKernelGGL((k_direct_sym_f32<IPT, true>), dim3(items), dim3(kBlock), 0, s, src,
+                           np_pad, work, eps2, part);
+    else
+        hipLaunchKernelGGL((k_direct_sym_f32<IPT, false>), dim3(items), dim3(kBlock), 0, s, src,
+                           np_pad, work, eps2, part);
+}
+
+int direct_sym(gh_ctx* c, const double* d_pos, const double* d_mass, int64_t np, int ipt,
+               double eps, double* d_acc, hipStream_t s)
+{
+    const int64_t blk = (int64_t)kBlock * ipt;
+    const int64_t np_pad = pad_to(np, blk);
+    const int nb = (int)(np_pad / blk);
+    const int64_t items = (int64_t)nb * (nb + 1) / 2;
+    if (items > 0x7fffffff) return fail(c, GH_EARG, "too many particles for one launch");
+    GH_TRY(ensure(c, c->src32a, sizeof(float4) * (size_t)np_pad));
+    GH_TRY(ensure(c, c->scratch, sizeof(float) * 3 * (size_t)np_pad * nb));
+    if (c->sym_nb != nb) {
+        // block pairs (I, J), I <= J, ordered by I
+        std::vector<int2> w((size_t)items);
+        size_t k = 0;
+        for (int I = 0; I < nb; I++)
+            for (int J = I; J < nb; J++) w[k++] = make_int2(I, J);
+        GH_TRY(ensure(c, c->work, sizeof(int2) * (size_t)items));
+        GH_HIP(c, hipMemcpyAsync(c->work.p, w.data(), sizeof(int2) * (size_t)items,
+                                 hipMemcpyHostToDevice, s));
+        GH_HIP(c, hipStreamSynchronize(s));
+        c->sym_nb = nb;
+    }
+    float4* src = c->src32a.as<float4>();
+    GH_TRY(pack_src32(c, d_pos, d_mass, np, src, np_pad, s));
+    const float eps2 = (float)(eps * eps);
+    const bool guard = !(eps2 >= 1e-30f);
+    float* part = c->scratch.as<float>();
+    {
+        TimedScope ts(c, s, GH_T_DIRECT_F32);
+        switch (ipt) {
+            case 1: launch_sym<1>(guard, (unsigned)items, s, src, np_pad, c->work.as<int2>(), eps2, part); break;
+            case 2: launch_sym<2>(guard, (unsigned)items, s, src, np_pad, c->work.as<int2>(), eps2, part); break;
+            case 4: launch_sym<4>(guard, (unsigned)items, s, src, np_pad, c->work.as<int2>(), eps2, part); break;
+            default: return fail(c, GH_EARG, "unsupported particles-per-lane %d", ipt);
+        }
+        GH_TRY(check_launch(c, "k_direct_sym_f32"));
+    }
+    hipLaunchKernelGGL(k_reduce_sym, dim3(blocks_for(3 * np, kBlock)), dim3(kBlock), 0, s, part,
+                       nb, np_pad, 3 * np, d_acc);
+    return check_launch(c, "k_reduce_sym");
+}
+
 int direct_f32_packed(gh_ctx* c, const float4* src, int64_t np_pad, const float4* tgt,
                       int64_t nt, bool self, double eps, double* d_acc, hipStream_t s)
 {
@@ -271,7 +481,8 @@ int direct_f32_packed(gh_ctx* c, const float4* src, int64_t np_pad, const float4
     // produce 0 * inf; guard it to 0 as the reference does for targets (:327-330).
     const bool guard = !(eps2 >= 1e-30f);
     F32Shape sh = pick_shape(c, nt, np_pad);
-    if (c->force_variant > 0) sh.ipt = c->force_variant;
+    const int forced = c->tune[GH_TUNE_DIRECT_IPT];
+    if (forced == 1 || forced == 2 || forced == 4) sh.ipt = forced;
     int64_t jps = pad_to((np_pad + sh.splits - 1) / sh.splits, kBlock);
     const int splits = (int)((np_pad + jps - 1) / jps);
     double* out = d_acc;
@@ -325,6 +536,8 @@ int direct_dev(gh_ctx* c, const double* d_pos, const double* d_mass, int64_t np,
         return check_launch(c, "k_direct_f64");
     }
     if (precision != GH_FP32) return fail(c, GH_EARG, "unknown precision %d", precision);
+    const int sym = self ? sym_ipt(c, np) : 0;
+    if (sym > 0) return direct_sym(c, d_pos, d_mass, np, sym, eps, d_acc, s);
     const int64_t np_pad = pad_to(np, kBlock);
     GH_TRY(ensure(c, c->src32a, sizeof(float4) * (size_t)np_pad));
     GH_TRY(ensure(c, c->tgt32, sizeof(float4) * (size_t)nt));

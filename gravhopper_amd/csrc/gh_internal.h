@@ -31,11 +31,6 @@ struct DevBuf {
 
 struct TreeBufs;  // gh_tree.hip
 
-struct Tuning {
-    int64_t n_lo = -1, n_hi = -1;  // bucket of N the choice was made for
-    int variant = -1;
-};
-
 }  // namespace gh
 
 struct gh_ctx {
@@ -46,8 +41,9 @@ struct gh_ctx {
     gh::DevBuf pos64, mass64, fpos64, acc64, src64, src32a, src32b, tgt32;
     gh::DevBuf xs, vs, xh, hist_p, hist_v, scratch, counters;
     gh::TreeBufs* tree = nullptr;
-    gh::Tuning tune_self, tune_pos;
-    int force_variant = -1;  // -1: autotune; otherwise a fixed FP32 direct variant
+    int tune[GH_TUNE_COUNT] = {0};  // gh_set_tuning values, 0 = automatic
+    gh::DevBuf work;                // block-pair list of the symmetric direct kernel
+    int sym_nb = -1;
     int64_t last_tree_nodes = 0;
     int last_tree_depth = 0;
     // kernel timing (gh_timing_enable / gh_timing_read)
@@ -108,6 +104,9 @@ int direct_dev(gh_ctx* c, const double* d_pos, const double* d_mass, int64_t np,
 int pack_src32(gh_ctx* c, const double* d_pos, const double* d_mass, int64_t np,
                float4* out, int64_t np_pad, hipStream_t s);
 int64_t pad_to(int64_t n, int64_t m);
+int sym_ipt(gh_ctx* c, int64_t np);
+int direct_sym(gh_ctx* c, const double* d_pos, const double* d_mass, int64_t np, int ipt,
+               double eps, double* d_acc, hipStream_t s);
 int direct_f32_packed(gh_ctx* c, const float4* src, int64_t np_pad, const float4* tgt,
                       int64_t nt, bool self, double eps, double* d_acc, hipStream_t s);
 

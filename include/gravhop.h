@@ -146,6 +146,16 @@ int gh_run_dev(gh_ctx *ctx, double *d_pos, double *d_vel, const double *d_mass, 
 int gh_timing_enable(gh_ctx *ctx, int on);
 int gh_timing_read(gh_ctx *ctx, int kernel_class, double *total_ms, int64_t *launches);
 
+/* Kernel-shape tuning (no reference counterpart).  Keys and values:
+ *   GH_TUNE_DIRECT_SELF  FP32 self field: 0 automatic, -1 one-sided kernel,
+ *                        1/2/4 pair-once kernel with that many particles per lane
+ *   GH_TUNE_DIRECT_IPT   FP32 one-sided kernel: 0 automatic, 1/2/4 targets per lane
+ * Results do not depend on the values beyond FP32 rounding. */
+#define GH_TUNE_DIRECT_SELF 0
+#define GH_TUNE_DIRECT_IPT 1
+#define GH_TUNE_COUNT 2
+int gh_set_tuning(gh_ctx *ctx, int key, int value);
+
 /* FP64 diagnostics (no reference counterpart): kinetic energy sum 0.5 m v^2
  * [Msun (km/s)^2], potential energy -sum_{i<j} m_i m_j / sqrt(r^2 + eps^2)
  * [Msun^2/kpc, G = 1] and momentum sum m v [Msun km/s] (3 doubles). */

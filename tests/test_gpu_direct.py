@@ -110,3 +110,41 @@ def test_energy_matches_oracle(ctx, oracle):
     assert abs(ke - ke0) <= 1e-12 * abs(ke0)
     assert abs(pe - pe0) <= 1e-12 * abs(pe0)
     assert np.allclose(p, p0, atol=1e-9 * np.abs(ic["mass"][:, None] * ic["vel"]).sum())
+
+
+@pytest.mark.parametrize("ipt", [-1, 1, 2, 4])
+@pytest.mark.parametrize("n,eps", [(5000, 1e-5), (20001, 1e-5), (3000, 0.0)])
+def test_fp32_self_kernels(ctx, oracle, ipt, n, eps):
+    """Every FP32 self-field kernel (one-sided, and pair-once with 1/2/4 particles per
+    lane) against the FP64 oracle, including padding (n not a block multiple) and
+    eps = 0 (zero-separation guard)."""
+    ic = workloads.plummer(n, b=1e-3, mass=1e6, seed=n + 7)
+    ref = oracle.direct(ic["pos"], ic["mass"], eps)
+    ctx.set_tuning(_native.TUNE_DIRECT_SELF, ipt)
+    try:
+        got = ctx.direct(ic["pos"], ic["mass"], eps, FP32)
+    finally:
+        ctx.set_tuning(_native.TUNE_DIRECT_SELF, 0)
+    assert np.all(np.isfinite(got))
+    e = rel_err(got, ref)
+    print("ipt=%d n=%d eps=%g median %.3g p99 %.3g" % (ipt, n, eps, np.median(e),
+                                                      np.percentile(e, 99)))
+    assert np.median(e) <= FP32_MEDIAN
+    assert np.percentile(e, 99) <= FP32_P99
+    # momentum: the pair-once kernels give exactly opposite pair forces
+    if ipt > 0:
+        tot = (ic["mass"][:, None] * got).sum(0)
+        assert np.all(np.abs(tot) <= 1e-5 * (ic["mass"][:, None] * np.abs(got)).sum(0))
+
+
+def test_fp32_self_kernels_agree_unequal_masses(ctx):
+    rng = np.random.default_rng(55)
+    pos = rng.normal(size=(17000, 3))
+    mass = rng.lognormal(size=17000)
+    out = {}
+    for ipt in (-1, 1, 4):
+        ctx.set_tuning(_native.TUNE_DIRECT_SELF, ipt)
+        out[ipt] = ctx.direct(pos, mass, 0.01, FP32)
+    ctx.set_tuning(_native.TUNE_DIRECT_SELF, 0)
+    for ipt in (1, 4):
+        assert np.median(rel_err(out[ipt], out[-1])) < 1e-6
