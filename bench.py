@@ -88,7 +88,7 @@ def cpu_baseline(budget_s=30.0):
                           what, n, len(times), dt, spent)}
 
 
-def bench_tree(ctx, torch, steps, warmup):
+def bench_tree(ctx, torch, steps, warmup, lite=False):
     """Barnes-Hut step at N = 1048576: Hernquist halo + exponential disk, theta = 0.7,
     eps = 0.05 kpc, dt = 0.1 Myr, FP32 monopoles, FP64 state."""
     from gravhopper_amd import _native, workloads
@@ -121,7 +121,9 @@ def bench_tree(ctx, torch, steps, warmup):
            "n_particles": n, "steps": steps, "steps_per_s": steps / el,
            "ms_per_step": el / steps * 1e3,
            "build_ms": b_ms / max(b_n, 1), "walk_ms": w_ms / max(w_n, 1),
-           "update_ms": u_ms / max(u_n, 1), "nodes": None}
+           "update_ms": u_ms / max(u_n, 1)}
+    if lite:
+        return out
     # force accuracy on a 16384-particle subsample against the FP64 direct sum
     pos = x.cpu().numpy()
     mass = g["mass"]
@@ -164,6 +166,13 @@ def main():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-tree", action="store_true")
     ap.add_argument("--tree-steps", type=int, default=5)
+    ap.add_argument("--only-tree", action="store_true",
+                    help="run only the N=1M tree steps (for profiling)")
+    ap.add_argument("--tree-lite", action="store_true",
+                    help="tree steps without the accuracy and comparison runs")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="first copy this many MiB device-to-device (a known-size kernel "
+                         "for calibrating HBM traffic counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this engine runs one process on one MI355X "
@@ -174,6 +183,19 @@ def main():
     from gravhopper_amd import _native, workloads
 
     ctx = _native.default_context()
+    if args.calibrate_copy_mib > 0:
+        nbytes = args.calibrate_copy_mib << 20
+        src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("bench: calibration copy of %d MiB" % args.calibrate_copy_mib)
+        del src, dst
+    if args.only_tree:
+        print(json.dumps({"tree": bench_tree(ctx, torch, args.tree_steps, 1,
+                                             lite=args.tree_lite)}), flush=True)
+        return
     n = args.n
     ic = workloads.plummer(n, b=1e-3, mass=1e6, seed=SEED_DIRECT)
     dev = torch.device("cuda", ctx.device)

@@ -110,20 +110,41 @@ __global__ __launch_bounds__(kBlock) void k_bbox_partial(const double* __restric
 }
 
 // box[0..2] = centre, box[3] = size (gravhopper/_jbgrav.c:764-772)
-__global__ void k_bbox_final(const double* __restrict__ part, int nparts,
-                             double eps, double* __restrict__ box)
+__global__ __launch_bounds__(kBlock) void k_bbox_final(const double* __restrict__ part,
+                                                       int nparts, double eps,
+                                                       double* __restrict__ box)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    __shared__ double smin[3][kBlock], smax[3][kBlock];
     double mn[3], mx[3];
     for (int k = 0; k < 3; k++) {
         mn[k] = part[k];
         mx[k] = part[3 + k];
     }
-    for (int b = 1; b < nparts; b++)
+    for (int b = threadIdx.x; b < nparts; b += kBlock)
         for (int k = 0; k < 3; k++) {
             if (part[6 * b + k] < mn[k]) mn[k] = part[6 * b + k];
             if (part[6 * b + 3 + k] > mx[k]) mx[k] = part[6 * b + 3 + k];
         }
+    for (int k = 0; k < 3; k++) {
+        smin[k][threadIdx.x] = mn[k];
+        smax[k][threadIdx.x] = mx[k];
+    }
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+            for (int k = 0; k < 3; k++) {
+                if (smin[k][threadIdx.x + w] < smin[k][threadIdx.x])
+                    smin[k][threadIdx.x] = smin[k][threadIdx.x + w];
+                if (smax[k][threadIdx.x + w] > smax[k][threadIdx.x])
+                    smax[k][threadIdx.x] = smax[k][threadIdx.x + w];
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    for (int k = 0; k < 3; k++) {
+        mn[k] = smin[k][0];
+        mx[k] = smax[k][0];
+    }
     double bs = mx[0] - mn[0] + eps;
     for (int k = 1; k < 3; k++)
         if ((mx[k] - mn[k]) > bs) bs = mx[k] - mn[k] + eps;
@@ -187,41 +208,56 @@ __global__ __launch_bounds__(kBlock) void k_rs_hist(const u64* __restrict__ key,
     hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
 }
 
-// exclusive scan of len u32 values by one workgroup of 1024 threads
-__global__ __launch_bounds__(1024) void k_rs_scan(u32* __restrict__ v, int64_t len)
+// Per digit d (one workgroup each): exclusive prefix of hist[d][0..nblocks) in place,
+// and the digit's total count in totals[d].
+__global__ __launch_bounds__(kBlock) void k_rs_scan(u32* __restrict__ hist, int nblocks,
+                                                    u32* __restrict__ totals)
 {
-    __shared__ u32 part[1024];
-    const int64_t per = (len + 1023) / 1024;
-    const int64_t b = (int64_t)threadIdx.x * per;
-    const int64_t e = min(len, b + per);
-    u32 s = 0;
-    for (int64_t k = b; k < e; k++) s += v[k];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const u32 t = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+    __shared__ u32 sh[kBlock];
+    u32* row = hist + (int64_t)blockIdx.x * nblocks;
+    u32 carry = 0;
+    for (int b0 = 0; b0 < nblocks; b0 += kBlock) {
+        const int b = b0 + threadIdx.x;
+        const u32 v = b < nblocks ? row[b] : 0u;
+        sh[threadIdx.x] = v;
         __syncthreads();
-        part[threadIdx.x] += t;
+        for (int off = 1; off < kBlock; off <<= 1) {
+            const u32 t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0u;
+            __syncthreads();
+            sh[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (b < nblocks) row[b] = carry + sh[threadIdx.x] - v;
+        carry += sh[kBlock - 1];
         __syncthreads();
     }
-    u32 run = threadIdx.x ? part[threadIdx.x - 1] : 0u;
-    for (int64_t k = b; k < e; k++) {
-        const u32 t = v[k];
-        v[k] = run;
-        run += t;
-    }
+    if (threadIdx.x == 0) totals[blockIdx.x] = carry;
 }
 
 __global__ __launch_bounds__(kBlock) void k_rs_scatter(
     const u64* __restrict__ kin, u64* __restrict__ kout, const u32* __restrict__ iin,
     u32* __restrict__ iout, int64_t n, int words, int wsel, int shift,
-    const u32* __restrict__ hist, int nblocks)
+    const u32* __restrict__ hist, int nblocks, const u32* __restrict__ totals)
 {
     __shared__ u32 gbase[256];
     __shared__ u32 wcount[kBlock / 64][256];
     __shared__ u32 woff[kBlock / 64][256];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    gbase[threadIdx.x] = hist[(int64_t)threadIdx.x * nblocks + blockIdx.x];
+    // digit bases: exclusive scan of the 256 digit totals
+    {
+        const u32 tot = totals[threadIdx.x];
+        gbase[threadIdx.x] = tot;
+        __syncthreads();
+        for (int off = 1; off < 256; off <<= 1) {
+            const u32 t = threadIdx.x >= off ? gbase[threadIdx.x - off] : 0u;
+            __syncthreads();
+            gbase[threadIdx.x] += t;
+            __syncthreads();
+        }
+        const u32 base = gbase[threadIdx.x] - tot;
+        __syncthreads();
+        gbase[threadIdx.x] = base + hist[(int64_t)threadIdx.x * nblocks + blockIdx.x];
+    }
     for (int w = 0; w < kBlock / 64; w++) wcount[w][threadIdx.x] = 0;
     __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * kRsChunk;
@@ -271,29 +307,34 @@ __global__ __launch_bounds__(kBlock) void k_delta(const u64* __restrict__ keys, 
                                                   int* __restrict__ flags)
 {
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    if (i == n - 1) {
-        dp[i] = 0;
-        return;
-    }
-    int delta = words * kLevelsPerWord;
-    for (int w = 0; w < words; w++) {
-        const u64 a = keys[(int64_t)w * n + i], b = keys[(int64_t)w * n + i + 1];
-        if (a != b) {
-            const int lz = __clzll(a ^ b);  // >= 1: bit 63 is never used
-            delta = w * kLevelsPerWord + (lz - 1) / 3;
-            break;
+    int m = 0;  // this lane's delta + 1; 0 for the sentinel and for lanes past the end
+    if (i < n - 1) {
+        int delta = words * kLevelsPerWord;
+        for (int w = 0; w < words; w++) {
+            const u64 a = keys[(int64_t)w * n + i], b = keys[(int64_t)w * n + i + 1];
+            if (a != b) {
+                const int lz = __clzll(a ^ b);  // >= 1: bit 63 is never used
+                delta = w * kLevelsPerWord + (lz - 1) / 3;
+                break;
+            }
         }
+        if (delta == words * kLevelsPerWord) {
+            atomicAdd(&flags[0], 1);
+            const u32 p = perm[i], q = perm[i + 1];
+            if (pos[3 * p] == pos[3 * q] && pos[3 * p + 1] == pos[3 * q + 1] &&
+                pos[3 * p + 2] == pos[3 * q + 2])
+                atomicAdd(&flags[1], 1);
+        }
+        m = delta + 1;
     }
-    if (delta == words * kLevelsPerWord) {
-        atomicAdd(&flags[0], 1);
-        const u32 p = perm[i], q = perm[i + 1];
-        if (pos[3 * p] == pos[3 * q] && pos[3 * p + 1] == pos[3 * q + 1] &&
-            pos[3 * p + 2] == pos[3 * q + 2])
-            atomicAdd(&flags[1], 1);
+    if (i < n) dp[i] = (unsigned char)m;
+    // deepest level: one atomic per wavefront (all lanes take part in the shuffles)
+    int mx = m;
+    for (int off = 32; off > 0; off >>= 1) {
+        const int o = __shfl_xor(mx, off);
+        mx = o > mx ? o : mx;
     }
-    dp[i] = (unsigned char)(delta + 1);
-    atomicMax(&flags[2], delta + 1);
+    if ((threadIdx.x & 63) == 0) atomicMax(&flags[2], mx);
 }
 
 // cnt[i] = internal nodes starting at sorted position i, plus its leaf
@@ -375,18 +416,36 @@ __global__ __launch_bounds__(kBlock) void k_scan_local(const typename Op::T* __r
     if (threadIdx.x == kBlock - 1) sums[blockIdx.x] = sh[kBlock - 1];
 }
 
+// Exclusive scan of the nb block totals by one workgroup (each thread a contiguous
+// run), total written to *total.
 template <class Op>
 __global__ __launch_bounds__(kBlock) void k_scan_sums(typename Op::T* __restrict__ sums,
                                                       int64_t nb, typename Op::T* __restrict__ total)
 {
-    if (threadIdx.x != 0) return;
-    typename Op::T run = Op::zero();
-    for (int64_t k = 0; k < nb; k++) {
-        const typename Op::T t = sums[k];
-        sums[k] = run;
-        run = Op::add(run, t);
+    typedef typename Op::T T;
+    __shared__ T sh[kBlock];
+    const int64_t per = (nb + kBlock - 1) / kBlock;
+    const int64_t b = (int64_t)threadIdx.x * per;
+    const int64_t e = b + per < nb ? b + per : nb;
+    T run = Op::zero();
+    for (int64_t k = b; k < e; k++) run = Op::add(run, sums[k]);
+    sh[threadIdx.x] = run;
+    __syncthreads();
+    for (int off = 1; off < kBlock; off <<= 1) {
+        T t = Op::zero();
+        const bool take = threadIdx.x >= off;
+        if (take) t = sh[threadIdx.x - off];
+        __syncthreads();
+        if (take) sh[threadIdx.x] = Op::add(t, sh[threadIdx.x]);
+        __syncthreads();
     }
-    *total = run;
+    T acc = threadIdx.x ? sh[threadIdx.x - 1] : Op::zero();
+    for (int64_t k = b; k < e; k++) {
+        const T t = sums[k];
+        sums[k] = acc;
+        acc = Op::add(acc, t);
+    }
+    if (threadIdx.x == kBlock - 1) *total = sh[kBlock - 1];
 }
 
 template <class Op>
@@ -484,7 +543,12 @@ struct FillArgs {
     short* level;
     signed char* branch;
     int* leaf;  // original particle index, -1 for internal nodes
+    int what;   // kFill* bits: which records this build needs
 };
+
+constexpr int kFillF64 = 1;   // NodeD + leaf flags (FP64 walk)
+constexpr int kFillF32 = 2;   // NodeF (FP32 walk)
+constexpr int kFillDump = 4;  // levels and branches (canonical dump)
 
 __device__ void emit_node(const FillArgs& a, int64_t n, int64_t id, int lv, int br, double cx,
                           double cy, double cz, double size, int64_t s, int64_t e, bool is_leaf,
@@ -508,7 +572,16 @@ __device__ void emit_node(const FillArgs& a, int64_t n, int64_t id, int lv, int 
         d.my = dd_diff(hi, lo, 2) / m;
         d.mz = dd_diff(hi, lo, 3) / m;
     }
-    a.nodes64[id] = d;
+    a.skip[id] = (int)a.off[e + 1];
+    if (a.what & (kFillF64 | kFillDump)) {
+        a.nodes64[id] = d;
+        a.leaf[id] = is_leaf ? (int)p : -1;
+    }
+    if (a.what & kFillDump) {
+        a.level[id] = (short)lv;
+        a.branch[id] = (signed char)br;
+    }
+    if (!(a.what & kFillF32)) return;
     NodeF f;
     float open2;
     if (is_leaf) {
@@ -522,10 +595,6 @@ __device__ void emit_node(const FillArgs& a, int64_t n, int64_t id, int lv, int 
     f.geo = make_float4((float)cx, (float)cy, (float)cz, open2);
     f.mon = make_float4((float)d.mx, (float)d.my, (float)d.mz, (float)d.mass);
     a.nodes32[id] = f;
-    a.skip[id] = (int)a.off[e + 1];
-    a.level[id] = (short)lv;
-    a.branch[id] = (signed char)br;
-    a.leaf[id] = is_leaf ? (int)p : -1;
 }
 
 __global__ __launch_bounds__(kBlock) void k_fill_nodes(FillArgs a, int64_t n)
@@ -690,7 +759,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_f64(const NodeD* __restrict__ n
 static int radix_sort(gh_ctx* c, TreeBufs* tb, int64_t n, int words, hipStream_t s)
 {
     const int nblocks = (int)((n + kRsChunk - 1) / kRsChunk);
-    GH_TRY(ensure(c, tb->hist, sizeof(u32) * 256 * (size_t)nblocks));
+    GH_TRY(ensure(c, tb->hist, sizeof(u32) * 256 * (size_t)(nblocks + 1)));
     int sel = 0;
     for (int w = words - 1; w >= 0; w--) {
         for (int shift = 0; shift < 64; shift += 8) {
@@ -698,13 +767,14 @@ static int radix_sort(gh_ctx* c, TreeBufs* tb, int64_t n, int words, hipStream_t
             hipLaunchKernelGGL(k_rs_hist, dim3(nblocks), dim3(kBlock), 0, s, kin + (int64_t)w * n,
                                n, shift, tb->hist.as<u32>(), nblocks);
             GH_TRY(check_launch(c, "k_rs_hist"));
-            hipLaunchKernelGGL(k_rs_scan, dim3(1), dim3(1024), 0, s, tb->hist.as<u32>(),
-                               (int64_t)256 * nblocks);
+            hipLaunchKernelGGL(k_rs_scan, dim3(256), dim3(kBlock), 0, s, tb->hist.as<u32>(),
+                               nblocks, tb->hist.as<u32>() + (int64_t)256 * nblocks);
             GH_TRY(check_launch(c, "k_rs_scan"));
             hipLaunchKernelGGL(k_rs_scatter, dim3(nblocks), dim3(kBlock), 0, s, kin,
                                tb->keys[sel ^ 1].as<u64>(), tb->idx[sel].as<u32>(),
                                tb->idx[sel ^ 1].as<u32>(), n, words, w, shift,
-                               tb->hist.as<u32>(), nblocks);
+                               tb->hist.as<u32>(), nblocks,
+                               tb->hist.as<u32>() + (int64_t)256 * nblocks);
             GH_TRY(check_launch(c, "k_rs_scatter"));
             sel ^= 1;
         }
@@ -716,7 +786,7 @@ static int radix_sort(gh_ctx* c, TreeBufs* tb, int64_t n, int words, hipStream_t
 // Builds the tree of (d_pos, d_mass) into c->tree.  One host synchronisation reads the
 // node count and the key checks.
 static int build_tree(gh_ctx* c, const double* d_pos, const double* d_mass, int64_t n,
-                      double eps, double theta, hipStream_t s)
+                      double eps, double theta, int what, hipStream_t s)
 {
     if (!c->tree) c->tree = new TreeBufs();
     TreeBufs* tb = c->tree;
@@ -729,7 +799,7 @@ static int build_tree(gh_ctx* c, const double* d_pos, const double* d_mass, int6
     hipLaunchKernelGGL(k_bbox_partial, dim3(nparts), dim3(kBlock), 0, s, d_pos, n,
                        tb->bbox_part.as<double>());
     GH_TRY(check_launch(c, "k_bbox_partial"));
-    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, s, tb->bbox_part.as<double>(), nparts,
+    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(kBlock), 0, s, tb->bbox_part.as<double>(), nparts,
                        eps, tb->box.as<double>());
     GH_TRY(check_launch(c, "k_bbox_final"));
 
@@ -810,12 +880,16 @@ static int build_tree(gh_ctx* c, const double* d_pos, const double* d_mass, int6
         return fail(c, GH_EDEPTH, "octree of %lld nodes is out of range", total);
     tb->n_nodes = total;
     const size_t T = (size_t)total;
-    GH_TRY(ensure(c, tb->nodes64, sizeof(NodeD) * T));
-    GH_TRY(ensure(c, tb->nodes32, sizeof(NodeF) * T));
     GH_TRY(ensure(c, tb->skip, sizeof(int) * T));
-    GH_TRY(ensure(c, tb->level, sizeof(short) * T));
-    GH_TRY(ensure(c, tb->branch, T));
-    GH_TRY(ensure(c, tb->leaf, sizeof(int) * T));
+    if (what & (kFillF64 | kFillDump)) {
+        GH_TRY(ensure(c, tb->nodes64, sizeof(NodeD) * T));
+        GH_TRY(ensure(c, tb->leaf, sizeof(int) * T));
+    }
+    if (what & kFillF32) GH_TRY(ensure(c, tb->nodes32, sizeof(NodeF) * T));
+    if (what & kFillDump) {
+        GH_TRY(ensure(c, tb->level, sizeof(short) * T));
+        GH_TRY(ensure(c, tb->branch, T));
+    }
 
     FillArgs fa;
     fa.pos = d_pos;
@@ -834,6 +908,7 @@ static int build_tree(gh_ctx* c, const double* d_pos, const double* d_mass, int6
     fa.level = tb->level.as<short>();
     fa.branch = tb->branch.as<signed char>();
     fa.leaf = tb->leaf.as<int>();
+    fa.what = what;
     hipLaunchKernelGGL(k_fill_nodes, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, fa, n);
     GH_TRY(check_launch(c, "k_fill_nodes"));
     c->last_tree_nodes = total;
@@ -875,7 +950,8 @@ int tree_dev(gh_ctx* c, const double* d_pos, const double* d_mass, int64_t np,
     if (np == 0) return zero_dev(c, d_acc, 3 * nt, s);
     if (precision != GH_FP64 && precision != GH_FP32)
         return fail(c, GH_EARG, "unknown precision %d", precision);
-    GH_TRY(build_tree(c, d_pos, d_mass, np, eps, theta, s));
+    GH_TRY(build_tree(c, d_pos, d_mass, np, eps, theta,
+                      precision == GH_FP64 ? kFillF64 : kFillF32, s));
     // self targets are walked in sorted (octant-path) order: neighbouring lanes take
     // neighbouring particles, so a wavefront's paths through the tree nearly coincide
     const u32* order = self ? c->tree->idx[c->tree->key_sel].as<u32>() : nullptr;
@@ -913,7 +989,8 @@ extern "C" int gh_tree_dump(gh_ctx* c, const double* pos, const double* mass, in
                              hipMemcpyHostToDevice, s));
     GH_HIP(c, hipMemcpyAsync(c->mass64.p, mass, sizeof(double) * (size_t)np,
                              hipMemcpyHostToDevice, s));
-    GH_TRY(build_tree(c, c->pos64.as<double>(), c->mass64.as<double>(), np, eps, 0.7, s));
+    GH_TRY(build_tree(c, c->pos64.as<double>(), c->mass64.as<double>(), np, eps, 0.7,
+                      kFillF64 | kFillDump, s));
     TreeBufs* tb = c->tree;
     const int64_t T = tb->n_nodes;
     if (n_nodes) *n_nodes = T;
@@ -971,7 +1048,8 @@ extern "C" int gh_tree_interactions(gh_ctx* c, const double* pos, const double* 
                                  hipMemcpyHostToDevice, s));
         tgt = c->fpos64.as<double>();
     }
-    GH_TRY(build_tree(c, c->pos64.as<double>(), c->mass64.as<double>(), np, eps, theta, s));
+    GH_TRY(build_tree(c, c->pos64.as<double>(), c->mass64.as<double>(), np, eps, theta,
+                      kFillF64, s));
     const u32* order = self ? c->tree->idx[c->tree->key_sel].as<u32>() : nullptr;
     GH_TRY(walk(c, tgt, order, nt, eps, theta, GH_FP64, c->acc64.as<double>(),
                 c->scratch.as<int>(), s));
