@@ -156,6 +156,54 @@ def bench_tree(ctx, torch, steps, warmup, lite=False):
     return out
 
 
+def bench_ensemble(ctx, torch, steps):
+    """128 independent Plummer spheres, N log-uniform in [1024, 65536], b in
+    [0.5, 2] pc, M in [1e5, 1e7] Msun, eps = 0.02 b, dt = 0.005 t_cr; FP32 pairs in one
+    launch per step (graph-replayed), against each simulation stepped on its own."""
+    from gravhopper_amd import _native, autotune, workloads
+    sims = workloads.ensemble(seed=4)
+    ipt, persistent = autotune.tune_ensemble(ctx, sims, use_cache=False)
+    ens = _native.Ensemble(ctx, sims, ipt=ipt, persistent=persistent)
+    info = ens.info()
+    ens.run(1)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ens.run(steps, graph=True)
+    ctx.sync()
+    el = time.perf_counter() - t0
+    ens.close()
+    pairs = info["pairs_per_step"]
+    out = {"workload": "128 Plummer spheres, N log-uniform [1024, 65536], eps=0.02 b, "
+                       "dt=0.005 t_cr, FP32 pairs, FP64 state",
+           "n_sims": len(sims), "n_particles": int(sum(len(s["mass"]) for s in sims)),
+           "pairs_per_step": pairs, "ipt": ipt, "persistent": bool(persistent),
+           "steps": steps, "ms_per_step": el / steps * 1e3,
+           "interactions_per_s": pairs * steps / el}
+    # each simulation alone: device-resident, one step at a time
+    per = []
+    dev = torch.device("cuda", ctx.device)
+    for s in sims:
+        n = len(s["mass"])
+        x = torch.tensor(s["pos"], dtype=torch.float64, device=dev)
+        v = torch.tensor(s["vel"], dtype=torch.float64, device=dev)
+        m = torch.tensor(s["mass"], dtype=torch.float64, device=dev)
+        ctx.run_dev(x.data_ptr(), v.data_ptr(), m.data_ptr(), n, s["dt"], s["eps"], 1,
+                    precision=_native.GH_FP32)
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.run_dev(x.data_ptr(), v.data_ptr(), m.data_ptr(), n, s["dt"], s["eps"], 2,
+                    precision=_native.GH_FP32)
+        ctx.sync()
+        per.append((time.perf_counter() - t0) / 2)
+    per = np.array(per)
+    out["sequential_ms_per_step_total"] = float(per.sum() * 1e3)
+    out["sequential_interactions_per_s"] = pairs / float(per.sum())
+    out["sequential_sim_step_ms_p50"] = float(np.percentile(per, 50) * 1e3)
+    out["sequential_sim_step_ms_p99"] = float(np.percentile(per, 99) * 1e3)
+    out["speedup_over_sequential"] = float(per.sum()) / (el / steps)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -166,6 +214,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-tree", action="store_true")
     ap.add_argument("--tree-steps", type=int, default=5)
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--ensemble-steps", type=int, default=5)
     ap.add_argument("--only-tree", action="store_true",
                     help="run only the N=1M tree steps (for profiling)")
     ap.add_argument("--tree-lite", action="store_true",
@@ -283,6 +333,9 @@ def main():
     if not args.no_extras and not args.no_tree:
         log("bench: tree step at N=1M")
         extras["tree"] = bench_tree(ctx, torch, args.tree_steps, 1)
+    if not args.no_extras and not args.no_ensemble:
+        log("bench: ensemble of 128 spheres")
+        extras["ensemble"] = bench_ensemble(ctx, torch, args.ensemble_steps)
     result["extras"] = extras
     if not args.no_cpu_baseline:
         log("bench: cpu baseline")

@@ -48,6 +48,12 @@ SIGNATURES = {
     "gh_run_dev": ([_P, _P, _P, _P, _I64, _D, _D, _I, _D, _I, _I64, _P, _P, _P], _I),
     "gh_timing_enable": ([_P, _I], _I),
     "gh_set_tuning": ([_P, _I, _I], _I),
+    "gh_ensemble_create": ([_P, _I, _P, _P, _P, _P, _P, _P, _I, _I, ctypes.POINTER(_P)], _I),
+    "gh_ensemble_run": ([_P, _I64, _I], _I),
+    "gh_ensemble_get": ([_P, _P, _P], _I),
+    "gh_ensemble_info": ([_P, ctypes.POINTER(_I64), ctypes.POINTER(_I64),
+                          ctypes.POINTER(_I64)], _I),
+    "gh_ensemble_destroy": ([_P], None),
     "gh_timing_read": ([_P, _I, ctypes.POINTER(_D), ctypes.POINTER(_I64)], _I),
     "gh_energy": ([_P, _P, _P, _P, _I64, _D, _P, _P, _P], _I),
 }
@@ -258,6 +264,58 @@ class Context:
                                       len(mass), float(eps), ctypes.byref(ke),
                                       ctypes.byref(pe), _ptr(mom)))
         return ke.value, pe.value, mom
+
+
+class Ensemble:
+    """gh_ensemble: independent direct-sum simulations advanced together.
+
+    sims: list of dicts with 'pos' (n,3), 'vel' (n,3), 'mass' (n,), 'eps', 'dt' in
+    code units (kpc, km/s, Msun, Myr)."""
+
+    def __init__(self, ctx, sims, ipt=4, persistent=False):
+        self.ctx = ctx
+        self.counts = np.array([len(s["mass"]) for s in sims], dtype=np.int64)
+        pos = np.ascontiguousarray(np.concatenate([s["pos"] for s in sims]), dtype=np.float64)
+        vel = np.ascontiguousarray(np.concatenate([s["vel"] for s in sims]), dtype=np.float64)
+        mass = np.ascontiguousarray(np.concatenate([s["mass"] for s in sims]), dtype=np.float64)
+        eps = np.array([s["eps"] for s in sims], dtype=np.float64)
+        dt = np.array([s["dt"] for s in sims], dtype=np.float64)
+        h = ctypes.c_void_p()
+        ctx.check(ctx.lib.gh_ensemble_create(ctx.handle, len(sims), _ptr(self.counts), _ptr(pos),
+                                             _ptr(vel), _ptr(mass), _ptr(eps), _ptr(dt),
+                                             int(ipt), 1 if persistent else 0,
+                                             ctypes.byref(h)))
+        self.handle = h
+        self.ipt = ipt
+        self.persistent = persistent
+
+    def run(self, nsteps, graph=True):
+        self.ctx.check(self.ctx.lib.gh_ensemble_run(self.handle, int(nsteps), 1 if graph else 0))
+
+    def state(self):
+        n = int(self.counts.sum())
+        pos = np.empty((n, 3))
+        vel = np.empty((n, 3))
+        self.ctx.check(self.ctx.lib.gh_ensemble_get(self.handle, _ptr(pos), _ptr(vel)))
+        splits = np.cumsum(self.counts)[:-1]
+        return np.split(pos, splits), np.split(vel, splits)
+
+    def info(self):
+        a, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        self.ctx.check(self.ctx.lib.gh_ensemble_info(self.handle, ctypes.byref(a),
+                                                     ctypes.byref(b), ctypes.byref(c)))
+        return {"n_padded": a.value, "block_pairs": b.value, "pairs_per_step": c.value}
+
+    def close(self):
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            self.ctx.lib.gh_ensemble_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 _default = None

@@ -219,20 +219,17 @@ __device__ __forceinline__ float rot_from_next(float v)
                                                    false));
 }
 
+// One block pair.  Work item {I, J, slot_i, slot_j}: global block indices I <= J (of
+// one simulation), the partial slot the i-side field goes to (J's index within its
+// simulation) and the slot of the j-side field (I's index within its simulation).
 template <int IPT, bool GUARD>
-__global__ __launch_bounds__(kBlock) void k_direct_sym_f32(const float4* __restrict__ src,
-                                                           int64_t np_pad,
-                                                           const int2* __restrict__ work,
-                                                           float eps2,
-                                                           float* __restrict__ part)
+__device__ __forceinline__ void sym_item(const float4* __restrict__ src, int64_t np_pad,
+                                         int4 it, float eps2, float* __restrict__ part,
+                                         float4* tile, float (*jp)[3][64])
 {
     constexpr int BLK = kBlock * IPT;
     constexpr int NCH = BLK / 64;
-    // each 64-particle chunk of block J is stored twice in a row, so that lane l's
-    // rotating index l + s needs no wrap-around
-    __shared__ float4 tile[NCH * 128];
-    __shared__ float jp[kBlock / 64][3][64];
-    const int I = work[blockIdx.x].x, J = work[blockIdx.x].y;
+    const int I = it.x, J = it.y;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t ibase = (int64_t)I * BLK + wave * 64 * IPT + lane;
     const int64_t jbase = (int64_t)J * BLK;
@@ -313,28 +310,68 @@ __global__ __launch_bounds__(kBlock) void k_direct_sym_f32(const float4* __restr
                 float sacc = jp[0][comp][t];
 #pragma unroll
                 for (int w = 1; w < kBlock / 64; w++) sacc += jp[w][comp][t];
-                part[((int64_t)I * np_pad + jbase + c * 64 + t) * 3 + comp] = -sacc;
+                part[((int64_t)it.w * np_pad + jbase + c * 64 + t) * 3 + comp] = -sacc;
             }
             __syncthreads();
         }
     }
 #pragma unroll
     for (int k = 0; k < IPT; k++) {
-        float* o = part + ((int64_t)J * np_pad + ibase + k * 64) * 3;
+        float* o = part + ((int64_t)it.z * np_pad + ibase + k * 64) * 3;
         o[0] = ax[k];
         o[1] = ay[k];
         o[2] = az[k];
     }
 }
 
+// One launch over a list of block pairs.  queue == nullptr: workgroup b takes item b.
+// Otherwise the workgroups are persistent and take items from *queue (which must be 0
+// at launch) until the list is exhausted.
+template <int IPT, bool GUARD>
+__global__ __launch_bounds__(kBlock) void k_direct_sym_f32(const float4* __restrict__ src,
+                                                           int64_t np_pad,
+                                                           const int4* __restrict__ work,
+                                                           int64_t items,
+                                                           const float* __restrict__ eps2_blk,
+                                                           float* __restrict__ part,
+                                                           int* __restrict__ queue)
+{
+    constexpr int NCH = kBlock * IPT / 64;
+    // each 64-particle chunk of block J is stored twice in a row, so that lane l's
+    // rotating index l + s needs no wrap-around
+    __shared__ float4 tile[NCH * 128];
+    __shared__ float jp[kBlock / 64][3][64];
+    __shared__ int next;
+    if (!queue) {
+        const int4 it = work[blockIdx.x];
+        sym_item<IPT, GUARD>(src, np_pad, it, eps2_blk[it.x], part, tile, jp);
+        return;
+    }
+    for (;;) {
+        if (threadIdx.x == 0) next = atomicAdd(queue, 1);
+        __syncthreads();
+        const int64_t k = next;
+        __syncthreads();
+        if (k >= items) break;
+        const int4 it = work[k];
+        sym_item<IPT, GUARD>(src, np_pad, it, eps2_blk[it.x], part, tile, jp);
+        __syncthreads();
+    }
+}
+
+// acc[p] = sum over the partial slots of p's simulation, in slot order, in FP64.
+// nb_blk[b] (nullable): number of slots of the simulation that block b belongs to;
+// otherwise every particle has nb slots.
 __global__ __launch_bounds__(kBlock) void k_reduce_sym(const float* __restrict__ part, int nb,
-                                                       int64_t np_pad, int64_t n3,
+                                                       const int* __restrict__ nb_blk,
+                                                       int blk, int64_t np_pad, int64_t n3,
                                                        double* __restrict__ acc)
 {
     const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (k >= n3) return;
+    const int slots = nb_blk ? nb_blk[(k / 3) / blk] : nb;
     double s = 0.0;
-    for (int b = 0; b < nb; b++) s += (double)part[(int64_t)b * np_pad * 3 + k];
+    for (int b = 0; b < slots; b++) s += (double)part[(int64_t)b * np_pad * 3 + k];
     acc[k] = s;
 }
 
@@ -417,15 +454,47 @@ int sym_ipt(gh_ctx* c, int64_t np)
 }
 
 template <int IPT>
-static void launch_sym(bool guard, unsigned items, hipStream_t s, const float4* src,
-                       int64_t np_pad, const int2* work, float eps2, float* part)
+static void launch_sym_t(bool guard, unsigned grid, hipStream_t s, const float4* src,
+                         int64_t np_pad, const int4* work, int64_t items, const float* eps2_blk,
+                         float* part, int* queue)
 {
     if (guard)
-        hipLaunchKernelGGL((k_direct_sym_f32<IPT, true>), dim3(items), dim3(kBlock), 0, s, src,
-                           np_pad, work, eps2, part);
+        hipLaunchKernelGGL((k_direct_sym_f32<IPT, true>), dim3(grid), dim3(kBlock), 0, s, src,
+                           np_pad, work, items, eps2_blk, part, queue);
     else
-        hipLaunchKernelGGL((k_direct_sym_f32<IPT, false>), dim3(items), dim3(kBlock), 0, s, src,
-                           np_pad, work, eps2, part);
+        hipLaunchKernelGGL((k_direct_sym_f32<IPT, false>), dim3(grid), dim3(kBlock), 0, s, src,
+                           np_pad, work, items, eps2_blk, part, queue);
+}
+
+int launch_sym(gh_ctx* c, int ipt, bool guard, const float4* src, int64_t np_pad,
+               const int4* work, int64_t items, const float* eps2_blk, float* part, int* queue,
+               unsigned persistent_grid, hipStream_t s)
+{
+    if (items > 0x7fffffff) return fail(c, GH_EARG, "too many block pairs for one launch");
+    const unsigned grid = queue ? persistent_grid : (unsigned)items;
+    if (grid == 0) return 0;
+    switch (ipt) {
+        case 1: launch_sym_t<1>(guard, grid, s, src, np_pad, work, items, eps2_blk, part, queue); break;
+        case 2: launch_sym_t<2>(guard, grid, s, src, np_pad, work, items, eps2_blk, part, queue); break;
+        case 4: launch_sym_t<4>(guard, grid, s, src, np_pad, work, items, eps2_blk, part, queue); break;
+        default: return fail(c, GH_EARG, "unsupported particles-per-lane %d", ipt);
+    }
+    return check_launch(c, "k_direct_sym_f32");
+}
+
+int reduce_sym(gh_ctx* c, const float* part, int nb, const int* nb_blk, int blk,
+               int64_t np_pad, int64_t n, double* acc, hipStream_t s)
+{
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_reduce_sym, dim3(blocks_for(3 * n, kBlock)), dim3(kBlock), 0, s, part,
+                       nb, nb_blk, blk, np_pad, 3 * n, acc);
+    return check_launch(c, "k_reduce_sym");
+}
+
+__global__ void k_fill_f32(float* __restrict__ p, int64_t n, float v)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = v;
 }
 
 int direct_sym(gh_ctx* c, const double* d_pos, const double* d_mass, int64_t np, int ipt,
@@ -438,36 +507,34 @@ int direct_sym(gh_ctx* c, const double* d_pos, const double* d_mass, int64_t np,
     if (items > 0x7fffffff) return fail(c, GH_EARG, "too many particles for one launch");
     GH_TRY(ensure(c, c->src32a, sizeof(float4) * (size_t)np_pad));
     GH_TRY(ensure(c, c->scratch, sizeof(float) * 3 * (size_t)np_pad * nb));
-    if (c->sym_nb != nb) {
-        // block pairs (I, J), I <= J, ordered by I
-        std::vector<int2> w((size_t)items);
+    GH_TRY(ensure(c, c->eps_blk, sizeof(float) * (size_t)nb));
+    const float eps2 = (float)(eps * eps);
+    if (c->sym_nb != nb || c->sym_ipt != ipt) {
+        // block pairs (I, J), I <= J, ordered by I; slots are the block indices
+        std::vector<int4> w((size_t)items);
         size_t k = 0;
         for (int I = 0; I < nb; I++)
-            for (int J = I; J < nb; J++) w[k++] = make_int2(I, J);
-        GH_TRY(ensure(c, c->work, sizeof(int2) * (size_t)items));
-        GH_HIP(c, hipMemcpyAsync(c->work.p, w.data(), sizeof(int2) * (size_t)items,
+            for (int J = I; J < nb; J++) w[k++] = make_int4(I, J, J, I);
+        GH_TRY(ensure(c, c->work, sizeof(int4) * (size_t)items));
+        GH_HIP(c, hipMemcpyAsync(c->work.p, w.data(), sizeof(int4) * (size_t)items,
                                  hipMemcpyHostToDevice, s));
         GH_HIP(c, hipStreamSynchronize(s));
         c->sym_nb = nb;
+        c->sym_ipt = ipt;
     }
+    hipLaunchKernelGGL(k_fill_f32, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, s,
+                       c->eps_blk.as<float>(), (int64_t)nb, eps2);
+    GH_TRY(check_launch(c, "k_fill_f32"));
     float4* src = c->src32a.as<float4>();
     GH_TRY(pack_src32(c, d_pos, d_mass, np, src, np_pad, s));
-    const float eps2 = (float)(eps * eps);
     const bool guard = !(eps2 >= 1e-30f);
     float* part = c->scratch.as<float>();
     {
         TimedScope ts(c, s, GH_T_DIRECT_F32);
-        switch (ipt) {
-            case 1: launch_sym<1>(guard, (unsigned)items, s, src, np_pad, c->work.as<int2>(), eps2, part); break;
-            case 2: launch_sym<2>(guard, (unsigned)items, s, src, np_pad, c->work.as<int2>(), eps2, part); break;
-            case 4: launch_sym<4>(guard, (unsigned)items, s, src, np_pad, c->work.as<int2>(), eps2, part); break;
-            default: return fail(c, GH_EARG, "unsupported particles-per-lane %d", ipt);
-        }
-        GH_TRY(check_launch(c, "k_direct_sym_f32"));
+        GH_TRY(launch_sym(c, ipt, guard, src, np_pad, c->work.as<int4>(), items,
+                          c->eps_blk.as<float>(), part, nullptr, 0, s));
     }
-    hipLaunchKernelGGL(k_reduce_sym, dim3(blocks_for(3 * np, kBlock)), dim3(kBlock), 0, s, part,
-                       nb, np_pad, 3 * np, d_acc);
-    return check_launch(c, "k_reduce_sym");
+    return reduce_sym(c, part, nb, nullptr, (int)blk, np_pad, np, d_acc, s);
 }
 
 int direct_f32_packed(gh_ctx* c, const float4* src, int64_t np_pad, const float4* tgt,

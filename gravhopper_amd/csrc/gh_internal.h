@@ -42,8 +42,9 @@ struct gh_ctx {
     gh::DevBuf xs, vs, xh, hist_p, hist_v, scratch, counters;
     gh::TreeBufs* tree = nullptr;
     int tune[GH_TUNE_COUNT] = {0};  // gh_set_tuning values, 0 = automatic
-    gh::DevBuf work;                // block-pair list of the symmetric direct kernel
-    int sym_nb = -1;
+    gh::DevBuf work;                // block-pair list of the pair-once direct kernel
+    gh::DevBuf eps_blk;             // its per-block squared softening
+    int sym_nb = -1, sym_ipt = -1;
     int64_t last_tree_nodes = 0;
     int last_tree_depth = 0;
     // kernel timing (gh_timing_enable / gh_timing_read)
@@ -107,6 +108,11 @@ int64_t pad_to(int64_t n, int64_t m);
 int sym_ipt(gh_ctx* c, int64_t np);
 int direct_sym(gh_ctx* c, const double* d_pos, const double* d_mass, int64_t np, int ipt,
                double eps, double* d_acc, hipStream_t s);
+int launch_sym(gh_ctx* c, int ipt, bool guard, const float4* src, int64_t np_pad,
+               const int4* work, int64_t items, const float* eps2_blk, float* part, int* queue,
+               unsigned persistent_grid, hipStream_t s);
+int reduce_sym(gh_ctx* c, const float* part, int nb, const int* nb_blk, int blk,
+               int64_t np_pad, int64_t n, double* acc, hipStream_t s);
 int direct_f32_packed(gh_ctx* c, const float4* src, int64_t np_pad, const float4* tgt,
                       int64_t nt, bool self, double eps, double* d_acc, hipStream_t s);
 

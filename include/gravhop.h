@@ -133,6 +133,26 @@ int gh_run_dev(gh_ctx *ctx, double *d_pos, double *d_vel, const double *d_mass, 
                double dt, double eps, int algorithm, double theta, int precision,
                int64_t nsteps, double *d_hist_pos, double *d_hist_vel, void *stream);
 
+/* Batched ensembles (no reference counterpart): n_sims independent direct-sum
+ * simulations of uneven size, each advanced by the leapfrog of gh_run with its own
+ * softening and time step, all with FP32 pair forces (pair-once kernel, `ipt`
+ * particles per lane: 1, 2 or 4; 0 selects 4) and FP64 state.  One force launch per
+ * step covers every simulation; `persistent` makes it a fixed grid fed from an atomic
+ * work counter.  pos/vel/mass are the simulations' (count[s],3) / (count[s]) arrays
+ * concatenated in order; eps and dt have one entry per simulation.  Each simulation's
+ * trajectory is bit-identical to running it alone with the same kernel shape. */
+typedef struct gh_ensemble gh_ensemble;
+int gh_ensemble_create(gh_ctx *ctx, int n_sims, const int64_t *counts, const double *pos,
+                       const double *vel, const double *mass, const double *eps,
+                       const double *dt, int ipt, int persistent, gh_ensemble **out);
+/* nsteps steps; use_graph != 0 replays one captured step as a HIP graph. */
+int gh_ensemble_run(gh_ensemble *ens, int64_t nsteps, int use_graph);
+/* current full-step positions and velocities (concatenated, host memory) */
+int gh_ensemble_get(gh_ensemble *ens, double *pos, double *vel);
+int gh_ensemble_info(const gh_ensemble *ens, int64_t *n_padded, int64_t *block_pairs,
+                     int64_t *pairs_per_step);
+void gh_ensemble_destroy(gh_ensemble *ens);
+
 /* Kernel timing (no reference counterpart).  While enabled, the library brackets its
  * kernels with HIP events on the stream they run on; gh_timing_read synchronises and
  * returns the summed duration [ms] and launch count of one kernel class since the
