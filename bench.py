@@ -204,6 +204,56 @@ def bench_ensemble(ctx, torch, steps):
     return out
 
 
+def bench_adjoint(ctx, torch, steps=200, every=20):
+    """Plummer N=16384, b=1 pc, 1e6 Msun, eps=0.05 pc, dt=0.005 Myr, FP32 forces, FP64
+    state; loss = mean |x_final - x_target|^2 with the target run from masses scaled
+    by 1 + 0.05 N(0,1); gradients w.r.t. x0, v0, m through the adjoint, checkpoint
+    every 20 steps."""
+    from gravhopper_amd import _native, workloads
+    from gravhopper_amd.autograd import leapfrog
+    n, dt, eps = 16384, 0.005, 5e-5
+    ic = workloads.plummer(n, b=1e-3, mass=1e6, seed=5)
+    rng = np.random.default_rng(6)
+    m_t = ic["mass"] * (1 + 0.05 * rng.normal(size=n))
+    dev = torch.device("cuda", ctx.device)
+    xt = torch.tensor(ic["pos"], device=dev)
+    vt = torch.tensor(ic["vel"], device=dev)
+    mt = torch.tensor(m_t, device=dev)
+    ctx.run_dev(xt.data_ptr(), vt.data_ptr(), mt.data_ptr(), n, dt, eps, steps,
+                precision=_native.GH_FP32)
+    ctx.sync()
+    x0 = torch.tensor(ic["pos"], device=dev, requires_grad=True)
+    v0 = torch.tensor(ic["vel"], device=dev, requires_grad=True)
+    m = torch.tensor(ic["mass"], device=dev, requires_grad=True)
+    # warm the kernels and workspaces
+    x, _ = leapfrog(x0, v0, m, dt, eps, 2, checkpoint_every=every)
+    ((x - xt) ** 2).sum(-1).mean().backward()
+    torch.cuda.synchronize()
+    x0.grad = v0.grad = m.grad = None
+    ctx.timing_enable(True)
+    t0 = time.perf_counter()
+    x, _ = leapfrog(x0, v0, m, dt, eps, steps, checkpoint_every=every)
+    loss = ((x - xt) ** 2).sum(-1).mean()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    loss.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    vjp_ms, vjp_n = ctx.timing_read(_native.T_VJP)
+    f_ms, f_n = ctx.timing_read(_native.T_DIRECT_F32)
+    ctx.timing_enable(False)
+    fwd, bwd = (t1 - t0) / steps, (t2 - t1) / steps
+    return {"workload": "Plummer N=16384, eps=0.05 pc, dt=0.005 Myr, 200 steps, checkpoint "
+                        "every 20, FP32 forces and VJP pairs, FP64 state",
+            "loss": float(loss), "forward_ms_per_step": fwd * 1e3,
+            "backward_ms_per_step": bwd * 1e3,
+            "adjoint_over_forward": bwd / fwd,
+            "vjp_kernel_ms": vjp_ms / max(vjp_n, 1),
+            "force_kernel_ms": f_ms / max(f_n, 1),
+            "vjp_over_force_kernel": (vjp_ms / max(vjp_n, 1)) / (f_ms / max(f_n, 1)),
+            "grad_norms": [float(t.grad.norm()) for t in (x0, v0, m)]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -215,6 +265,7 @@ def main():
     ap.add_argument("--no-tree", action="store_true")
     ap.add_argument("--tree-steps", type=int, default=5)
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=5)
     ap.add_argument("--only-tree", action="store_true",
                     help="run only the N=1M tree steps (for profiling)")
@@ -333,6 +384,9 @@ def main():
     if not args.no_extras and not args.no_tree:
         log("bench: tree step at N=1M")
         extras["tree"] = bench_tree(ctx, torch, args.tree_steps, 1)
+    if not args.no_extras and not args.no_adjoint:
+        log("bench: adjoint at N=16384")
+        extras["adjoint"] = bench_adjoint(ctx, torch)
     if not args.no_extras and not args.no_ensemble:
         log("bench: ensemble of 128 spheres")
         extras["ensemble"] = bench_ensemble(ctx, torch, args.ensemble_steps)

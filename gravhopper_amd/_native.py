@@ -54,6 +54,8 @@ SIGNATURES = {
     "gh_ensemble_info": ([_P, ctypes.POINTER(_I64), ctypes.POINTER(_I64),
                           ctypes.POINTER(_I64)], _I),
     "gh_ensemble_destroy": ([_P], None),
+    "gh_adjoint_forward": ([_P, _P, _P, _P, _I64, _D, _D, _I, _I64, _I64, _P, _P], _I),
+    "gh_adjoint_backward": ([_P, _P, _P, _I64, _D, _D, _I, _I64, _I64, _P, _P, _P, _P], _I),
     "gh_timing_read": ([_P, _I, ctypes.POINTER(_D), ctypes.POINTER(_I64)], _I),
     "gh_energy": ([_P, _P, _P, _P, _I64, _D, _P, _P, _P], _I),
 }
@@ -62,7 +64,7 @@ SIGNATURES = {
 TUNE_DIRECT_SELF, TUNE_DIRECT_IPT = 0, 1
 
 # kernel classes of gh_timing_read
-T_DIRECT_F32, T_DIRECT_F64, T_TREE_BUILD, T_TREE_WALK, T_UPDATE = range(5)
+T_DIRECT_F32, T_DIRECT_F64, T_TREE_BUILD, T_TREE_WALK, T_UPDATE, T_VJP = range(6)
 
 NODE_DTYPE = np.dtype([
     ("level", np.int32), ("branch", np.int32), ("leaf", np.int64),

@@ -202,7 +202,8 @@ void gh_ctx_destroy(gh_ctx* c)
     DevBuf* bufs[] = {&c->pos64, &c->mass64, &c->fpos64, &c->acc64, &c->src64,
                       &c->src32a, &c->src32b, &c->tgt32,  &c->xs,    &c->vs,
                       &c->xh,     &c->hist_p, &c->hist_v, &c->scratch, &c->counters,
-                      &c->work,   &c->eps_blk};
+                      &c->work,   &c->eps_blk, &c->adj_work, &c->adj_src, &c->adj_mu,
+                      &c->adj_mv, &c->adj_part, &c->adj_seg};
     for (DevBuf* b : bufs) free_buf(*b);
     release_spans(c);
     for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);

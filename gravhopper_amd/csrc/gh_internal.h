@@ -45,6 +45,9 @@ struct gh_ctx {
     gh::DevBuf work;                // block-pair list of the pair-once direct kernel
     gh::DevBuf eps_blk;             // its per-block squared softening
     int sym_nb = -1, sym_ipt = -1;
+    // adjoint workspaces (gh_adjoint.hip)
+    gh::DevBuf adj_work, adj_src, adj_mu, adj_mv, adj_part, adj_seg;
+    int adj_nb = -1, adj_ipt = -1;
     int64_t last_tree_nodes = 0;
     int last_tree_depth = 0;
     // kernel timing (gh_timing_enable / gh_timing_read)

@@ -133,6 +133,21 @@ int gh_run_dev(gh_ctx *ctx, double *d_pos, double *d_vel, const double *d_mass, 
                double dt, double eps, int algorithm, double theta, int precision,
                int64_t nsteps, double *d_hist_pos, double *d_hist_vel, void *stream);
 
+/* Adjoint of gh_run's direct-sum leapfrog (no reference counterpart).
+ * gh_adjoint_forward advances device arrays d_x, d_v (n,3) by nsteps, saving (x, v)
+ * every `every` steps into d_ckpt (ceil(nsteps/every) x 2 x (n,3) doubles).
+ * gh_adjoint_backward takes dL/dx and dL/dv at the end of the run in d_lx, d_lv and
+ * overwrites them with dL/dx and dL/dv at the start; d_lm (n) receives dL/dm.  Each
+ * segment is re-run from its checkpoint with the forward kernels of `precision`
+ * (bit-identical to the forward pass); the vector-Jacobian products through the
+ * softened tidal tensor use FP32 pairs (pair-once kernel) and FP64 sums. */
+int gh_adjoint_forward(gh_ctx *ctx, double *d_x, double *d_v, const double *d_m, int64_t n,
+                       double dt, double eps, int precision, int64_t nsteps, int64_t every,
+                       double *d_ckpt, void *stream);
+int gh_adjoint_backward(gh_ctx *ctx, const double *d_ckpt, const double *d_m, int64_t n,
+                        double dt, double eps, int precision, int64_t nsteps, int64_t every,
+                        double *d_lx, double *d_lv, double *d_lm, void *stream);
+
 /* Batched ensembles (no reference counterpart): n_sims independent direct-sum
  * simulations of uneven size, each advanced by the leapfrog of gh_run with its own
  * softening and time step, all with FP32 pair forces (pair-once kernel, `ipt`
@@ -162,7 +177,8 @@ void gh_ensemble_destroy(gh_ensemble *ens);
 #define GH_T_TREE_BUILD 2 /* octree build (keys, sort, linking, moments)       */
 #define GH_T_TREE_WALK 3  /* octree walk                                       */
 #define GH_T_UPDATE 4     /* fused leapfrog update                             */
-#define GH_T_COUNT 5
+#define GH_T_VJP 5        /* adjoint pair kernel (vector-Jacobian product)     */
+#define GH_T_COUNT 6
 int gh_timing_enable(gh_ctx *ctx, int on);
 int gh_timing_read(gh_ctx *ctx, int kernel_class, double *total_ms, int64_t *launches);
 
