@@ -99,6 +99,7 @@ def bench_tree(ctx, torch, steps, warmup, lite=False):
     x = torch.tensor(g["pos"], dtype=torch.float64, device=dev).contiguous()
     v = torch.tensor(g["vel"], dtype=torch.float64, device=dev).contiguous()
     m = torch.tensor(g["mass"], dtype=torch.float64, device=dev).contiguous()
+    torch.cuda.synchronize()  # library kernels run on the context's own stream
     FP32, FP64, TREE = _native.GH_FP32, _native.GH_FP64, _native.GH_TREE
     ctx.run_dev(x.data_ptr(), v.data_ptr(), m.data_ptr(), n, dt, eps, warmup,
                 algorithm=TREE, theta=theta, precision=FP32)
@@ -133,6 +134,7 @@ def bench_tree(ctx, torch, steps, warmup, lite=False):
     ctx.tree_dev(x.data_ptr(), m.data_ptr(), n, eps, theta, FP32, acc_t.data_ptr())
     tsub = torch.tensor(pos[sub], dtype=torch.float64, device=dev).contiguous()
     acc_d = torch.empty((len(sub), 3), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
     ctx.direct_dev(x.data_ptr(), m.data_ptr(), n, eps, FP64, acc_d.data_ptr(),
                    d_force_pos=tsub.data_ptr(), nf=len(sub))
     ctx.sync()
@@ -187,6 +189,7 @@ def bench_ensemble(ctx, torch, steps):
         x = torch.tensor(s["pos"], dtype=torch.float64, device=dev)
         v = torch.tensor(s["vel"], dtype=torch.float64, device=dev)
         m = torch.tensor(s["mass"], dtype=torch.float64, device=dev)
+        torch.cuda.synchronize()
         ctx.run_dev(x.data_ptr(), v.data_ptr(), m.data_ptr(), n, s["dt"], s["eps"], 1,
                     precision=_native.GH_FP32)
         ctx.sync()
@@ -219,6 +222,7 @@ def bench_adjoint(ctx, torch, steps=200, every=20):
     xt = torch.tensor(ic["pos"], device=dev)
     vt = torch.tensor(ic["vel"], device=dev)
     mt = torch.tensor(m_t, device=dev)
+    torch.cuda.synchronize()
     ctx.run_dev(xt.data_ptr(), vt.data_ptr(), mt.data_ptr(), n, dt, eps, steps,
                 precision=_native.GH_FP32)
     ctx.sync()
@@ -303,6 +307,7 @@ def main():
     x = torch.tensor(ic["pos"], dtype=torch.float64, device=dev).contiguous()
     v = torch.tensor(ic["vel"], dtype=torch.float64, device=dev).contiguous()
     m = torch.tensor(ic["mass"], dtype=torch.float64, device=dev).contiguous()
+    torch.cuda.synchronize()  # library kernels run on the context's own stream
     FP32, FP64 = _native.GH_FP32, _native.GH_FP64
 
     log("bench: N=%d warmup=%d steps=%d" % (n, args.warmup, args.steps))
@@ -359,6 +364,7 @@ def main():
         vs = v.cpu().numpy()
         acc64 = torch.empty((n, 3), dtype=torch.float64, device=dev)
         acc32 = torch.empty((n, 3), dtype=torch.float64, device=dev)
+        torch.cuda.synchronize()
         ctx.timing_enable(True)
         ctx.direct_dev(x.data_ptr(), m.data_ptr(), n, EPS_DIRECT, FP64, acc64.data_ptr())
         ctx.direct_dev(x.data_ptr(), m.data_ptr(), n, EPS_DIRECT, FP32, acc32.data_ptr())

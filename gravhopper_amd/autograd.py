@@ -9,7 +9,7 @@ eps in kpc.  The forward is the drift-kick-drift of Simulation.run (gh_adjoint_f
 the same kernels as gh_run) and keeps (x, v) every `checkpoint_every` steps; backward
 re-runs each segment from its checkpoint and applies the reverse step with the
 vector-Jacobian products through the softened tidal tensor (gh_adjoint_backward).
-Both run on torch's current stream.
+Both are ordered with torch's current stream.
 """
 import ctypes
 import math
@@ -23,6 +23,27 @@ def _p(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
+class _OnTorchStream:
+    """Orders library work with torch's current stream.  A real stream is handed to the
+    library; torch's legacy default stream (handle 0) cannot be, since 0 selects the
+    context's own stream, so then the call is fenced by synchronisation on both sides."""
+
+    def __init__(self, lib, device):
+        self.lib = lib
+        self.torch_stream = torch.cuda.current_stream(device)
+        self.handle = self.torch_stream.cuda_stream
+
+    def __enter__(self):
+        if self.handle == 0:
+            self.torch_stream.synchronize()
+        return ctypes.c_void_p(self.handle or None)
+
+    def __exit__(self, *exc):
+        if self.handle == 0:
+            self.lib.sync()
+        return False
+
+
 class DirectLeapfrog(torch.autograd.Function):
     @staticmethod
     def forward(ctx, x0, v0, m, dt, eps, nsteps, every, precision):
@@ -33,10 +54,10 @@ class DirectLeapfrog(torch.autograd.Function):
         mm = m.detach().to(torch.float64).contiguous()
         nseg = max(1, math.ceil(nsteps / every))
         ckpt = torch.empty((nseg, 2, n, 3), dtype=torch.float64, device=x.device)
-        stream = ctypes.c_void_p(torch.cuda.current_stream(x.device).cuda_stream)
-        lib.check(lib.lib.gh_adjoint_forward(lib.handle, _p(x), _p(v), _p(mm), n, float(dt),
-                                             float(eps), int(precision), int(nsteps),
-                                             int(every), _p(ckpt), stream))
+        with _OnTorchStream(lib, x.device) as stream:
+            lib.check(lib.lib.gh_adjoint_forward(lib.handle, _p(x), _p(v), _p(mm), n, float(dt),
+                                                 float(eps), int(precision), int(nsteps),
+                                                 int(every), _p(ckpt), stream))
         ctx.save_for_backward(mm, ckpt)
         ctx.params = (float(dt), float(eps), int(precision), int(nsteps), int(every))
         return x, v
@@ -53,10 +74,10 @@ class DirectLeapfrog(torch.autograd.Function):
             torch.float64).contiguous().clone()
         lm = torch.empty(n, dtype=torch.float64, device=dev)
         lib = _native.default_context()
-        stream = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
-        lib.check(lib.lib.gh_adjoint_backward(lib.handle, _p(ckpt), _p(mm), n, dt, eps,
-                                              precision, nsteps, every, _p(lx), _p(lv),
-                                              _p(lm), stream))
+        with _OnTorchStream(lib, dev) as stream:
+            lib.check(lib.lib.gh_adjoint_backward(lib.handle, _p(ckpt), _p(mm), n, dt, eps,
+                                                  precision, nsteps, every, _p(lx), _p(lv),
+                                                  _p(lm), stream))
         return lx, lv, lm, None, None, None, None, None
 
 
