@@ -147,6 +147,27 @@ def bench_tree(ctx, torch, steps, warmup, lite=False):
     _, cnt = ctx.tree_interactions(pos, mass, eps, theta)
     out["interactions_per_target"] = float(cnt.mean())
     out["walk_achieved_tflops"] = 20.0 * cnt.sum() / (out["walk_ms"] * 1e-3) / 1e12
+    # opening angle trade-off: step time and force error at smaller theta
+    sweep = []
+    for th in (0.7, 0.5, 0.4, 0.3):
+        xs = torch.tensor(pos, dtype=torch.float64, device=dev)
+        vs = torch.tensor(g["vel"], dtype=torch.float64, device=dev)
+        torch.cuda.synchronize()
+        ctx.run_dev(xs.data_ptr(), vs.data_ptr(), m.data_ptr(), n, dt, eps, 1,
+                    algorithm=TREE, theta=th, precision=FP32)
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.run_dev(xs.data_ptr(), vs.data_ptr(), m.data_ptr(), n, dt, eps, 3,
+                    algorithm=TREE, theta=th, precision=FP32)
+        ctx.sync()
+        st = (time.perf_counter() - t0) / 3
+        ctx.tree_dev(x.data_ptr(), m.data_ptr(), n, eps, th, FP32, acc_t.data_ptr())
+        ctx.sync()
+        e = np.linalg.norm(acc_t.cpu().numpy()[sub] - ad, axis=1) / np.linalg.norm(ad, axis=1)
+        sweep.append({"theta": th, "ms_per_step": st * 1e3,
+                      "force_rel_err_median": float(np.median(e)),
+                      "force_rel_err_p99": float(np.percentile(e, 99))})
+    out["theta_sweep"] = sweep
     # the FP32 direct sum at the same N, one evaluation
     ctx.timing_enable(True)
     ctx.direct_dev(x.data_ptr(), m.data_ptr(), n, eps, FP32, acc_t.data_ptr())

@@ -299,7 +299,7 @@ __global__ __launch_bounds__(kBlock) void k_rs_scatter(
 // ------------------------------------------------------------------------------
 // dp[i] = delta_i + 1 (delta_i = shared leading levels of sorted keys i and i+1), with
 // dp[n-1] = 0; flags[0] counts pairs equal over all words, flags[1] counts those at
-// identical positions, flags[2] = max delta + 1.
+// identical positions (both rare, so their atomics cost nothing).
 __global__ __launch_bounds__(kBlock) void k_delta(const u64* __restrict__ keys, int64_t n,
                                                   int words, const u32* __restrict__ perm,
                                                   const double* __restrict__ pos,
@@ -328,13 +328,6 @@ __global__ __launch_bounds__(kBlock) void k_delta(const u64* __restrict__ keys, 
         m = delta + 1;
     }
     if (i < n) dp[i] = (unsigned char)m;
-    // deepest level: one atomic per wavefront (all lanes take part in the shuffles)
-    int mx = m;
-    for (int off = 32; off > 0; off >>= 1) {
-        const int o = __shfl_xor(mx, off);
-        mx = o > mx ? o : mx;
-    }
-    if ((threadIdx.x & 63) == 0) atomicMax(&flags[2], mx);
 }
 
 // cnt[i] = internal nodes starting at sorted position i, plus its leaf
@@ -630,7 +623,9 @@ __global__ __launch_bounds__(kBlock) void k_fill_nodes(FillArgs a, int64_t n)
 // ------------------------------------------------------------------------------
 // FP32 monopoles, FP64 accumulation.  Targets: tgt (AoS f64) in the order given by
 // order[] (nullable), results written to acc[order[k]].
-template <bool GUARD>
+// Each wavefront runs WPW independent walks (WPW groups of 64 consecutive targets);
+// their node loads are issued together, so one walk's load latency overlaps the others'.
+template <int WPW, bool GUARD>
 __global__ __launch_bounds__(kBlock) void k_walk_f32(const NodeF* __restrict__ nodes,
                                                      const int* __restrict__ skip,
                                                      int64_t n_nodes,
@@ -638,51 +633,80 @@ __global__ __launch_bounds__(kBlock) void k_walk_f32(const NodeF* __restrict__ n
                                                      const u32* __restrict__ order, int64_t nt,
                                                      float eps2, double* __restrict__ acc)
 {
-    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    const bool live = k < nt;
-    int64_t t = 0;
-    float x = 0.f, y = 0.f, z = 0.f;
-    if (live) {
-        t = order ? (int64_t)order[k] : k;
-        x = (float)tgt[3 * t];
-        y = (float)tgt[3 * t + 1];
-        z = (float)tgt[3 * t + 2];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t base = (int64_t)blockIdx.x * kBlock * WPW + (int64_t)wave * 64 * WPW + lane;
+    int64_t t[WPW];
+    float x[WPW], y[WPW], z[WPW];
+    double ax[WPW], ay[WPW], az[WPW];
+    int my_next[WPW], node[WPW];
+#pragma unroll
+    for (int w = 0; w < WPW; w++) {
+        const int64_t k = base + w * 64;
+        const bool live = k < nt;
+        t[w] = 0;
+        x[w] = y[w] = z[w] = 0.f;
+        if (live) {
+            t[w] = order ? (int64_t)order[k] : k;
+            x[w] = (float)tgt[3 * t[w]];
+            y[w] = (float)tgt[3 * t[w] + 1];
+            z[w] = (float)tgt[3 * t[w] + 2];
+        }
+        ax[w] = ay[w] = az[w] = 0.0;
+        my_next[w] = live ? 0 : 0x7fffffff;
+        node[w] = 0;
     }
-    double ax = 0.0, ay = 0.0, az = 0.0;
-    int my_next = live ? 0 : 0x7fffffff;
-    int node = 0;
-    while (node < n_nodes) {
-        const NodeF nd = nodes[node];
-        const int sk = skip[node];
-        const bool active = (my_next == node);
-        bool open = false;
-        if (active) {
-            const float cdx = nd.geo.x - x, cdy = nd.geo.y - y, cdz = nd.geo.z - z;
-            const float d2 = __builtin_fmaf(cdx, cdx, __builtin_fmaf(cdy, cdy, cdz * cdz));
-            open = !(d2 > nd.geo.w);
-            if (!open) {
-                const float dx = nd.mon.x - x, dy = nd.mon.y - y, dz = nd.mon.z - z;
-                float r2 = __builtin_fmaf(dx, dx, eps2);
-                r2 = __builtin_fmaf(dy, dy, r2);
-                r2 = __builtin_fmaf(dz, dz, r2);
-                const float ri = __builtin_amdgcn_rsqf(r2);
-                float sc = (nd.mon.w * ri) * (ri * ri);
-                if (GUARD) sc = (r2 > 0.f) ? sc : 0.f;
-                ax += (double)(dx * sc);
-                ay += (double)(dy * sc);
-                az += (double)(dz * sc);
-                my_next = sk;
-            } else {
-                my_next = node + 1;
+    for (;;) {
+        bool more = false;
+#pragma unroll
+        for (int w = 0; w < WPW; w++) more = more || node[w] < n_nodes;
+        if (!more) break;
+        NodeF nd[WPW];
+        int sk[WPW];
+#pragma unroll
+        for (int w = 0; w < WPW; w++) {
+            if (node[w] < n_nodes) {
+                nd[w] = nodes[node[w]];
+                sk[w] = skip[node[w]];
             }
         }
-        const bool any_open = __ballot(active && open) != 0ull;
-        node = __builtin_amdgcn_readfirstlane(any_open ? node + 1 : sk);
+#pragma unroll
+        for (int w = 0; w < WPW; w++) {
+            if (node[w] >= n_nodes) continue;
+            const bool active = (my_next[w] == node[w]);
+            bool open = false;
+            if (active) {
+                const float cdx = nd[w].geo.x - x[w], cdy = nd[w].geo.y - y[w],
+                            cdz = nd[w].geo.z - z[w];
+                const float d2 = __builtin_fmaf(cdx, cdx, __builtin_fmaf(cdy, cdy, cdz * cdz));
+                open = !(d2 > nd[w].geo.w);
+                if (!open) {
+                    const float dx = nd[w].mon.x - x[w], dy = nd[w].mon.y - y[w],
+                                dz = nd[w].mon.z - z[w];
+                    float r2 = __builtin_fmaf(dx, dx, eps2);
+                    r2 = __builtin_fmaf(dy, dy, r2);
+                    r2 = __builtin_fmaf(dz, dz, r2);
+                    const float ri = __builtin_amdgcn_rsqf(r2);
+                    float sc = (nd[w].mon.w * ri) * (ri * ri);
+                    if (GUARD) sc = (r2 > 0.f) ? sc : 0.f;
+                    ax[w] += (double)(dx * sc);
+                    ay[w] += (double)(dy * sc);
+                    az[w] += (double)(dz * sc);
+                    my_next[w] = sk[w];
+                } else {
+                    my_next[w] = node[w] + 1;
+                }
+            }
+            const bool any_open = __ballot(active && open) != 0ull;
+            node[w] = __builtin_amdgcn_readfirstlane(any_open ? node[w] + 1 : sk[w]);
+        }
     }
-    if (live) {
-        acc[3 * t] = ax;
-        acc[3 * t + 1] = ay;
-        acc[3 * t + 2] = az;
+#pragma unroll
+    for (int w = 0; w < WPW; w++) {
+        if (base + w * 64 < nt) {
+            acc[3 * t[w]] = ax[w];
+            acc[3 * t[w] + 1] = ay[w];
+            acc[3 * t[w] + 2] = az[w];
+        }
     }
 }
 
@@ -830,10 +854,7 @@ static int build_tree(gh_ctx* c, const double* d_pos, const double* d_mass, int6
             return fail(c, GH_ECOINCIDENT,
                         "%d pair(s) of particles share a position; the octree cannot "
                         "separate them", got[1]);
-        if (got[0] == 0) {
-            c->last_tree_depth = got[2];
-            break;
-        }
+        if (got[0] == 0) break;
         if (++words > kMaxWords)
             return fail(c, GH_EDEPTH,
                         "particles closer than %d octree levels below the root box",
@@ -929,14 +950,29 @@ static int walk(gh_ctx* c, const double* tgt, const u32* order, int64_t nt, doub
     }
     const float eps2 = (float)(eps * eps);
     const bool guard = !(eps2 >= 1e-30f);
-    if (guard)
-        hipLaunchKernelGGL(k_walk_f32<true>, dim3(blocks_for(nt, kBlock)), dim3(kBlock), 0, s,
-                           tb->nodes32.as<NodeF>(), tb->skip.as<int>(), tb->n_nodes, tgt, order,
-                           nt, eps2, d_acc);
-    else
-        hipLaunchKernelGGL(k_walk_f32<false>, dim3(blocks_for(nt, kBlock)), dim3(kBlock), 0, s,
-                           tb->nodes32.as<NodeF>(), tb->skip.as<int>(), tb->n_nodes, tgt, order,
-                           nt, eps2, d_acc);
+    // two walks per wavefront once there are more walks than resident wavefronts
+    const int wpw = c->tune[GH_TUNE_WALKS_PER_WAVE] == 1 || c->tune[GH_TUNE_WALKS_PER_WAVE] == 2
+                        ? c->tune[GH_TUNE_WALKS_PER_WAVE]
+                        : (nt >= ((int64_t)1 << 18) ? 2 : 1);
+    const NodeF* nd = tb->nodes32.as<NodeF>();
+    const int* sk = tb->skip.as<int>();
+    const int64_t T = tb->n_nodes;
+    const dim3 grid(blocks_for(nt, kBlock * wpw));
+    if (wpw == 2) {
+        if (guard)
+            hipLaunchKernelGGL((k_walk_f32<2, true>), grid, dim3(kBlock), 0, s, nd, sk, T, tgt,
+                               order, nt, eps2, d_acc);
+        else
+            hipLaunchKernelGGL((k_walk_f32<2, false>), grid, dim3(kBlock), 0, s, nd, sk, T, tgt,
+                               order, nt, eps2, d_acc);
+    } else {
+        if (guard)
+            hipLaunchKernelGGL((k_walk_f32<1, true>), grid, dim3(kBlock), 0, s, nd, sk, T, tgt,
+                               order, nt, eps2, d_acc);
+        else
+            hipLaunchKernelGGL((k_walk_f32<1, false>), grid, dim3(kBlock), 0, s, nd, sk, T, tgt,
+                               order, nt, eps2, d_acc);
+    }
     return check_launch(c, "k_walk_f32");
 }
 

@@ -142,3 +142,15 @@ def test_simulation_tree_device_vs_host(ctx):
     b.run(4)
     assert_bitwise(a.positions, b.positions)
     assert_bitwise(a.velocities, b.velocities)
+
+
+def test_walks_per_wave_bitwise(ctx):
+    # one or two independent walks per wavefront: every target accepts the same nodes
+    # in the same order, so the forces are bit-identical
+    g = workloads.galaxy(n_halo=20000, n_disk=20000, seed=82)
+    out = []
+    for wpw in (1, 2):
+        ctx.set_tuning(_native.TUNE_WALKS_PER_WAVE, wpw)
+        out.append(ctx.tree(g["pos"], g["mass"], 0.05, 0.7, FP32))
+    ctx.set_tuning(_native.TUNE_WALKS_PER_WAVE, 0)
+    assert_bitwise(out[0], out[1])
