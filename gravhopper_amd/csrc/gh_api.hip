@@ -9,7 +9,9 @@ namespace gh {
 int half_drift(gh_ctx* c, const double* x, const double* v, int64_t n, double dt, double* xh,
                hipStream_t s);
 int kick_drift(gh_ctx* c, const double* acc, int64_t n, double dt, double* xh, double* v,
-               double* x, double* hp, double* hv, hipStream_t s);
+               double* x, double* hp, double* hv, hipStream_t s,
+               const long long* slot = nullptr);
+int set_slot(gh_ctx* c, long long* slot, long long value, int add, hipStream_t s);
 int energy_dev(gh_ctx* c, const double* d_pos, const double* d_vel, const double* d_mass,
                int64_t n, double eps, double* d_out5, hipStream_t s);
 
@@ -131,7 +133,32 @@ static int check_common(gh_ctx* c, int64_t np, int64_t nf, double eps, int preci
     return 0;
 }
 
+// One step: force at xh, then the fused kick/drift/half-drift.  slot != nullptr: the
+// history slot is read on the device (graph replay); otherwise hp/hv point at it.
+static int one_step(gh_ctx* c, double* x, double* v, const double* m, int64_t np,
+                    double dt, double eps, int algorithm, double theta, int precision,
+                    double* hp, double* hv, const long long* slot, hipStream_t s)
+{
+    const int64_t n3 = 3 * np;
+    double* xh = c->xh.as<double>();
+    double* acc = c->acc64.as<double>();
+    if (np > 1) {
+        if (algorithm == GH_DIRECT)
+            GH_TRY(direct_dev(c, xh, m, np, nullptr, np, eps, precision, acc, s));
+        else
+            GH_TRY(tree_dev(c, xh, m, np, nullptr, np, eps, theta, precision, acc, s));
+    } else {
+        // gravhopper.py:442,449-450: a single particle feels no N-body force
+        GH_TRY(zero_dev(c, acc, n3, s));
+    }
+    TimedScope ts(c, s, GH_T_UPDATE);
+    return kick_drift(c, acc, np, dt, xh, v, x, hp, hv, s, slot);
+}
+
 // nsteps DKD steps on device state; d_hist_* (nullable) receive (nsteps, n, 3).
+// Direct-sum runs replay one captured step as a HIP graph after a first eager step
+// (which sizes every buffer); the history slot then comes from a device counter.  The
+// tree's build reads its node count back to the host, so tree runs stay eager.
 static int run_steps(gh_ctx* c, double* x, double* v, const double* m, int64_t np, double dt,
                      double eps, int algorithm, double theta, int precision, int64_t nsteps,
                      double* hist_p, double* hist_v, hipStream_t s)
@@ -140,25 +167,44 @@ static int run_steps(gh_ctx* c, double* x, double* v, const double* m, int64_t n
     const int64_t n3 = 3 * np;
     GH_TRY(ensure(c, c->xh, sizeof(double) * (size_t)n3));
     GH_TRY(ensure(c, c->acc64, sizeof(double) * (size_t)n3));
-    double* xh = c->xh.as<double>();
-    double* acc = c->acc64.as<double>();
-    GH_TRY(half_drift(c, x, v, np, dt, xh, s));
-    for (int64_t st = 0; st < nsteps; st++) {
-        if (np > 1) {
-            if (algorithm == GH_DIRECT)
-                GH_TRY(direct_dev(c, xh, m, np, nullptr, np, eps, precision, acc, s));
-            else
-                GH_TRY(tree_dev(c, xh, m, np, nullptr, np, eps, theta, precision, acc, s));
-        } else {
-            // gravhopper.py:442,449-450: a single particle feels no N-body force
-            GH_TRY(zero_dev(c, acc, n3, s));
-        }
-        double* hp = hist_p ? hist_p + st * n3 : nullptr;
-        double* hv = hist_v ? hist_v + st * n3 : nullptr;
-        TimedScope ts(c, s, GH_T_UPDATE);
-        GH_TRY(kick_drift(c, acc, np, dt, xh, v, x, hp, hv, s));
+    GH_TRY(half_drift(c, x, v, np, dt, c->xh.as<double>(), s));
+    GH_TRY(one_step(c, x, v, m, np, dt, eps, algorithm, theta, precision, hist_p, hist_v,
+                    nullptr, s));
+    const bool graph = algorithm == GH_DIRECT && !c->timing && nsteps >= 4 &&
+                       c->tune[GH_TUNE_RUN_GRAPH] >= 0;
+    if (!graph) {
+        for (int64_t st = 1; st < nsteps; st++)
+            GH_TRY(one_step(c, x, v, m, np, dt, eps, algorithm, theta, precision,
+                            hist_p ? hist_p + st * n3 : nullptr,
+                            hist_v ? hist_v + st * n3 : nullptr, nullptr, s));
+        return 0;
     }
-    return 0;
+    GH_TRY(ensure(c, c->slot, 256));
+    long long* slot = c->slot.as<long long>();
+    GH_TRY(set_slot(c, slot, 1, 0, s));
+    GH_HIP(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int rc = one_step(c, x, v, m, np, dt, eps, algorithm, theta, precision, hist_p, hist_v,
+                      slot, s);
+    if (!rc) rc = set_slot(c, slot, 1, 1, s);
+    hipGraph_t g = nullptr;
+    const hipError_t ee = hipStreamEndCapture(s, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (ee != hipSuccess)
+        return fail(c, GH_EHIP, "step capture failed: %s", hipGetErrorString(ee));
+    hipGraphExec_t ex = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ie != hipSuccess)
+        return fail(c, GH_EHIP, "step graph instantiation failed: %s", hipGetErrorString(ie));
+    for (int64_t st = 1; st < nsteps && !rc; st++)
+        if (hipGraphLaunch(ex, s) != hipSuccess) rc = fail(c, GH_EHIP, "graph launch failed");
+    // the executable graph may be destroyed once its launches are enqueued
+    (void)hipStreamSynchronize(s);
+    (void)hipGraphExecDestroy(ex);
+    return rc;
 }
 
 }  // namespace gh
@@ -203,7 +249,7 @@ void gh_ctx_destroy(gh_ctx* c)
                       &c->src32a, &c->src32b, &c->tgt32,  &c->xs,    &c->vs,
                       &c->xh,     &c->hist_p, &c->hist_v, &c->scratch, &c->counters,
                       &c->work,   &c->eps_blk, &c->adj_work, &c->adj_src, &c->adj_mu,
-                      &c->adj_mv, &c->adj_part, &c->adj_seg};
+                      &c->adj_mv, &c->adj_part, &c->adj_seg, &c->slot};
     for (DevBuf* b : bufs) free_buf(*b);
     release_spans(c);
     for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);

@@ -48,6 +48,7 @@ struct gh_ctx {
     // adjoint workspaces (gh_adjoint.hip)
     gh::DevBuf adj_work, adj_src, adj_mu, adj_mv, adj_part, adj_seg;
     int adj_nb = -1, adj_ipt = -1;
+    gh::DevBuf slot;  // history slot counter of graph-replayed runs
     int64_t last_tree_nodes = 0;
     int last_tree_depth = 0;
     // kernel timing (gh_timing_enable / gh_timing_read)

@@ -21,16 +21,24 @@ __global__ __launch_bounds__(kBlock) void k_half_drift(const double* __restrict_
     if (k < n3) xh[k] = x[k] + 0.5 * v[k] * dt * kDriftScale;
 }
 
+// hist_p / hist_v (nullable): snapshot slot of this step, or, when slot != nullptr,
+// the base of the history, the slot index being read from *slot (graph replay).
 __global__ __launch_bounds__(kBlock) void k_kick_drift(const double* __restrict__ acc,
                                                        int64_t n3, double dt,
                                                        double* __restrict__ xh,
                                                        double* __restrict__ v,
                                                        double* __restrict__ x,
                                                        double* __restrict__ hist_p,
-                                                       double* __restrict__ hist_v)
+                                                       double* __restrict__ hist_v,
+                                                       const long long* __restrict__ slot)
 {
     const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (k >= n3) return;
+    if (slot) {
+        const int64_t off = (int64_t)(*slot) * n3;
+        if (hist_p) hist_p += off;
+        if (hist_v) hist_v += off;
+    }
     const double a = acc[k] * kAccelScale + 0.0;
     const double vn = v[k] + a * dt;
     const double inc = 0.5 * vn * dt * kDriftScale;
@@ -67,12 +75,25 @@ int half_drift(gh_ctx* c, const double* x, const double* v, int64_t n, double dt
 }
 
 int kick_drift(gh_ctx* c, const double* acc, int64_t n, double dt, double* xh, double* v,
-               double* x, double* hp, double* hv, hipStream_t s)
+               double* x, double* hp, double* hv, hipStream_t s, const long long* slot)
 {
     if (n <= 0) return 0;
     hipLaunchKernelGGL(k_kick_drift, dim3(blocks_for(3 * n, kBlock)), dim3(kBlock), 0, s, acc,
-                       3 * n, dt, xh, v, x, hp, hv);
+                       3 * n, dt, xh, v, x, hp, hv, slot);
     return check_launch(c, "k_kick_drift");
+}
+
+__global__ void k_set_slot(long long* __restrict__ slot, long long value, int add)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *slot = add ? *slot + value : value;
+}
+
+// *slot = value (add == 0) or *slot += value (add != 0), by a one-thread kernel so
+// that it can sit in a captured graph.
+int set_slot(gh_ctx* c, long long* slot, long long value, int add, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_set_slot, dim3(1), dim3(64), 0, s, slot, value, add);
+    return check_launch(c, "k_set_slot");
 }
 
 // ------------------------------------------------------------------------------

@@ -188,11 +188,14 @@ int gh_timing_read(gh_ctx *ctx, int kernel_class, double *total_ms, int64_t *lau
  *   GH_TUNE_DIRECT_IPT   FP32 one-sided kernel: 0 automatic, 1/2/4 targets per lane
  *   GH_TUNE_WALKS_PER_WAVE  FP32 tree walk: 0 automatic, 1 or 2 independent 64-target
  *                        walks per wavefront
+ *   GH_TUNE_RUN_GRAPH    direct-sum gh_run / gh_run_dev: 0 replay a captured step as a
+ *                        HIP graph (from 4 steps on), -1 launch every kernel eagerly
  * Results do not depend on the values beyond FP32 rounding. */
 #define GH_TUNE_DIRECT_SELF 0
 #define GH_TUNE_DIRECT_IPT 1
 #define GH_TUNE_WALKS_PER_WAVE 2
-#define GH_TUNE_COUNT 3
+#define GH_TUNE_RUN_GRAPH 3
+#define GH_TUNE_COUNT 4
 int gh_set_tuning(gh_ctx *ctx, int key, int value);
 
 /* FP64 diagnostics (no reference counterpart): kinetic energy sum 0.5 m v^2

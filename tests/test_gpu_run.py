@@ -90,3 +90,24 @@ def test_simulation_fp32_close_to_fp64(ctx):
     disp = np.linalg.norm(res['fp64'] - ic['pos'], axis=1)
     diff = np.linalg.norm(res['fp32'] - res['fp64'], axis=1)
     assert np.median(diff / disp) < 1e-5
+
+
+@pytest.mark.parametrize("prec", [_native.GH_FP64, _native.GH_FP32])
+def test_run_graph_replay_equals_eager(ctx, prec):
+    """Direct-sum runs replay a captured step as a HIP graph; every replayed step's
+    snapshot must equal the eagerly launched run, over several calls with host work and
+    synchronisation in between."""
+    ic = workloads.plummer(20000 if prec == _native.GH_FP32 else 1500, seed=35)
+    x, v = ic["pos"], ic["vel"]
+    xe, ve = x, v
+    for call in range(3):
+        ctx.set_tuning(_native.TUNE_RUN_GRAPH, 0)
+        x, v, hp, hv = ctx.run(x, v, ic["mass"], 0.002, 1e-5, 7, precision=prec, history=True)
+        ctx.set_tuning(_native.TUNE_RUN_GRAPH, -1)
+        xe, ve, hpe, hve = ctx.run(xe, ve, ic["mass"], 0.002, 1e-5, 7, precision=prec,
+                                   history=True)
+        ctx.set_tuning(_native.TUNE_RUN_GRAPH, 0)
+        assert_bitwise(hp, hpe, "call %d history pos" % call)
+        assert_bitwise(hv, hve, "call %d history vel" % call)
+        assert_bitwise(x, xe)
+        assert_bitwise(v, ve)
