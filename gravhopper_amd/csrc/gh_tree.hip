@@ -625,16 +625,29 @@ __global__ __launch_bounds__(kBlock) void k_fill_nodes(FillArgs a, int64_t n)
 // order[] (nullable), results written to acc[order[k]].
 // Each wavefront runs WPW independent walks (WPW groups of 64 consecutive targets);
 // their node loads are issued together, so one walk's load latency overlaps the others'.
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD b mod 8);
+// xcd_block() renumbers them so that XCD x walks the x-th contiguous eighth of the
+// (Morton-ordered) targets and its L2 holds the nodes of one region of the tree.
+// The grid must have 8 * per_xcd workgroups; numbers >= nblk are idle.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int64_t xcd_block(int64_t per_xcd)
+{
+    return (int64_t)(blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
+}
+
 template <int WPW, bool GUARD>
 __global__ __launch_bounds__(kBlock) void k_walk_f32(const NodeF* __restrict__ nodes,
                                                      const int* __restrict__ skip,
                                                      int64_t n_nodes,
                                                      const double* __restrict__ tgt,
                                                      const u32* __restrict__ order, int64_t nt,
-                                                     float eps2, double* __restrict__ acc)
+                                                     float eps2, double* __restrict__ acc,
+                                                     int64_t per_xcd)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int64_t base = (int64_t)blockIdx.x * kBlock * WPW + (int64_t)wave * 64 * WPW + lane;
+    const int64_t blk = xcd_block(per_xcd);
+    const int64_t base = blk * kBlock * WPW + (int64_t)wave * 64 * WPW + lane;
+    if (blk * kBlock * WPW >= nt) return;  // whole workgroup idle
     int64_t t[WPW];
     float x[WPW], y[WPW], z[WPW];
     double ax[WPW], ay[WPW], az[WPW];
@@ -957,21 +970,23 @@ static int walk(gh_ctx* c, const double* tgt, const u32* order, int64_t nt, doub
     const NodeF* nd = tb->nodes32.as<NodeF>();
     const int* sk = tb->skip.as<int>();
     const int64_t T = tb->n_nodes;
-    const dim3 grid(blocks_for(nt, kBlock * wpw));
+    const int64_t nblk = blocks_for(nt, kBlock * wpw);
+    const int64_t per = (nblk + kXcds - 1) / kXcds;
+    const dim3 grid((unsigned)(per * kXcds));
     if (wpw == 2) {
         if (guard)
             hipLaunchKernelGGL((k_walk_f32<2, true>), grid, dim3(kBlock), 0, s, nd, sk, T, tgt,
-                               order, nt, eps2, d_acc);
+                               order, nt, eps2, d_acc, per);
         else
             hipLaunchKernelGGL((k_walk_f32<2, false>), grid, dim3(kBlock), 0, s, nd, sk, T, tgt,
-                               order, nt, eps2, d_acc);
+                               order, nt, eps2, d_acc, per);
     } else {
         if (guard)
             hipLaunchKernelGGL((k_walk_f32<1, true>), grid, dim3(kBlock), 0, s, nd, sk, T, tgt,
-                               order, nt, eps2, d_acc);
+                               order, nt, eps2, d_acc, per);
         else
             hipLaunchKernelGGL((k_walk_f32<1, false>), grid, dim3(kBlock), 0, s, nd, sk, T, tgt,
-                               order, nt, eps2, d_acc);
+                               order, nt, eps2, d_acc, per);
     }
     return check_launch(c, "k_walk_f32");
 }
