@@ -32,8 +32,9 @@ int ensure(gh_ctx* c, DevBuf& b, size_t bytes)
 {
     if (bytes <= b.cap) return 0;
     if (b.p) {
-        // the buffer may still be read by queued work
-        GH_HIP(c, hipStreamSynchronize(c->stream));
+        // the buffer may still be read by queued work, on the context's stream or on a
+        // stream a _dev caller passed in (growth is rare, so wait for the whole device)
+        GH_HIP(c, hipDeviceSynchronize());
         GH_HIP(c, hipFree(b.p));
         b.p = nullptr;
         b.cap = 0;
