@@ -10,7 +10,8 @@ from .simulation import (Simulation, GravHopperException, UninitializedSimulatio
                          ICException, UnknownAlgorithmException, ExternalPackageException)
 from . import jbgrav as grav
 from . import _jbgrav
+from .ic import IC, force_centers
 
-__all__ = ['Simulation', 'grav', '_jbgrav', 'GravHopperException',
+__all__ = ['Simulation', 'IC', 'force_centers', 'grav', '_jbgrav', 'GravHopperException',
            'UninitializedSimulationException', 'ICException', 'UnknownAlgorithmException',
            'ExternalPackageException', '__version__']
