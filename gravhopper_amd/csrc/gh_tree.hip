@@ -733,9 +733,12 @@ __global__ __launch_bounds__(kBlock) void k_walk_f64(const NodeD* __restrict__ n
                                                      const u32* __restrict__ order, int64_t nt,
                                                      double eps, double theta,
                                                      double* __restrict__ acc,
-                                                     int* __restrict__ counts)
+                                                     int* __restrict__ counts,
+                                                     int64_t per_xcd)
 {
-    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t blk = xcd_block(per_xcd);
+    if (blk * kBlock >= nt) return;  // whole workgroup idle
+    const int64_t k = blk * kBlock + threadIdx.x;
     const bool live = k < nt;
     int64_t t = 0;
     double x = 0.0, y = 0.0, z = 0.0;
@@ -956,9 +959,10 @@ static int walk(gh_ctx* c, const double* tgt, const u32* order, int64_t nt, doub
     if (nt <= 0) return 0;
     TimedScope ts(c, s, GH_T_TREE_WALK);
     if (precision == GH_FP64) {
-        hipLaunchKernelGGL(k_walk_f64, dim3(blocks_for(nt, kBlock)), dim3(kBlock), 0, s,
+        const int64_t per = (blocks_for(nt, kBlock) + kXcds - 1) / kXcds;
+        hipLaunchKernelGGL(k_walk_f64, dim3((unsigned)(per * kXcds)), dim3(kBlock), 0, s,
                            tb->nodes64.as<NodeD>(), tb->skip.as<int>(), tb->leaf.as<int>(),
-                           tb->n_nodes, tgt, order, nt, eps, theta, d_acc, counts);
+                           tb->n_nodes, tgt, order, nt, eps, theta, d_acc, counts, per);
         return check_launch(c, "k_walk_f64");
     }
     const float eps2 = (float)(eps * eps);
