@@ -88,6 +88,39 @@ def cpu_baseline(budget_s=30.0):
                           what, n, len(times), dt, spent)}
 
 
+def cpu_tree_baseline(n_targets=131072):
+    """The reference's own C tree (gravhopper/_jbgrav.c tree_force_position, built from
+    its source into oracle/_ref) on one host core at N = 1048576: one build with a single
+    target, then one build with a bounded sample of targets; the walk time per target is
+    extrapolated to all N targets."""
+    from gravhopper_amd import workloads
+    from oracle import oracle as O
+    ref = O.ref_module()
+    if ref is None:
+        return None
+    try:
+        os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
+    except (AttributeError, OSError):
+        pass
+    g = workloads.galaxy(seed=3)
+    n = len(g["mass"])
+    sub = np.random.default_rng(1).choice(n, n_targets, replace=False)
+    t0 = time.perf_counter()
+    ref.tree_force_position(g["pos"], g["mass"], g["pos"][sub[:1]], 0.05, 0.7)
+    t1 = time.perf_counter()
+    ref.tree_force_position(g["pos"], g["mass"], g["pos"][sub], 0.05, 0.7)
+    t2 = time.perf_counter()
+    build = t1 - t0
+    walk = max((t2 - t1) - build, 0.0) / n_targets * n
+    return {"value": 1.0 / (build + walk), "unit": "tree steps/s", "cores": 1,
+            "kind": "reference",
+            "build_s": build, "walk_s_extrapolated": walk,
+            "sample": "gravhopper/_jbgrav.c tree_force_position (built from the reference "
+                      "source), halo+disk N=%d, theta=0.7: one build, then one build plus "
+                      "a walk over %d random targets; walk time scaled to all %d targets "
+                      "(%.1f s measured)" % (n, n_targets, n, t2 - t0)}
+
+
 def bench_tree(ctx, torch, steps, warmup, lite=False):
     """Barnes-Hut step at N = 1048576: Hernquist halo + exponential disk, theta = 0.7,
     eps = 0.05 kpc, dt = 0.1 Myr, FP32 monopoles, FP64 state."""
@@ -270,7 +303,7 @@ def bench_adjoint(ctx, torch, steps=200, every=20):
     fwd, bwd = (t1 - t0) / steps, (t2 - t1) / steps
     return {"workload": "Plummer N=16384, eps=0.05 pc, dt=0.005 Myr, 200 steps, checkpoint "
                         "every 20, FP32 forces and VJP pairs, FP64 state",
-            "loss": float(loss), "forward_ms_per_step": fwd * 1e3,
+            "loss": float(loss.detach()), "forward_ms_per_step": fwd * 1e3,
             "backward_ms_per_step": bwd * 1e3,
             "adjoint_over_forward": bwd / fwd,
             "vjp_kernel_ms": vjp_ms / max(vjp_n, 1),
@@ -282,7 +315,7 @@ def bench_adjoint(ctx, torch, steps=200, every=20):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=N_DIRECT)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -294,6 +327,10 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=5)
     ap.add_argument("--only-tree", action="store_true",
                     help="run only the N=1M tree steps (for profiling)")
+    ap.add_argument("--only-adjoint", action="store_true",
+                    help="run only the N=16384 adjoint benchmark (for profiling)")
+    ap.add_argument("--only-ensemble", action="store_true",
+                    help="run only the 128-sphere ensemble benchmark (for profiling)")
     ap.add_argument("--tree-lite", action="store_true",
                     help="tree steps without the accuracy and comparison runs")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -321,6 +358,13 @@ def main():
     if args.only_tree:
         print(json.dumps({"tree": bench_tree(ctx, torch, args.tree_steps, 1,
                                              lite=args.tree_lite)}), flush=True)
+        return
+    if args.only_adjoint:
+        print(json.dumps({"adjoint": bench_adjoint(ctx, torch)}), flush=True)
+        return
+    if args.only_ensemble:
+        print(json.dumps({"ensemble": bench_ensemble(ctx, torch, args.ensemble_steps)}),
+              flush=True)
         return
     n = args.n
     ic = workloads.plummer(n, b=1e-3, mass=1e6, seed=SEED_DIRECT)
@@ -408,6 +452,22 @@ def main():
             "energy_rel_drift": abs(e1 - e0) / abs(e0),
             "energy_drift_steps": args.warmup + args.steps,
         })
+        # the same steps from the same initial state with the plain FP64 kernel
+        x64 = torch.tensor(ic["pos"], dtype=torch.float64, device=dev).contiguous()
+        v64 = torch.tensor(ic["vel"], dtype=torch.float64, device=dev).contiguous()
+        torch.cuda.synchronize()
+        ctx.run_dev(x64.data_ptr(), v64.data_ptr(), m.data_ptr(), n, DT_DIRECT, EPS_DIRECT,
+                    args.warmup + args.steps, precision=FP64)
+        ctx.sync()
+        ke2, pe2, _ = ctx.energy(x64.cpu().numpy(), v64.cpu().numpy(), ic["mass"], EPS_DIRECT)
+        e2 = ke2 + G * pe2
+        dx = np.linalg.norm(x64.cpu().numpy() - xs, axis=1)
+        extras.update({
+            "energy_rel_drift_fp64_kernel": abs(e2 - e0) / abs(e0),
+            "final_pos_diff_vs_fp64_kernel_kpc_median": float(np.median(dx)),
+            "final_pos_diff_vs_fp64_kernel_kpc_max": float(dx.max()),
+        })
+        del x64, v64
     if not args.no_extras and not args.no_tree:
         log("bench: tree step at N=1M")
         extras["tree"] = bench_tree(ctx, torch, args.tree_steps, 1)
@@ -421,6 +481,14 @@ def main():
     if not args.no_cpu_baseline:
         log("bench: cpu baseline")
         result["cpu_baseline"] = cpu_baseline()
+        if not args.no_extras and not args.no_tree:
+            log("bench: cpu baseline, reference tree at N=1M")
+            tb = cpu_tree_baseline()
+            if tb is not None:
+                result["cpu_baseline"]["tree"] = tb
+                if "tree" in extras:
+                    extras["tree"]["speedup_over_cpu_reference"] = (
+                        extras["tree"]["steps_per_s"] / tb["value"])
     print(json.dumps(result), flush=True)
 
 

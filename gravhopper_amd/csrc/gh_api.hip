@@ -232,6 +232,10 @@ int gh_ctx_create(int device, gh_ctx** out)
     if (hipSetDevice(device) != hipSuccess) return GH_EHIP;
     gh_ctx* c = new gh_ctx();
     c->device = device;
+    if (hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device) !=
+            hipSuccess ||
+        c->n_cu <= 0)
+        c->n_cu = 256;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return GH_EHIP;

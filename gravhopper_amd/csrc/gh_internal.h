@@ -35,6 +35,7 @@ struct TreeBufs;  // gh_tree.hip
 
 struct gh_ctx {
     int device = 0;
+    int n_cu = 256;  // compute units of the device
     hipStream_t stream = nullptr;
     std::string err;
     // generic arenas

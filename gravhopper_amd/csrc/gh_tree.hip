@@ -21,8 +21,9 @@
 //                     nodes of levels (delta_{i-1}, delta_i] followed by the leaf of i at
 //                     level max(delta_{i-1}, delta_i) + 1, so one scan of
 //                     max(0, delta_i - delta_{i-1}) + 1 numbers the whole tree; a node's
-//                     range ends at the first j >= i with delta_j < level (sparse-table
-//                     query) and its skip pointer is the first node of position end+1
+//                     range ends at the last j whose key shares the node's level count
+//                     of leading digits (exponential search over the sorted keys) and
+//                     its skip pointer is the first node of position end+1
 //   6. moments        node mass and first moment as differences of double-double prefix
 //                     sums over the sorted particles (error far below one FP64 ulp)
 //   7. walk           stackless pre-order walk with skip pointers.  A wavefront walks 64
@@ -55,6 +56,8 @@ struct NodeF {
     float4 geo;
     float4 mon;
 };
+typedef float f32x8 __attribute__((ext_vector_type(8)));  // a NodeF as one load
+static_assert(sizeof(NodeF) == sizeof(f32x8), "NodeF is 32 bytes");
 
 struct DD4 {
     double h[4], l[4];
@@ -63,9 +66,10 @@ struct DD4 {
 struct TreeBufs {
     DevBuf keys[2];  // W words per element, word-major: keys[w * n + i]
     DevBuf idx[2];
-    DevBuf hist, bbox_part, box, flags, delta, off, sparse, prefix, block_sums;
+    DevBuf hist, bbox_part, box, flags, delta, off, prefix, block_sums;
     DevBuf nodes64, nodes32, skip, level, branch, leaf;
     DevBuf tgt_order;
+    DevBuf walk_queue;  // 8 unit counters of the dynamically scheduled FP32 walk
     int64_t n_nodes = 0;
     int words = 0;
     int key_sel = 0;  // which of keys[0]/keys[1] holds the sorted keys
@@ -471,29 +475,51 @@ static int scan_exclusive(gh_ctx* c, const typename Op::T* in, int64_t n, typena
 }
 
 // ------------------------------------------------------------------------------
-// 5./6. sparse table, moments, node records
+// 5./6. node ends, moments, node records
 // ------------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_sparse_level(unsigned char* __restrict__ st,
-                                                         int64_t n, int k)
+// The sorted keys a and b share their first L octant digits (L <= words * 21).
+__host__ __device__ __forceinline__ bool same_prefix(const u64* __restrict__ keys, int64_t n, int64_t a,
+                                            int64_t b, int L)
 {
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const unsigned char* prev = st + (int64_t)(k - 1) * n;
-    const int64_t half = (int64_t)1 << (k - 1);
-    unsigned char v = prev[i];
-    if (i + half < n && prev[i + half] < v) v = prev[i + half];
-    st[(int64_t)k * n + i] = v;
+    int w = 0;
+    for (; L >= kLevelsPerWord; L -= kLevelsPerWord, w++)
+        if (keys[w * n + a] != keys[w * n + b]) return false;
+    if (L == 0) return true;
+    return ((keys[w * n + a] ^ keys[w * n + b]) >> (63 - 3 * L)) == 0;
 }
 
-// first j >= s with dp[j] < lv1 (dp[n-1] = 0 guarantees one exists for lv1 >= 1)
-__device__ __forceinline__ int64_t next_below(const unsigned char* __restrict__ st, int64_t n,
-                                              int K, int64_t s, int lv1)
+// Last sorted position of the node of level L that starts at s: the particles sharing
+// the first L octant digits with s form the contiguous range [s, end].  Exponential
+// search from s (probes s+1, s+2, s+4, s+8 issued together, then the next four powers),
+// then bisection of the bracket, so a node of m particles costs about 2 log2(m) loads
+// and small nodes, the common case, only a few.
+__host__ __device__ __forceinline__ int64_t node_end(const u64* __restrict__ keys, int64_t n,
+                                            int64_t s, int L)
 {
-    int64_t p = s;
-    for (int k = K - 1; k >= 0; k--) {
-        if (p + ((int64_t)1 << k) <= n && st[(int64_t)k * n + p] >= lv1) p += (int64_t)1 << k;
+    int64_t lo = s, hi = n;  // prefix(lo) == prefix(s); hi == n or prefix(hi) differs
+    for (int k = 0; k < 62; k += 4) {
+        int64_t j[4];
+        bool same[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            j[u] = s + ((int64_t)1 << (k + u));
+            same[u] = j[u] < n && same_prefix(keys, n, s, j[u], L);
+        }
+        int u = 0;
+        while (u < 4 && same[u]) lo = j[u++];
+        if (u < 4) {
+            hi = j[u] < n ? j[u] : n;
+            break;
+        }
     }
-    return p;
+    while (hi - lo > 1) {
+        const int64_t mid = lo + ((hi - lo) >> 1);
+        if (same_prefix(keys, n, s, mid, L))
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
 }
 
 __global__ __launch_bounds__(kBlock) void k_moment_inputs(const double* __restrict__ pos,
@@ -525,8 +551,7 @@ struct FillArgs {
     const u32* perm;
     const unsigned char* dp;
     const long long* off;  // n+1 entries
-    const unsigned char* st;
-    int K;
+    const u64* keys;  // sorted keys, word-major
     const double* box;
     const DD4* pref;  // n+1 entries
     double theta;
@@ -604,7 +629,7 @@ __global__ __launch_bounds__(kBlock) void k_fill_nodes(FillArgs a, int64_t n)
     int br = -1;
     for (int lv = 0; lv <= leaf_lv; lv++) {
         if (lv > lprev && lv <= lcur) {
-            const int64_t e = next_below(a.st, n, a.K, i, lv + 1);
+            const int64_t e = node_end(a.keys, n, i, lv);
             emit_node(a, n, base + (lv - lprev - 1), lv, br, cx, cy, cz, size, i, e, false, p);
         }
         if (lv == leaf_lv) {
@@ -621,35 +646,47 @@ __global__ __launch_bounds__(kBlock) void k_fill_nodes(FillArgs a, int64_t n)
 // ------------------------------------------------------------------------------
 // 7. walks
 // ------------------------------------------------------------------------------
-// FP32 monopoles, FP64 accumulation.  Targets: tgt (AoS f64) in the order given by
-// order[] (nullable), results written to acc[order[k]].
-// Each wavefront runs WPW independent walks (WPW groups of 64 consecutive targets);
-// their node loads are issued together, so one walk's load latency overlaps the others'.
-// Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD b mod 8);
-// xcd_block() renumbers them so that XCD x walks the x-th contiguous eighth of the
-// (Morton-ordered) targets and its L2 holds the nodes of one region of the tree.
-// The grid must have 8 * per_xcd workgroups; numbers >= nblk are idle.
+// FP32 walk.  Targets: tgt (AoS f64) in the order given by order[] (nullable), results
+// written to acc[order[k]].
+//
+// A "unit" is 64 * WPW consecutive targets, walked by one wavefront as WPW independent
+// 64-lane walks whose node loads are issued together, so one walk's load latency
+// overlaps the others'.  The node index of a walk is wave-uniform, so node records and
+// skip pointers come through scalar loads.
+//
+// Per visited node, every lane waiting for it applies the opening test (a leaf has
+// geo.w = -1 and is always accepted), the wave moves to the first child if any lane
+// opened the node and past its subtree otherwise.  Accepted monopoles are summed in FP32 (fused multiply-add) and the FP32 partial sums
+// are added into FP64 accumulators every 64 visits and at the end.
+//
+// Scheduling.  Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD
+// b mod 8), and the units are split into 8 contiguous (Morton-ordered) ranges, so the
+// L2 of XCD x holds the nodes of one region of the tree.
+//  - static (DYN = false): xcd_block() renumbers the workgroups so that XCD x takes
+//    range x, one unit per wavefront; the grid has 8 * per_xcd workgroups and numbers
+//    past the last unit are idle;
+//  - dynamic (DYN = true): a wavefront takes units of its XCD's range from the atomic
+//    counter queue[x] until the range is exhausted, then from the other ranges in turn.
+//    Every wavefront ends after at most (units + 8) counter increments in total, so
+//    the grid drains.  queue[0..7] must be zero at launch.
 constexpr int kXcds = 8;
+constexpr int kFlushVisits = 64;
 __device__ __forceinline__ int64_t xcd_block(int64_t per_xcd)
 {
     return (int64_t)(blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
 }
 
 template <int WPW, bool GUARD>
-__global__ __launch_bounds__(kBlock) void k_walk_f32(const NodeF* __restrict__ nodes,
-                                                     const int* __restrict__ skip,
-                                                     int64_t n_nodes,
-                                                     const double* __restrict__ tgt,
-                                                     const u32* __restrict__ order, int64_t nt,
-                                                     float eps2, double* __restrict__ acc,
-                                                     int64_t per_xcd)
+__device__ __forceinline__ void walk_unit_f32(const char* __restrict__ nodes,
+                                              const int* __restrict__ skip, int n_nodes,
+                                              const double* __restrict__ tgt,
+                                              const u32* __restrict__ order, int64_t nt,
+                                              float eps2, double* __restrict__ acc,
+                                              int64_t unit, int lane)
 {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int64_t blk = xcd_block(per_xcd);
-    const int64_t base = blk * kBlock * WPW + (int64_t)wave * 64 * WPW + lane;
-    if (blk * kBlock * WPW >= nt) return;  // whole workgroup idle
+    const int64_t base = unit * 64 * WPW + lane;
     int64_t t[WPW];
-    float x[WPW], y[WPW], z[WPW];
+    float x[WPW], y[WPW], z[WPW], fx[WPW], fy[WPW], fz[WPW];
     double ax[WPW], ay[WPW], az[WPW];
     int my_next[WPW], node[WPW];
 #pragma unroll
@@ -664,63 +701,109 @@ __global__ __launch_bounds__(kBlock) void k_walk_f32(const NodeF* __restrict__ n
             y[w] = (float)tgt[3 * t[w] + 1];
             z[w] = (float)tgt[3 * t[w] + 2];
         }
+        fx[w] = fy[w] = fz[w] = 0.f;
         ax[w] = ay[w] = az[w] = 0.0;
         my_next[w] = live ? 0 : 0x7fffffff;
-        node[w] = 0;
+        // a walk with no live lane never starts
+        node[w] = __builtin_amdgcn_readfirstlane((unit * 64 * WPW + w * 64 < nt) ? 0 : n_nodes);
     }
-    for (;;) {
+    for (int it = 1;; it++) {
         bool more = false;
 #pragma unroll
         for (int w = 0; w < WPW; w++) more = more || node[w] < n_nodes;
         if (!more) break;
-        NodeF nd[WPW];
+        // loads of every walk issued together (a finished walk reloads the last node)
+        // (one 32-byte vector load per record, so that the compiler issues it whole)
+        f32x8 nd[WPW];
         int sk[WPW];
 #pragma unroll
         for (int w = 0; w < WPW; w++) {
-            if (node[w] < n_nodes) {
-                nd[w] = nodes[node[w]];
-                sk[w] = skip[node[w]];
-            }
+            const u32 nw = (u32)(node[w] < n_nodes ? node[w] : n_nodes - 1);
+            nd[w] = *(const f32x8*)(nodes + nw * (u32)sizeof(NodeF));
+            sk[w] = *(const int*)((const char*)skip + nw * (u32)sizeof(int));
         }
 #pragma unroll
         for (int w = 0; w < WPW; w++) {
             if (node[w] >= n_nodes) continue;
+            // opening test; a leaf has geo.w = -1 and is never opened
             const bool active = (my_next[w] == node[w]);
-            bool open = false;
-            if (active) {
-                const float cdx = nd[w].geo.x - x[w], cdy = nd[w].geo.y - y[w],
-                            cdz = nd[w].geo.z - z[w];
-                const float d2 = __builtin_fmaf(cdx, cdx, __builtin_fmaf(cdy, cdy, cdz * cdz));
-                open = !(d2 > nd[w].geo.w);
-                if (!open) {
-                    const float dx = nd[w].mon.x - x[w], dy = nd[w].mon.y - y[w],
-                                dz = nd[w].mon.z - z[w];
-                    float r2 = __builtin_fmaf(dx, dx, eps2);
-                    r2 = __builtin_fmaf(dy, dy, r2);
-                    r2 = __builtin_fmaf(dz, dz, r2);
-                    const float ri = __builtin_amdgcn_rsqf(r2);
-                    float sc = (nd[w].mon.w * ri) * (ri * ri);
-                    if (GUARD) sc = (r2 > 0.f) ? sc : 0.f;
-                    ax[w] += (double)(dx * sc);
-                    ay[w] += (double)(dy * sc);
-                    az[w] += (double)(dz * sc);
-                    my_next[w] = sk[w];
-                } else {
-                    my_next[w] = node[w] + 1;
-                }
+            const float cdx = nd[w][0] - x[w], cdy = nd[w][1] - y[w], cdz = nd[w][2] - z[w];
+            const float d2 = __builtin_fmaf(cdx, cdx, __builtin_fmaf(cdy, cdy, cdz * cdz));
+            const bool open = active && !(d2 > nd[w][3]);
+            const u64 opened = __builtin_amdgcn_ballot_w64(open);
+            // uniform by construction; readfirstlane lets the compiler keep it scalar
+            const int next = __builtin_amdgcn_readfirstlane(opened ? node[w] + 1 : sk[w]);
+            if (open) my_next[w] = node[w] + 1;
+            if (active && !open) {
+                const float dx = nd[w][4] - x[w], dy = nd[w][5] - y[w], dz = nd[w][6] - z[w];
+                float r2 = __builtin_fmaf(dx, dx, eps2);
+                r2 = __builtin_fmaf(dy, dy, r2);
+                r2 = __builtin_fmaf(dz, dz, r2);
+                const float ri = __builtin_amdgcn_rsqf(r2);
+                float sc = (nd[w][7] * ri) * (ri * ri);
+                if (GUARD) sc = (r2 > 0.f) ? sc : 0.f;
+                fx[w] = __builtin_fmaf(dx, sc, fx[w]);
+                fy[w] = __builtin_fmaf(dy, sc, fy[w]);
+                fz[w] = __builtin_fmaf(dz, sc, fz[w]);
+                my_next[w] = sk[w];
             }
-            const bool any_open = __ballot(active && open) != 0ull;
-            node[w] = __builtin_amdgcn_readfirstlane(any_open ? node[w] + 1 : sk[w]);
+            node[w] = next;
+        }
+        if (it % kFlushVisits == 0) {
+#pragma unroll
+            for (int w = 0; w < WPW; w++) {
+                ax[w] += (double)fx[w];
+                ay[w] += (double)fy[w];
+                az[w] += (double)fz[w];
+                fx[w] = fy[w] = fz[w] = 0.f;
+            }
         }
     }
 #pragma unroll
     for (int w = 0; w < WPW; w++) {
         if (base + w * 64 < nt) {
-            acc[3 * t[w]] = ax[w];
-            acc[3 * t[w] + 1] = ay[w];
-            acc[3 * t[w] + 2] = az[w];
+            acc[3 * t[w]] = ax[w] + (double)fx[w];
+            acc[3 * t[w] + 1] = ay[w] + (double)fy[w];
+            acc[3 * t[w] + 2] = az[w] + (double)fz[w];
         }
     }
+}
+
+template <int WPW, bool GUARD, bool DYN>
+__global__ __launch_bounds__(kBlock) void k_walk_f32(const NodeF* __restrict__ nodes,
+                                                     const int* __restrict__ skip, int n_nodes,
+                                                     const double* __restrict__ tgt,
+                                                     const u32* __restrict__ order, int64_t nt,
+                                                     float eps2, double* __restrict__ acc,
+                                                     int64_t n_units, int64_t per_xcd,
+                                                     u32* __restrict__ queue)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const char* nb = (const char*)nodes;
+    if (!DYN) {
+        const int64_t unit = xcd_block(per_xcd) * (kBlock / 64) + wave;
+        if (unit < n_units)
+            walk_unit_f32<WPW, GUARD>(nb, skip, n_nodes, tgt, order, nt, eps2, acc, unit, lane);
+        return;
+    }
+    const int home = (int)(blockIdx.x % kXcds);
+    for (int i = 0; i < kXcds; i++) {
+        const int q = (home + i) % kXcds;
+        const int64_t lo = q * n_units / kXcds, hi = (q + 1) * n_units / kXcds;
+        for (;;) {
+            u32 k = 0;
+            if (lane == 0) k = atomicAdd(&queue[q], 1u);
+            k = __builtin_amdgcn_readfirstlane(k);
+            if (lo + (int64_t)k >= hi) break;
+            walk_unit_f32<WPW, GUARD>(nb, skip, n_nodes, tgt, order, nt, eps2, acc,
+                                      lo + (int64_t)k, lane);
+        }
+    }
+}
+
+__global__ void k_zero_queue(u32* __restrict__ q)
+{
+    if (threadIdx.x < kXcds) q[threadIdx.x] = 0u;
 }
 
 // FP64 in the reference's operation order for the opening test and the monopole
@@ -892,17 +975,6 @@ static int build_tree(gh_ctx* c, const double* d_pos, const double* d_mass, int6
     long long total = 0;
     GH_HIP(c, hipMemcpyAsync(&total, off + n, sizeof(total), hipMemcpyDeviceToHost, s));
 
-    // sparse table over dp (levels of shared prefix)
-    int K = 1;  // windows of up to 2^(K-1) cover any distance < n <= 2^K
-    while (((int64_t)1 << K) < n) K++;
-    GH_TRY(ensure(c, tb->sparse, (size_t)n * K));
-    GH_HIP(c, hipMemcpyAsync(tb->sparse.p, tb->delta.p, (size_t)n, hipMemcpyDeviceToDevice, s));
-    for (int k = 1; k < K; k++) {
-        hipLaunchKernelGGL(k_sparse_level, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s,
-                           tb->sparse.as<unsigned char>(), n, k);
-        GH_TRY(check_launch(c, "k_sparse_level"));
-    }
-
     // 6. moments: exclusive double-double prefix sums of (m, m x, m y, m z)
     GH_TRY(ensure(c, tb->prefix, sizeof(DD4) * (size_t)(2 * n + 1)));
     DD4* vin = tb->prefix.as<DD4>() + (n + 1);
@@ -932,10 +1004,9 @@ static int build_tree(gh_ctx* c, const double* d_pos, const double* d_mass, int6
     fa.pos = d_pos;
     fa.mass = d_mass;
     fa.perm = perm;
-    fa.dp = tb->sparse.as<unsigned char>();
+    fa.dp = tb->delta.as<unsigned char>();
     fa.off = off;
-    fa.st = tb->sparse.as<unsigned char>();
-    fa.K = K;
+    fa.keys = tb->keys[tb->key_sel].as<u64>();
     fa.box = tb->box.as<double>();
     fa.pref = pref;
     fa.theta = theta;
@@ -967,31 +1038,52 @@ static int walk(gh_ctx* c, const double* tgt, const u32* order, int64_t nt, doub
     }
     const float eps2 = (float)(eps * eps);
     const bool guard = !(eps2 >= 1e-30f);
-    // two walks per wavefront once there are more walks than resident wavefronts
-    const int wpw = c->tune[GH_TUNE_WALKS_PER_WAVE] == 1 || c->tune[GH_TUNE_WALKS_PER_WAVE] == 2
-                        ? c->tune[GH_TUNE_WALKS_PER_WAVE]
-                        : (nt >= ((int64_t)1 << 18) ? 2 : 1);
+    // node offsets are 32-bit byte offsets inside the kernel (records below 4 GiB)
+    const int64_t max_nodes = ((int64_t)1 << 32) / (int64_t)sizeof(NodeF);
+    if (tb->n_nodes >= max_nodes)
+        return fail(c, GH_EARG, "FP32 tree walk supports at most %lld nodes (tree has %lld)",
+                    (long long)(max_nodes - 1), (long long)tb->n_nodes);
+    // measured at N = 2^20 (halo + disk): one walk per wavefront and the static schedule
+    // are fastest (1.30 ms; two walks 1.34 ms; dynamic 1.67-1.69 ms, as consecutive
+    // units then land on different CUs and share less of the scalar cache)
+    const int tw = c->tune[GH_TUNE_WALKS_PER_WAVE];
+    const int wpw = (tw == 1 || tw == 2) ? tw : 1;
+    const bool dyn = c->tune[GH_TUNE_WALK_SCHEDULE] > 0;
     const NodeF* nd = tb->nodes32.as<NodeF>();
     const int* sk = tb->skip.as<int>();
-    const int64_t T = tb->n_nodes;
-    const int64_t nblk = blocks_for(nt, kBlock * wpw);
+    const int T = (int)tb->n_nodes;
+    const int waves_per_block = kBlock / 64;
+    const int64_t n_units = (nt + 64 * wpw - 1) / (64 * wpw);
+    const int64_t nblk = (n_units + waves_per_block - 1) / waves_per_block;
     const int64_t per = (nblk + kXcds - 1) / kXcds;
-    const dim3 grid((unsigned)(per * kXcds));
-    if (wpw == 2) {
-        if (guard)
-            hipLaunchKernelGGL((k_walk_f32<2, true>), grid, dim3(kBlock), 0, s, nd, sk, T, tgt,
-                               order, nt, eps2, d_acc, per);
-        else
-            hipLaunchKernelGGL((k_walk_f32<2, false>), grid, dim3(kBlock), 0, s, nd, sk, T, tgt,
-                               order, nt, eps2, d_acc, per);
-    } else {
-        if (guard)
-            hipLaunchKernelGGL((k_walk_f32<1, true>), grid, dim3(kBlock), 0, s, nd, sk, T, tgt,
-                               order, nt, eps2, d_acc, per);
-        else
-            hipLaunchKernelGGL((k_walk_f32<1, false>), grid, dim3(kBlock), 0, s, nd, sk, T, tgt,
-                               order, nt, eps2, d_acc, per);
+    u32* q = nullptr;
+    dim3 grid((unsigned)(per * kXcds));
+    if (dyn) {
+        GH_TRY(ensure(c, tb->walk_queue, sizeof(u32) * kXcds));
+        q = tb->walk_queue.as<u32>();
+        hipLaunchKernelGGL(k_zero_queue, dim3(1), dim3(64), 0, s, q);
+        GH_TRY(check_launch(c, "k_zero_queue"));
+        // one resident wave per SIMD slot: 8 wavefronts on each of the 4 SIMDs of a CU
+        const int64_t resident = (int64_t)c->n_cu * 32 / waves_per_block;
+        grid = dim3((unsigned)(nblk < resident ? nblk : resident));
     }
+#define GH_WALK32(W, G, D)                                                                 \
+    hipLaunchKernelGGL((k_walk_f32<W, G, D>), grid, dim3(kBlock), 0, s, nd, sk, T, tgt,  \
+                       order, nt, eps2, d_acc, n_units, per, q)
+    if (wpw == 2) {
+        if (dyn) {
+            if (guard) GH_WALK32(2, true, true); else GH_WALK32(2, false, true);
+        } else {
+            if (guard) GH_WALK32(2, true, false); else GH_WALK32(2, false, false);
+        }
+    } else {
+        if (dyn) {
+            if (guard) GH_WALK32(1, true, true); else GH_WALK32(1, false, true);
+        } else {
+            if (guard) GH_WALK32(1, true, false); else GH_WALK32(1, false, false);
+        }
+    }
+#undef GH_WALK32
     return check_launch(c, "k_walk_f32");
 }
 
@@ -1019,8 +1111,9 @@ void tree_free(gh_ctx* c)
     TreeBufs* tb = c->tree;
     DevBuf* bufs[] = {&tb->keys[0], &tb->keys[1], &tb->idx[0],    &tb->idx[1],  &tb->hist,
                       &tb->bbox_part, &tb->box,   &tb->flags,     &tb->delta,   &tb->off,
-                      &tb->sparse,  &tb->prefix,  &tb->block_sums, &tb->nodes64, &tb->nodes32,
-                      &tb->skip,    &tb->level,   &tb->branch,    &tb->leaf,    &tb->tgt_order};
+                      &tb->prefix,  &tb->block_sums, &tb->nodes64, &tb->nodes32,
+                      &tb->skip,    &tb->level,   &tb->branch,    &tb->leaf,    &tb->tgt_order,
+                      &tb->walk_queue};
     for (DevBuf* b : bufs)
         if (b->p) (void)hipFree(b->p);
     delete tb;

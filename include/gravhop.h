@@ -190,12 +190,17 @@ int gh_timing_read(gh_ctx *ctx, int kernel_class, double *total_ms, int64_t *lau
  *                        walks per wavefront
  *   GH_TUNE_RUN_GRAPH    direct-sum gh_run / gh_run_dev: 0 replay a captured step as a
  *                        HIP graph (from 4 steps on), -1 launch every kernel eagerly
- * Results do not depend on the values beyond FP32 rounding. */
+ *   GH_TUNE_WALK_SCHEDULE  FP32 tree walk: 0 automatic (static), -1 static (one unit of
+ *                        targets per wavefront), 1 dynamic (wavefronts take units from
+ *                        per-XCD atomic queues)
+ * The direct-sum keys change results only by FP32 rounding; the walk keys and
+ * GH_TUNE_RUN_GRAPH do not change results. */
 #define GH_TUNE_DIRECT_SELF 0
 #define GH_TUNE_DIRECT_IPT 1
 #define GH_TUNE_WALKS_PER_WAVE 2
 #define GH_TUNE_RUN_GRAPH 3
-#define GH_TUNE_COUNT 4
+#define GH_TUNE_WALK_SCHEDULE 4
+#define GH_TUNE_COUNT 5
 int gh_set_tuning(gh_ctx *ctx, int key, int value);
 
 /* FP64 diagnostics (no reference counterpart): kinetic energy sum 0.5 m v^2

@@ -145,12 +145,21 @@ def test_simulation_tree_device_vs_host(ctx):
 
 
 def test_walks_per_wave_bitwise(ctx):
-    # one or two independent walks per wavefront: every target accepts the same nodes
-    # in the same order, so the forces are bit-identical
+    # one or two independent walks per wavefront, static or queue-driven scheduling:
+    # every target accepts the same nodes in the same order, so the forces are
+    # bit-identical
     g = workloads.galaxy(n_halo=20000, n_disk=20000, seed=82)
     out = []
-    for wpw in (1, 2):
-        ctx.set_tuning(_native.TUNE_WALKS_PER_WAVE, wpw)
-        out.append(ctx.tree(g["pos"], g["mass"], 0.05, 0.7, FP32))
-    ctx.set_tuning(_native.TUNE_WALKS_PER_WAVE, 0)
-    assert_bitwise(out[0], out[1])
+    try:
+        for sched in (-1, 1):
+            for wpw in (1, 2):
+                ctx.set_tuning(_native.TUNE_WALK_SCHEDULE, sched)
+                ctx.set_tuning(_native.TUNE_WALKS_PER_WAVE, wpw)
+                out.append(ctx.tree(g["pos"], g["mass"], 0.05, 0.7, FP32))
+                # a second call reuses the queue counters, which must start from zero
+                out.append(ctx.tree(g["pos"], g["mass"], 0.05, 0.7, FP32))
+    finally:
+        ctx.set_tuning(_native.TUNE_WALKS_PER_WAVE, 0)
+        ctx.set_tuning(_native.TUNE_WALK_SCHEDULE, 0)
+    for o in out[1:]:
+        assert_bitwise(out[0], o)
