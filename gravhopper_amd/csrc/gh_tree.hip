@@ -654,9 +654,10 @@ __global__ __launch_bounds__(kBlock) void k_fill_nodes(FillArgs a, int64_t n)
 // overlaps the others'.  The node index of a walk is wave-uniform, so node records and
 // skip pointers come through scalar loads.
 //
-// Per visited node, every lane waiting for it applies the opening test (a leaf has
-// geo.w = -1 and is always accepted), the wave moves to the first child if any lane
-// opened the node and past its subtree otherwise.  Accepted monopoles are summed in FP32 (fused multiply-add) and the FP32 partial sums
+// Per visited node: a leaf (geo.w = -1, tested on the scalar unit) is accepted by every
+// lane waiting for it; an internal node gets the opening test in each waiting lane, and
+// the wave moves to the first child if any lane opened it and past its subtree
+// otherwise.  Accepted monopoles are summed in FP32 (fused multiply-add) and the FP32 partial sums
 // are added into FP64 accumulators every 64 visits and at the end.
 //
 // Scheduling.  Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD
@@ -724,17 +725,29 @@ __device__ __forceinline__ void walk_unit_f32(const char* __restrict__ nodes,
         }
 #pragma unroll
         for (int w = 0; w < WPW; w++) {
-            if (node[w] >= n_nodes) continue;
-            // opening test; a leaf has geo.w = -1 and is never opened
-            const bool active = (my_next[w] == node[w]);
+            if (node[w] >= n_nodes) continue;  // this walk is done
+            // Opening test in every lane waiting for this node (my_next == node; the
+            // other lanes accepted an ancestor).  A leaf has geo.w = -1, so it is
+            // always far, i.e. accepted.  Lane masks come straight from the compares
+            // (predicates 32: equal, 2: ordered greater).
             const float cdx = nd[w][0] - x[w], cdy = nd[w][1] - y[w], cdz = nd[w][2] - z[w];
             const float d2 = __builtin_fmaf(cdx, cdx, __builtin_fmaf(cdy, cdy, cdz * cdz));
-            const bool open = active && !(d2 > nd[w][3]);
-            const u64 opened = __builtin_amdgcn_ballot_w64(open);
+            const u64 waiting = __builtin_amdgcn_uicmp((u32)my_next[w], (u32)node[w], 32);
+            const u64 far = __builtin_amdgcn_fcmpf(d2, nd[w][3], 2);
+            const u64 opened = waiting & ~far;  // the wave descends if any lane opens
+            const bool take = (my_next[w] == node[w]) && (d2 > nd[w][3]);
+            // my_next = node + 1 in the lanes of `opened`: one select on the lane mask
+            {
+                int r;
+                asm volatile("v_cndmask_b32_e64 %0, %1, %2, %3"
+                             : "=v"(r)
+                             : "v"(my_next[w]), "v"(node[w] + 1), "s"(opened));
+                my_next[w] = r;
+            }
+            int next = opened ? node[w] + 1 : sk[w];
             // uniform by construction; readfirstlane lets the compiler keep it scalar
-            const int next = __builtin_amdgcn_readfirstlane(opened ? node[w] + 1 : sk[w]);
-            if (open) my_next[w] = node[w] + 1;
-            if (active && !open) {
+            next = __builtin_amdgcn_readfirstlane(next);
+            if (take) {
                 const float dx = nd[w][4] - x[w], dy = nd[w][5] - y[w], dz = nd[w][6] - z[w];
                 float r2 = __builtin_fmaf(dx, dx, eps2);
                 r2 = __builtin_fmaf(dy, dy, r2);
