@@ -68,8 +68,9 @@ struct TreeBufs {
     DevBuf idx[2];
     DevBuf hist, bbox_part, box, flags, delta, off, prefix, block_sums;
     DevBuf nodes64, nodes32, skip, level, branch, leaf;
-    DevBuf tgt_order;
+    DevBuf tkeys[2], tidx[2];  // Morton order of position targets (tree_force_position)
     DevBuf walk_queue;  // 8 unit counters of the dynamically scheduled FP32 walk
+    long long* readback = nullptr;  // pinned host words: flags[2], node count
     int64_t n_nodes = 0;
     int words = 0;
     int key_sel = 0;  // which of keys[0]/keys[1] holds the sorted keys
@@ -173,9 +174,11 @@ __device__ __forceinline__ int descend(double x, double y, double z, double& cx,
 
 __global__ __launch_bounds__(kBlock) void k_keys(const double* __restrict__ pos, int64_t n,
                                                  const double* __restrict__ box, int words,
-                                                 u64* __restrict__ keys, u32* __restrict__ idx)
+                                                 u64* __restrict__ keys, u32* __restrict__ idx,
+                                                 int* __restrict__ flags)
 {
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (flags && i < 4) flags[i] = 0;
     if (i >= n) return;
     const double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
     double cx = box[0], cy = box[1], cz = box[2];
@@ -672,6 +675,7 @@ __global__ __launch_bounds__(kBlock) void k_fill_nodes(FillArgs a, int64_t n)
 //    the grid drains.  queue[0..7] must be zero at launch.
 constexpr int kXcds = 8;
 constexpr int kFlushVisits = 64;
+constexpr int64_t kSortTargetsMin = 4096;  // position targets sorted from this many on
 __device__ __forceinline__ int64_t xcd_block(int64_t per_xcd)
 {
     return (int64_t)(blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
@@ -892,30 +896,32 @@ __global__ __launch_bounds__(kBlock) void k_walk_f64(const NodeD* __restrict__ n
 // ------------------------------------------------------------------------------
 // host orchestration
 // ------------------------------------------------------------------------------
-static int radix_sort(gh_ctx* c, TreeBufs* tb, int64_t n, int words, hipStream_t s)
+// Stable LSD sort of (keys[0], idx[0]) by the key bits [lo_shift, 64) of every word
+// (word-major, `words` words); returns which of the two buffer pairs holds the result.
+static int radix_sort(gh_ctx* c, DevBuf* keys, DevBuf* idx, DevBuf& hist, int64_t n, int words,
+                      int lo_shift, hipStream_t s, int* sel_out)
 {
     const int nblocks = (int)((n + kRsChunk - 1) / kRsChunk);
-    GH_TRY(ensure(c, tb->hist, sizeof(u32) * 256 * (size_t)(nblocks + 1)));
+    GH_TRY(ensure(c, hist, sizeof(u32) * 256 * (size_t)(nblocks + 1)));
     int sel = 0;
     for (int w = words - 1; w >= 0; w--) {
-        for (int shift = 0; shift < 64; shift += 8) {
-            const u64* kin = tb->keys[sel].as<u64>();
+        for (int shift = lo_shift; shift < 64; shift += 8) {
+            const u64* kin = keys[sel].as<u64>();
             hipLaunchKernelGGL(k_rs_hist, dim3(nblocks), dim3(kBlock), 0, s, kin + (int64_t)w * n,
-                               n, shift, tb->hist.as<u32>(), nblocks);
+                               n, shift, hist.as<u32>(), nblocks);
             GH_TRY(check_launch(c, "k_rs_hist"));
-            hipLaunchKernelGGL(k_rs_scan, dim3(256), dim3(kBlock), 0, s, tb->hist.as<u32>(),
-                               nblocks, tb->hist.as<u32>() + (int64_t)256 * nblocks);
+            hipLaunchKernelGGL(k_rs_scan, dim3(256), dim3(kBlock), 0, s, hist.as<u32>(), nblocks,
+                               hist.as<u32>() + (int64_t)256 * nblocks);
             GH_TRY(check_launch(c, "k_rs_scan"));
             hipLaunchKernelGGL(k_rs_scatter, dim3(nblocks), dim3(kBlock), 0, s, kin,
-                               tb->keys[sel ^ 1].as<u64>(), tb->idx[sel].as<u32>(),
-                               tb->idx[sel ^ 1].as<u32>(), n, words, w, shift,
-                               tb->hist.as<u32>(), nblocks,
-                               tb->hist.as<u32>() + (int64_t)256 * nblocks);
+                               keys[sel ^ 1].as<u64>(), idx[sel].as<u32>(), idx[sel ^ 1].as<u32>(),
+                               n, words, w, shift, hist.as<u32>(), nblocks,
+                               hist.as<u32>() + (int64_t)256 * nblocks);
             GH_TRY(check_launch(c, "k_rs_scatter"));
             sel ^= 1;
         }
     }
-    tb->key_sel = sel;
+    *sel_out = sel;
     return 0;
 }
 
@@ -943,61 +949,68 @@ static int build_tree(gh_ctx* c, const double* d_pos, const double* d_mass, int6
     GH_TRY(ensure(c, tb->idx[1], sizeof(u32) * (size_t)n));
     GH_TRY(ensure(c, tb->delta, (size_t)n));
     GH_TRY(ensure(c, tb->flags, sizeof(int) * 4));
-    int hflags[4] = {0, 0, 0, 0};
-    int words = 1;
-    for (;;) {
+    if (!tb->readback)
+        GH_HIP(c, hipHostMalloc((void**)&tb->readback, 4 * sizeof(long long), hipHostMallocDefault));
+    const int64_t nb = (n + kScanChunk - 1) / kScanChunk;
+    GH_TRY(ensure(c, tb->off, sizeof(long long) * (size_t)(2 * n + 2)));
+    GH_TRY(ensure(c, tb->block_sums, sizeof(DD4) * (size_t)(nb + 1)));
+    GH_TRY(ensure(c, tb->prefix, sizeof(DD4) * (size_t)(2 * n + 1)));
+    long long* cnt = tb->off.as<long long>() + (n + 1);
+    long long* off = tb->off.as<long long>();
+    DD4* vin = tb->prefix.as<DD4>() + (n + 1);
+    DD4* pref = tb->prefix.as<DD4>();
+    long long total = 0;
+    // Keys with `words` words, sort, shared levels, numbering and moments, then one
+    // readback of (flags, node count).  Keys still equal over all words (rare) mean
+    // particles closer than words * 21 levels: the pass is redone with one more word.
+    for (int words = 1;; words++) {
+        if (words > kMaxWords)
+            return fail(c, GH_EDEPTH,
+                        "particles closer than %d octree levels below the root box",
+                        kMaxWords * kLevelsPerWord);
         GH_TRY(ensure(c, tb->keys[0], sizeof(u64) * (size_t)n * words));
         GH_TRY(ensure(c, tb->keys[1], sizeof(u64) * (size_t)n * words));
+        // 2. keys (the first thread also clears the flags read after k_delta)
         hipLaunchKernelGGL(k_keys, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, d_pos, n,
                            tb->box.as<double>(), words, tb->keys[0].as<u64>(),
-                           tb->idx[0].as<u32>());
+                           tb->idx[0].as<u32>(), tb->flags.as<int>());
         GH_TRY(check_launch(c, "k_keys"));
-        GH_TRY(radix_sort(c, tb, n, words, s));
-        GH_HIP(c, hipMemcpyAsync(tb->flags.p, hflags, sizeof(hflags), hipMemcpyHostToDevice, s));
+        // 3. sort
+        GH_TRY(radix_sort(c, tb->keys, tb->idx, tb->hist, n, words, 0, s, &tb->key_sel));
+        // 4. shared levels
         hipLaunchKernelGGL(k_delta, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s,
                            tb->keys[tb->key_sel].as<u64>(), n, words,
                            tb->idx[tb->key_sel].as<u32>(), d_pos, tb->delta.as<unsigned char>(),
                            tb->flags.as<int>());
         GH_TRY(check_launch(c, "k_delta"));
-        int got[4];
-        GH_HIP(c, hipMemcpyAsync(got, tb->flags.p, sizeof(got), hipMemcpyDeviceToHost, s));
+        const u32* perm = tb->idx[tb->key_sel].as<u32>();
+        // 5. numbering
+        hipLaunchKernelGGL(k_node_counts, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s,
+                           tb->delta.as<unsigned char>(), n, cnt);
+        GH_TRY(check_launch(c, "k_node_counts"));
+        GH_TRY(scan_exclusive<OpI64>(c, cnt, n, off, tb->block_sums.as<long long>(), s));
+        // 6. moments: exclusive double-double prefix sums of (m, m x, m y, m z)
+        hipLaunchKernelGGL(k_moment_inputs, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s,
+                           d_pos, d_mass, perm, n, vin);
+        GH_TRY(check_launch(c, "k_moment_inputs"));
+        GH_TRY(scan_exclusive<OpDD4>(c, vin, n, pref, tb->block_sums.as<DD4>(), s));
+        GH_HIP(c, hipMemcpyAsync(tb->readback, tb->flags.p, 2 * sizeof(int),
+                                 hipMemcpyDeviceToHost, s));
+        GH_HIP(c, hipMemcpyAsync(tb->readback + 1, off + n, sizeof(long long),
+                                 hipMemcpyDeviceToHost, s));
         GH_HIP(c, hipStreamSynchronize(s));
+        const int* got = (const int*)tb->readback;
         if (got[1] > 0)
             return fail(c, GH_ECOINCIDENT,
                         "%d pair(s) of particles share a position; the octree cannot "
                         "separate them", got[1]);
-        if (got[0] == 0) break;
-        if (++words > kMaxWords)
-            return fail(c, GH_EDEPTH,
-                        "particles closer than %d octree levels below the root box",
-                        kMaxWords * kLevelsPerWord);
+        if (got[0] == 0) {
+            tb->words = words;
+            total = tb->readback[1];
+            break;
+        }
     }
-    tb->words = words;
     const u32* perm = tb->idx[tb->key_sel].as<u32>();
-
-    // 5. numbering
-    const int64_t nb = (n + kScanChunk - 1) / kScanChunk;
-    GH_TRY(ensure(c, tb->off, sizeof(long long) * (size_t)(2 * n + 2)));
-    GH_TRY(ensure(c, tb->block_sums, sizeof(DD4) * (size_t)(nb + 1)));
-    long long* cnt = tb->off.as<long long>() + (n + 1);
-    long long* off = tb->off.as<long long>();
-    hipLaunchKernelGGL(k_node_counts, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s,
-                       tb->delta.as<unsigned char>(), n, cnt);
-    GH_TRY(check_launch(c, "k_node_counts"));
-    GH_TRY(scan_exclusive<OpI64>(c, cnt, n, off, tb->block_sums.as<long long>(), s));
-    long long total = 0;
-    GH_HIP(c, hipMemcpyAsync(&total, off + n, sizeof(total), hipMemcpyDeviceToHost, s));
-
-    // 6. moments: exclusive double-double prefix sums of (m, m x, m y, m z)
-    GH_TRY(ensure(c, tb->prefix, sizeof(DD4) * (size_t)(2 * n + 1)));
-    DD4* vin = tb->prefix.as<DD4>() + (n + 1);
-    DD4* pref = tb->prefix.as<DD4>();
-    hipLaunchKernelGGL(k_moment_inputs, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, d_pos,
-                       d_mass, perm, n, vin);
-    GH_TRY(check_launch(c, "k_moment_inputs"));
-    GH_TRY(scan_exclusive<OpDD4>(c, vin, n, pref, tb->block_sums.as<DD4>(), s));
-
-    GH_HIP(c, hipStreamSynchronize(s));
     if (total <= 0 || total >= ((long long)1 << 31))
         return fail(c, GH_EDEPTH, "octree of %lld nodes is out of range", total);
     tb->n_nodes = total;
@@ -1036,12 +1049,35 @@ static int build_tree(gh_ctx* c, const double* d_pos, const double* d_mass, int6
     return 0;
 }
 
-static int walk(gh_ctx* c, const double* tgt, const u32* order, int64_t nt, double eps,
-                double theta, int precision, double* d_acc, int* counts, hipStream_t s)
+// Walks nt targets through the tree built last: the particles themselves (self) or the
+// positions tgt.  Targets are walked in octant-path (Morton) order, so that neighbouring
+// lanes take neighbouring targets and a wavefront's paths through the tree nearly
+// coincide: self targets in the particles' sorted order, kSortTargetsMin or more
+// position targets in an order sorted here by the top 40 key bits (13 levels of the
+// particles' root box; targets outside the box fall in its edge cells).  Results are
+// written in the caller's order.
+static int walk(gh_ctx* c, const double* tgt, bool self, int64_t nt, double eps, double theta,
+                int precision, double* d_acc, int* counts, hipStream_t s)
 {
     TreeBufs* tb = c->tree;
     if (nt <= 0) return 0;
     TimedScope ts(c, s, GH_T_TREE_WALK);
+    const u32* order = nullptr;
+    if (self) {
+        order = tb->idx[tb->key_sel].as<u32>();
+    } else if (nt >= kSortTargetsMin) {
+        for (int k = 0; k < 2; k++) {
+            GH_TRY(ensure(c, tb->tkeys[k], sizeof(u64) * (size_t)nt));
+            GH_TRY(ensure(c, tb->tidx[k], sizeof(u32) * (size_t)nt));
+        }
+        hipLaunchKernelGGL(k_keys, dim3(blocks_for(nt, kBlock)), dim3(kBlock), 0, s, tgt, nt,
+                           tb->box.as<double>(), 1, tb->tkeys[0].as<u64>(), tb->tidx[0].as<u32>(),
+                           (int*)nullptr);
+        GH_TRY(check_launch(c, "k_keys"));
+        int sel = 0;
+        GH_TRY(radix_sort(c, tb->tkeys, tb->tidx, tb->hist, nt, 1, 24, s, &sel));
+        order = tb->tidx[sel].as<u32>();
+    }
     if (precision == GH_FP64) {
         const int64_t per = (blocks_for(nt, kBlock) + kXcds - 1) / kXcds;
         hipLaunchKernelGGL(k_walk_f64, dim3((unsigned)(per * kXcds)), dim3(kBlock), 0, s,
@@ -1112,10 +1148,7 @@ int tree_dev(gh_ctx* c, const double* d_pos, const double* d_mass, int64_t np,
         return fail(c, GH_EARG, "unknown precision %d", precision);
     GH_TRY(build_tree(c, d_pos, d_mass, np, eps, theta,
                       precision == GH_FP64 ? kFillF64 : kFillF32, s));
-    // self targets are walked in sorted (octant-path) order: neighbouring lanes take
-    // neighbouring particles, so a wavefront's paths through the tree nearly coincide
-    const u32* order = self ? c->tree->idx[c->tree->key_sel].as<u32>() : nullptr;
-    return walk(c, self ? d_pos : d_fpos, order, nt, eps, theta, precision, d_acc, nullptr, s);
+    return walk(c, self ? d_pos : d_fpos, self, nt, eps, theta, precision, d_acc, nullptr, s);
 }
 
 void tree_free(gh_ctx* c)
@@ -1125,10 +1158,11 @@ void tree_free(gh_ctx* c)
     DevBuf* bufs[] = {&tb->keys[0], &tb->keys[1], &tb->idx[0],    &tb->idx[1],  &tb->hist,
                       &tb->bbox_part, &tb->box,   &tb->flags,     &tb->delta,   &tb->off,
                       &tb->prefix,  &tb->block_sums, &tb->nodes64, &tb->nodes32,
-                      &tb->skip,    &tb->level,   &tb->branch,    &tb->leaf,    &tb->tgt_order,
-                      &tb->walk_queue};
+                      &tb->skip,    &tb->level,   &tb->branch,    &tb->leaf,    &tb->tkeys[0],
+                      &tb->tkeys[1], &tb->tidx[0], &tb->tidx[1],  &tb->walk_queue};
     for (DevBuf* b : bufs)
         if (b->p) (void)hipFree(b->p);
+    if (tb->readback) (void)hipHostFree(tb->readback);
     delete tb;
     c->tree = nullptr;
 }
@@ -1211,8 +1245,7 @@ extern "C" int gh_tree_interactions(gh_ctx* c, const double* pos, const double* 
     }
     GH_TRY(build_tree(c, c->pos64.as<double>(), c->mass64.as<double>(), np, eps, theta,
                       kFillF64, s));
-    const u32* order = self ? c->tree->idx[c->tree->key_sel].as<u32>() : nullptr;
-    GH_TRY(walk(c, tgt, order, nt, eps, theta, GH_FP64, c->acc64.as<double>(),
+    GH_TRY(walk(c, tgt, self, nt, eps, theta, GH_FP64, c->acc64.as<double>(),
                 c->scratch.as<int>(), s));
     std::vector<int> cnt((size_t)(nt > 0 ? nt : 1));
     GH_HIP(c, hipMemcpyAsync(acc, c->acc64.p, sizeof(double) * 3 * (size_t)nt,

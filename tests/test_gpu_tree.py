@@ -16,6 +16,11 @@ pytestmark = pytest.mark.gpu
 
 FP64, FP32 = _native.GH_FP64, _native.GH_FP32
 
+# FP32 walk tolerance against the FP64 reference tree: FP32 monopoles and opening test
+# (a node on the opening boundary may go either way), FP32 partial sums added into FP64
+# every 64 node visits.
+FP32_TREE_MEDIAN = 1e-5
+
 
 def compare_topology(got, want, name):
     assert len(got) == len(want), "%s: %d vs %d nodes" % (name, len(got), len(want))
@@ -84,20 +89,23 @@ def test_fp64_tree_golden(ctx, golden):
                                        err_msg=name + " position")
 
 
-def test_position_walk_counts(ctx, oracle):
+@pytest.mark.parametrize("n_targets", [400, 9000])
+def test_position_walk_counts(ctx, oracle, n_targets):
+    # 9000 targets are walked in the library's own Morton order of the targets (from
+    # 4096 on); results must come back in the caller's order.  Some targets lie
+    # outside the particles' root box.
     rng = np.random.default_rng(91)
     pos = rng.normal(size=(5000, 3))
     mass = rng.uniform(size=5000)
-    fpos = np.concatenate([pos[:100], rng.normal(size=(300, 3)) * 3])
+    fpos = np.concatenate([pos[:100], rng.normal(size=(n_targets - 100, 3)) * 3])
     want, wcnt = oracle.tree(pos, mass, 0.01, 0.7, force_pos=fpos, count=True)
     got, gcnt = ctx.tree_interactions(pos, mass, 0.01, 0.7, force_pos=fpos)
     assert np.array_equal(gcnt, wcnt)
     np.testing.assert_allclose(got, want, rtol=1e-10, atol=1e-12 * np.abs(want).max())
-
-
-# FP32 walk tolerance against the FP64 reference tree: FP32 monopoles and opening test
-# (a node on the opening boundary may go either way), FP64 accumulation.
-FP32_TREE_MEDIAN = 1e-5
+    got = ctx.tree(pos, mass, 0.01, 0.7, FP64, force_pos=fpos)
+    np.testing.assert_allclose(got, want, rtol=1e-10, atol=1e-12 * np.abs(want).max())
+    got32 = ctx.tree(pos, mass, 0.01, 0.7, FP32, force_pos=fpos)
+    assert np.median(rel_err(got32, want)) < FP32_TREE_MEDIAN
 
 
 def test_fp32_walk_accuracy(ctx, oracle):
