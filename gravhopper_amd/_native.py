@@ -22,6 +22,7 @@ GH_EHIP = -2
 GH_ENOMEM = -3
 GH_ECOINCIDENT = -4
 GH_EDEPTH = -5
+GH_ECALLBACK = -6
 
 PRECISIONS = {"fp64": GH_FP64, "fp32": GH_FP32}
 
@@ -30,6 +31,10 @@ _P = ctypes.c_void_p
 _I64 = ctypes.c_int64
 _D = ctypes.c_double
 _I = ctypes.c_int
+# gh_external_fn of gh_run_ext
+EXTERNAL_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int64,
+                               ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                               ctypes.c_double, ctypes.POINTER(ctypes.c_double))
 SIGNATURES = {
     "gh_ctx_create": ([_I, ctypes.POINTER(_P)], _I),
     "gh_ctx_destroy": ([_P], None),
@@ -46,6 +51,8 @@ SIGNATURES = {
     "gh_tree_interactions": ([_P, _P, _P, _I64, _P, _I64, _D, _D, _P, _P], _I),
     "gh_run": ([_P, _P, _P, _P, _I64, _D, _D, _I, _D, _I, _I64, _P, _P], _I),
     "gh_run_dev": ([_P, _P, _P, _P, _I64, _D, _D, _I, _D, _I, _I64, _P, _P, _P], _I),
+    "gh_run_ext": ([_P, _P, _P, _P, _I64, _D, _D, _I, _D, _I, _I64, _D, EXTERNAL_FN, _P, _I,
+                    _P, _P], _I),
     "gh_timing_enable": ([_P, _I], _I),
     "gh_set_tuning": ([_P, _I, _I], _I),
     "gh_ensemble_create": ([_P, _I, _P, _P, _P, _P, _P, _P, _I, _I, ctypes.POINTER(_P)], _I),
@@ -217,6 +224,41 @@ class Context:
         self.check(self.lib.gh_run(self.handle, _ptr(pos), _ptr(vel), _ptr(mass), n,
                                    float(dt), float(eps), int(algorithm), float(theta),
                                    int(precision), int(nsteps), _ptr(hp), _ptr(hv)))
+        return (pos, vel, hp, hv) if history else (pos, vel)
+
+    def run_ext(self, pos, vel, mass, dt, eps, nsteps, external, t0=0.0, pass_velocity=False,
+                algorithm=GH_DIRECT, theta=0.7, precision=GH_FP64, history=False):
+        """gh_run with external accelerations: external(x_half, v, t) -> (n, 3) array in
+        km/s/Myr, called once per step with copies of the half-step positions and (when
+        pass_velocity) the pre-kick velocities; v is None otherwise.  An exception raised
+        by `external` stops the run and is re-raised here."""
+        pos = np.array(pos, dtype=np.float64, order="C", copy=True)
+        vel = np.array(vel, dtype=np.float64, order="C", copy=True)
+        mass = np.ascontiguousarray(mass, dtype=np.float64)
+        n = len(mass)
+        hp = np.empty((nsteps, n, 3)) if history else None
+        hv = np.empty((nsteps, n, 3)) if history else None
+        failure = []
+
+        def trampoline(user, n_, x_ptr, v_ptr, t, acc_ptr):
+            try:
+                xh = np.ctypeslib.as_array(x_ptr, shape=(n_, 3)).copy()
+                vv = np.ctypeslib.as_array(v_ptr, shape=(n_, 3)).copy() if v_ptr else None
+                a = np.asarray(external(xh, vv, t), dtype=np.float64)
+                np.ctypeslib.as_array(acc_ptr, shape=(n_, 3))[...] = a
+                return 0
+            except BaseException as e:  # re-raised after the C call returns
+                failure.append(e)
+                return 1
+
+        cb = EXTERNAL_FN(trampoline)
+        rc = self.lib.gh_run_ext(self.handle, _ptr(pos), _ptr(vel), _ptr(mass), n, float(dt),
+                                 float(eps), int(algorithm), float(theta), int(precision),
+                                 int(nsteps), float(t0), cb, None, 1 if pass_velocity else 0,
+                                 _ptr(hp), _ptr(hv))
+        if failure:
+            raise failure[0]
+        self.check(rc)
         return (pos, vel, hp, hv) if history else (pos, vel)
 
     def run_dev(self, d_pos, d_vel, d_mass, n, dt, eps, nsteps, algorithm=GH_DIRECT,

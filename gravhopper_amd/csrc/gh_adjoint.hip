@@ -27,11 +27,6 @@
 
 namespace gh {
 
-int half_drift(gh_ctx* c, const double* x, const double* v, int64_t n, double dt, double* xh,
-               hipStream_t s);
-int kick_drift(gh_ctx* c, const double* acc, int64_t n, double dt, double* xh, double* v,
-               double* x, double* hp, double* hv, hipStream_t s,
-               const long long* slot = nullptr);
 
 constexpr int kRotNext = 0x134;  // DPP wave_rol:1 -- lane l reads lane l+1 (mod 64)
 

@@ -6,11 +6,6 @@
 
 namespace gh {
 
-int half_drift(gh_ctx* c, const double* x, const double* v, int64_t n, double dt, double* xh,
-               hipStream_t s);
-int kick_drift(gh_ctx* c, const double* acc, int64_t n, double dt, double* xh, double* v,
-               double* x, double* hp, double* hv, hipStream_t s,
-               const long long* slot = nullptr);
 int set_slot(gh_ctx* c, long long* slot, long long value, int add, hipStream_t s);
 int energy_dev(gh_ctx* c, const double* d_pos, const double* d_vel, const double* d_mass,
                int64_t n, double eps, double* d_out5, hipStream_t s);
@@ -208,6 +203,60 @@ static int run_steps(gh_ctx* c, double* x, double* v, const double* m, int64_t n
     return rc;
 }
 
+// nsteps DKD steps on device state with external accelerations from a host callback
+// (gravhopper.py:405-473).  Per step, on the context stream s: the N-body force of the
+// half-step positions, then the kick and drifts with a = F A + ext.  Meanwhile, on a
+// copy stream: x_half (and the pre-kick v when asked) go to pinned host memory, `fn`
+// fills ext there at the kick time t + dt/2, and ext goes back to the device.  Times
+// advance as t <- t + dt, the reference's running sum (gravhopper.py:320).
+static int run_ext_steps(gh_ctx* c, double* x, double* v, const double* m, int64_t np,
+                         double dt, double eps, int algorithm, double theta, int precision,
+                         int64_t nsteps, double* t, gh_external_fn fn, void* user, int pass_v,
+                         double* hist_p, double* hist_v, hipStream_t s)
+{
+    const int64_t n3 = 3 * np;
+    double* xh = c->xh.as<double>();
+    double* acc = c->acc64.as<double>();
+    double* ext = c->ext.as<double>();
+    double* h_xh = c->pinned;
+    double* h_v = c->pinned + n3;
+    double* h_ext = c->pinned + 2 * n3;
+    hipStream_t cs = c->copy_stream;
+    hipEvent_t ready = c->ext_events[0], host = c->ext_events[1], back = c->ext_events[2];
+    const size_t bytes = sizeof(double) * (size_t)n3;
+    GH_TRY(half_drift(c, x, v, np, dt, xh, s));
+    for (int64_t st = 0; st < nsteps; st++) {
+        GH_HIP(c, hipEventRecord(ready, s));
+        GH_HIP(c, hipStreamWaitEvent(cs, ready, 0));
+        GH_HIP(c, hipMemcpyAsync(h_xh, xh, bytes, hipMemcpyDeviceToHost, cs));
+        if (pass_v) GH_HIP(c, hipMemcpyAsync(h_v, v, bytes, hipMemcpyDeviceToHost, cs));
+        GH_HIP(c, hipEventRecord(host, cs));
+        if (np > 1) {
+            if (algorithm == GH_DIRECT)
+                GH_TRY(direct_dev(c, xh, m, np, nullptr, np, eps, precision, acc, s));
+            else
+                GH_TRY(tree_dev(c, xh, m, np, nullptr, np, eps, theta, precision, acc, s));
+        } else {
+            GH_TRY(zero_dev(c, acc, n3, s));
+        }
+        GH_HIP(c, hipEventSynchronize(host));
+        const int rc = fn(user, np, h_xh, pass_v ? h_v : nullptr, *t + 0.5 * dt, h_ext);
+        if (rc != 0)
+            return fail(c, GH_ECALLBACK, "external force callback returned %d at step %lld",
+                        rc, (long long)st);
+        GH_HIP(c, hipMemcpyAsync(ext, h_ext, bytes, hipMemcpyHostToDevice, cs));
+        GH_HIP(c, hipEventRecord(back, cs));
+        GH_HIP(c, hipStreamWaitEvent(s, back, 0));
+        {
+            TimedScope ts(c, s, GH_T_UPDATE);
+            GH_TRY(kick_drift(c, acc, np, dt, xh, v, x, hist_p ? hist_p + st * n3 : nullptr,
+                              hist_v ? hist_v + st * n3 : nullptr, s, nullptr, ext));
+        }
+        *t = *t + dt;
+    }
+    return 0;
+}
+
 }  // namespace gh
 
 using namespace gh;
@@ -256,8 +305,16 @@ void gh_ctx_destroy(gh_ctx* c)
                       &c->work,   &c->eps_blk, &c->adj_work, &c->adj_src, &c->adj_mu,
                       &c->adj_mv, &c->adj_part, &c->adj_seg, &c->slot};
     for (DevBuf* b : bufs) free_buf(*b);
+    free_buf(c->ext);
     release_spans(c);
     for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
+    for (hipEvent_t e : c->ext_events)
+        if (e) (void)hipEventDestroy(e);
+    if (c->copy_stream) {
+        (void)hipStreamSynchronize(c->copy_stream);
+        (void)hipStreamDestroy(c->copy_stream);
+    }
+    if (c->pinned) (void)hipHostFree(c->pinned);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -411,6 +468,79 @@ int gh_run(gh_ctx* c, double* pos, double* vel, const double* mass, int64_t np, 
     }
     GH_HIP(c, hipMemcpyAsync(pos, x, sizeof(double) * (size_t)n3, hipMemcpyDeviceToHost, s));
     GH_HIP(c, hipMemcpyAsync(vel, v, sizeof(double) * (size_t)n3, hipMemcpyDeviceToHost, s));
+    GH_HIP(c, hipStreamSynchronize(s));
+    return 0;
+}
+
+int gh_run_ext(gh_ctx* c, double* pos, double* vel, const double* mass, int64_t np, double dt,
+               double eps, int algorithm, double theta, int precision, int64_t nsteps,
+               double t0, gh_external_fn fn, void* user, int pass_velocity, double* hist_pos,
+               double* hist_vel)
+{
+    GH_TRY(check_common(c, np, 0, eps, precision));
+    if (algorithm != GH_DIRECT && algorithm != GH_TREE)
+        return fail(c, GH_EARG, "unknown algorithm %d", algorithm);
+    if (nsteps < 0) return fail(c, GH_EARG, "negative step count");
+    if (!fn) return fail(c, GH_EARG, "no external force callback");
+    if (np == 0 || nsteps == 0) return 0;
+    GH_HIP(c, hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    const int64_t n3 = 3 * np;
+    const size_t bytes = sizeof(double) * (size_t)n3;
+    if (!c->copy_stream) GH_HIP(c, hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
+    for (hipEvent_t& e : c->ext_events)
+        if (!e) GH_HIP(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    if (c->pinned_bytes < 3 * bytes) {
+        if (c->pinned) {
+            GH_HIP(c, hipStreamSynchronize(c->copy_stream));
+            GH_HIP(c, hipHostFree(c->pinned));
+            c->pinned = nullptr;
+            c->pinned_bytes = 0;
+        }
+        GH_HIP(c, hipHostMalloc((void**)&c->pinned, 3 * bytes, hipHostMallocDefault));
+        c->pinned_bytes = 3 * bytes;
+    }
+    GH_TRY(ensure(c, c->xh, bytes));
+    GH_TRY(ensure(c, c->acc64, bytes));
+    GH_TRY(ensure(c, c->ext, bytes));
+    GH_TRY(upload(c, c->xs, pos, n3));
+    GH_TRY(upload(c, c->vs, vel, n3));
+    GH_TRY(upload(c, c->mass64, mass, np));
+    double* x = c->xs.as<double>();
+    double* v = c->vs.as<double>();
+    const double* m = c->mass64.as<double>();
+    // history through a device ring of `chunk` snapshots, as in gh_run
+    const bool hist = hist_pos || hist_vel;
+    int64_t chunk = nsteps;
+    if (hist) {
+        const int64_t cap = (int64_t)((size_t)1 << 31) / (int64_t)bytes;
+        chunk = cap < 1 ? 1 : (cap < nsteps ? cap : nsteps);
+        if (hist_pos) GH_TRY(ensure(c, c->hist_p, bytes * (size_t)chunk));
+        if (hist_vel) GH_TRY(ensure(c, c->hist_v, bytes * (size_t)chunk));
+    }
+    double t = t0;
+    for (int64_t s0 = 0; s0 < nsteps; s0 += chunk) {
+        const int64_t s1 = (s0 + chunk < nsteps) ? s0 + chunk : nsteps;
+        const int rc = run_ext_steps(c, x, v, m, np, dt, eps, algorithm, theta, precision,
+                                     s1 - s0, &t, fn, user, pass_velocity,
+                                     hist_pos ? c->hist_p.as<double>() : nullptr,
+                                     hist_vel ? c->hist_v.as<double>() : nullptr, s);
+        if (rc) {
+            // leave both streams idle before returning the error
+            (void)hipStreamSynchronize(c->copy_stream);
+            (void)hipStreamSynchronize(s);
+            return rc;
+        }
+        if (hist_pos)
+            GH_HIP(c, hipMemcpyAsync(hist_pos + s0 * n3, c->hist_p.p, bytes * (size_t)(s1 - s0),
+                                     hipMemcpyDeviceToHost, s));
+        if (hist_vel)
+            GH_HIP(c, hipMemcpyAsync(hist_vel + s0 * n3, c->hist_v.p, bytes * (size_t)(s1 - s0),
+                                     hipMemcpyDeviceToHost, s));
+        if (hist) GH_HIP(c, hipStreamSynchronize(s));
+    }
+    GH_HIP(c, hipMemcpyAsync(pos, x, bytes, hipMemcpyDeviceToHost, s));
+    GH_HIP(c, hipMemcpyAsync(vel, v, bytes, hipMemcpyDeviceToHost, s));
     GH_HIP(c, hipStreamSynchronize(s));
     return 0;
 }

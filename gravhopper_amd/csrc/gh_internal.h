@@ -50,6 +50,13 @@ struct gh_ctx {
     gh::DevBuf adj_work, adj_src, adj_mu, adj_mv, adj_part, adj_seg;
     int adj_nb = -1, adj_ipt = -1;
     gh::DevBuf slot;  // history slot counter of graph-replayed runs
+    // gh_run_ext: copy stream, pinned host staging (x_half, v, external acceleration)
+    // and the device copy of the external acceleration
+    hipStream_t copy_stream = nullptr;
+    double* pinned = nullptr;
+    size_t pinned_bytes = 0;
+    gh::DevBuf ext;
+    hipEvent_t ext_events[3] = {nullptr, nullptr, nullptr};
     int64_t last_tree_nodes = 0;
     int last_tree_depth = 0;
     // kernel timing (gh_timing_enable / gh_timing_read)
@@ -120,6 +127,15 @@ int reduce_sym(gh_ctx* c, const float* part, int nb, const int* nb_blk, int blk,
                int64_t np_pad, int64_t n, double* acc, hipStream_t s);
 int direct_f32_packed(gh_ctx* c, const float4* src, int64_t np_pad, const float4* tgt,
                       int64_t nt, bool self, double eps, double* d_acc, hipStream_t s);
+
+// leapfrog update (gh_step.hip).  kick_drift: ext (nullable) is an extra acceleration
+// in km/s/Myr added to the N-body one, as a = acc*A + ext; slot (nullable) is the
+// history slot counter of graph-replayed runs.
+int half_drift(gh_ctx* c, const double* x, const double* v, int64_t n, double dt, double* xh,
+               hipStream_t s);
+int kick_drift(gh_ctx* c, const double* acc, int64_t n, double dt, double* xh, double* v,
+               double* x, double* hp, double* hv, hipStream_t s,
+               const long long* slot = nullptr, const double* ext = nullptr);
 
 // tree (gh_tree.hip)
 int tree_dev(gh_ctx* c, const double* d_pos, const double* d_mass, int64_t np,

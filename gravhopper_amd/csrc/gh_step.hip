@@ -23,6 +23,9 @@ __global__ __launch_bounds__(kBlock) void k_half_drift(const double* __restrict_
 
 // hist_p / hist_v (nullable): snapshot slot of this step, or, when slot != nullptr,
 // the base of the history, the slot index being read from *slot (graph replay).
+// ext (nullable): external acceleration in km/s/Myr, added as the reference adds the
+// summed external forces to the N-body acceleration (gravhopper.py:419-459); without
+// external forces the sum is with 0.0, as in the reference.
 __global__ __launch_bounds__(kBlock) void k_kick_drift(const double* __restrict__ acc,
                                                        int64_t n3, double dt,
                                                        double* __restrict__ xh,
@@ -30,7 +33,8 @@ __global__ __launch_bounds__(kBlock) void k_kick_drift(const double* __restrict_
                                                        double* __restrict__ x,
                                                        double* __restrict__ hist_p,
                                                        double* __restrict__ hist_v,
-                                                       const long long* __restrict__ slot)
+                                                       const long long* __restrict__ slot,
+                                                       const double* __restrict__ ext)
 {
     const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (k >= n3) return;
@@ -39,7 +43,7 @@ __global__ __launch_bounds__(kBlock) void k_kick_drift(const double* __restrict_
         if (hist_p) hist_p += off;
         if (hist_v) hist_v += off;
     }
-    const double a = acc[k] * kAccelScale + 0.0;
+    const double a = acc[k] * kAccelScale + (ext ? ext[k] : 0.0);
     const double vn = v[k] + a * dt;
     const double inc = 0.5 * vn * dt * kDriftScale;
     const double xn = xh[k] + inc;
@@ -75,11 +79,12 @@ int half_drift(gh_ctx* c, const double* x, const double* v, int64_t n, double dt
 }
 
 int kick_drift(gh_ctx* c, const double* acc, int64_t n, double dt, double* xh, double* v,
-               double* x, double* hp, double* hv, hipStream_t s, const long long* slot)
+               double* x, double* hp, double* hv, hipStream_t s, const long long* slot,
+               const double* ext)
 {
     if (n <= 0) return 0;
     hipLaunchKernelGGL(k_kick_drift, dim3(blocks_for(3 * n, kBlock)), dim3(kBlock), 0, s, acc,
-                       3 * n, dt, xh, v, x, hp, hv, slot);
+                       3 * n, dt, xh, v, x, hp, hv, slot, ext);
     return check_launch(c, "k_kick_drift");
 }
 

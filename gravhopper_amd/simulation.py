@@ -9,10 +9,14 @@ What changes is where the steps run:
   - without external forces, run(N) hands the whole loop to the GPU (gh_run): the
     state stays in HBM, every step is one force evaluation plus one fused update
     kernel, and the (N+1, Np, 3) history is filled from device snapshots;
-  - with external forces (host Python callbacks), each step computes the N-body
-    force on the GPU and adds the callbacks on the host, exactly as the reference
-    does (gravhopper.py:405-473).
-Both paths give bit-identical trajectories for the same force.
+  - with external forces (host Python callbacks), the state still stays on the GPU
+    (gh_run_ext): each step sends the half-step positions (and, for velocity-dependent
+    forces, the pre-kick velocities) to the host, calls the callbacks there at the
+    reference's kick time while the GPU computes the N-body force, and sends the
+    summed external acceleration back for the kick (gravhopper.py:405-473);
+  - with device_loop=False, external-force runs use a host loop instead: each step
+    calls the GPU force and does the leapfrog in numpy, as the reference does.
+All paths give bit-identical trajectories for the same force.
 
 Units: internally kpc, km/s, Msun, Myr (gravhopper.py:150-154).  With astropy
 installed, Quantities are accepted and the history arrays are Quantities; without it
@@ -82,9 +86,14 @@ class Simulation(object):
         compensated sums, FP64 state)
     theta : opening angle of the tree (the reference always uses 0.7,
         gravhopper.py:446 with jbgrav.py:52)
+    device_loop : with external forces, keep the state on the GPU and call the
+        external-force callbacks once per step from the library's loop (True), or
+        step in a host loop as the reference does (False); same trajectories
     """
 
-    def __init__(self, dt=None, eps=None, algorithm='tree', precision='fp64', theta=0.7):
+    def __init__(self, dt=None, eps=None, algorithm='tree', precision='fp64', theta=0.7,
+                 device_loop=True):
+        self.device_loop = bool(device_loop)
         self.ICarrays = False
         self.Np = 0
         self.Nsnap = 0
@@ -178,13 +187,15 @@ class Simulation(object):
 
         if N <= 0:
             return
-        if self._has_external_forces():
+        if not self._has_external_forces():
+            self._run_on_device(N)
+        elif self.device_loop:
+            self._run_on_device_external(N)
+        else:
             for _ in range(self.timestep, self.timestep + N):
                 self.timestep += 1
                 self.perform_timestep()
                 self._times[self.timestep] = self._times[self.timestep - 1] + self._dt()
-        else:
-            self._run_on_device(N)
 
     def _has_external_forces(self):
         return bool(self.extra_force_functions or self.extra_timedependent_force_functions
@@ -197,6 +208,28 @@ class Simulation(object):
         _, _, hp, hv = _native.default_context().run(
             self._pos[t0], self._vel[t0], self._mass, self._dt(), self._eps(), N,
             algorithm=algo, theta=self.theta, precision=prec, history=True)
+        self._pos[t0 + 1:t0 + N + 1] = hp
+        self._vel[t0 + 1:t0 + N + 1] = hv
+        dt = self._dt()
+        for k in range(t0 + 1, t0 + N + 1):
+            self._times[k] = self._times[k - 1] + dt
+        self.timestep = t0 + N
+
+    def _run_on_device_external(self, N):
+        """N steps of gh_run_ext; the callbacks get what perform_timestep gives them:
+        half-step positions, kick time t + dt/2, pre-kick velocities."""
+        t0 = self.timestep
+        algo = _native.GH_DIRECT if self.params['algorithm'] == 'direct' else _native.GH_TREE
+        prec = _native.PRECISIONS[self.params['precision']]
+        pass_v = bool(self.extra_velocitydependent_force_functions)
+
+        def external(xh, v, t):
+            return self._extra_raw(xh, xh, t, v)
+
+        _, _, hp, hv = _native.default_context().run_ext(
+            self._pos[t0], self._vel[t0], self._mass, self._dt(), self._eps(), N, external,
+            t0=self._times[t0], pass_velocity=pass_v, algorithm=algo, theta=self.theta,
+            precision=prec, history=True)
         self._pos[t0 + 1:t0 + N + 1] = hp
         self._vel[t0 + 1:t0 + N + 1] = hv
         dt = self._dt()
@@ -289,7 +322,8 @@ class Simulation(object):
     def _extra_raw(self, pos, template, time, vel):
         ext = np.zeros_like(template)
         if HAVE_ASTROPY:  # pragma: no cover
-            posq, velq, timeq = pos << u.kpc, vel << (u.km / u.s), time * u.Myr
+            posq, timeq = pos << u.kpc, time * u.Myr
+            velq = None if vel is None else vel << (u.km / u.s)
         else:
             posq, velq, timeq = pos, vel, time
         for fn, args in self.extra_force_functions:

@@ -35,6 +35,7 @@ typedef struct gh_ctx gh_ctx;
 #define GH_ENOMEM -3      /* device allocation failed                                */
 #define GH_ECOINCIDENT -4 /* tree: two particles at the same position                */
 #define GH_EDEPTH -5      /* tree: particles closer than the deepest supported level */
+#define GH_ECALLBACK -6   /* gh_run_ext: the external force callback failed          */
 
 /* precision of the pair arithmetic */
 #define GH_FP64 0 /* IEEE FP64 in the reference's operation order                  */
@@ -126,6 +127,27 @@ int gh_tree_interactions(gh_ctx *ctx, const double *pos, const double *mass, int
 int gh_run(gh_ctx *ctx, double *pos, double *vel, const double *mass, int64_t np,
            double dt, double eps, int algorithm, double theta, int precision,
            int64_t nsteps, double *hist_pos, double *hist_vel);
+
+/* External accelerations for gh_run_ext, called once per step from the calling thread
+ * with host arrays: the half-step positions x_half (n,3, kpc), the velocities before
+ * the kick v (n,3, km/s; NULL unless pass_velocity), and the kick time t (Myr).  Writes
+ * the summed external acceleration in km/s/Myr into acc (n,3).  The arrays are
+ * library staging buffers, valid during the call only.  Returns 0, or nonzero to stop
+ * the run (gh_run_ext then returns GH_ECALLBACK). */
+typedef int (*gh_external_fn)(void *user, int64_t n, const double *x_half, const double *v,
+                              double t, double *acc);
+
+/* gh_run with external accelerations (gravhopper/gravhopper.py:405-473): the kick uses
+ *   a = F(xh) A + ext,  ext = fn(xh, v, t_k + dt/2)
+ * where t_0 = t0 and t_{k+1} = t_k + dt, as Simulation.perform_timestep adds the
+ * position-, time- and velocity-dependent external forces at the half step (the
+ * velocity-dependent ones see the pre-kick velocity).  The state stays on the device;
+ * each step sends x_half (and v) to the host and ext back, overlapped with the N-body
+ * force.  Arguments otherwise as gh_run. */
+int gh_run_ext(gh_ctx *ctx, double *pos, double *vel, const double *mass, int64_t np,
+               double dt, double eps, int algorithm, double theta, int precision,
+               int64_t nsteps, double t0, gh_external_fn fn, void *user, int pass_velocity,
+               double *hist_pos, double *hist_vel);
 
 /* Device-pointer form of gh_run: d_pos/d_vel (np,3) are advanced in place on the
  * device; d_hist_pos/d_hist_vel (nullable) are device arrays of nsteps*(np,3). */

@@ -39,7 +39,7 @@ def test_simulation_device_vs_host_loop_bitwise(ctx):
     a = Simulation(dt=0.005, eps=5e-5, algorithm='direct')
     a.add_IC(ic)
     a.run(6)
-    b = Simulation(dt=0.005, eps=5e-5, algorithm='direct')
+    b = Simulation(dt=0.005, eps=5e-5, algorithm='direct', device_loop=False)
     b.add_IC(ic)
     # a zero external force forces the host-driven loop (gravhopper.py:317-320)
     b.add_external_force(lambda pos, args: np.zeros_like(pos))
@@ -47,6 +47,62 @@ def test_simulation_device_vs_host_loop_bitwise(ctx):
     assert_bitwise(a.positions, b.positions, "positions")
     assert_bitwise(a.velocities, b.velocities, "velocities")
     np.testing.assert_array_equal(a.times, b.times)
+
+
+def _add_external_forces(sim, velocity=True):
+    # position-, time- and velocity-dependent accelerations in km/s/Myr
+    sim.add_external_force(lambda pos, k: -k * pos, 3.0)
+    sim.add_external_timedependent_force(
+        lambda pos, t, a: a * np.sin(0.37 * t) * np.ones_like(pos) + 0.0 * pos, 0.05)
+    if velocity:
+        sim.add_external_velocitydependent_force(lambda pos, vel, g: -g * vel, 0.02)
+
+
+@pytest.mark.parametrize("algorithm", ["direct", "tree"])
+@pytest.mark.parametrize("velocity", [False, True])
+def test_external_forces_device_loop_equals_host_loop(ctx, algorithm, velocity):
+    """gh_run_ext keeps the state on the GPU and calls the callbacks per step; the
+    trajectory must equal the host-driven loop of perform_timestep bit for bit, also
+    across a continued run (kick times from the running time sum)."""
+    ic = workloads.plummer(400, seed=36)
+    sims = []
+    for device_loop in (True, False):
+        s = Simulation(dt=0.004, eps=5e-5, algorithm=algorithm, device_loop=device_loop)
+        s.add_IC(ic)
+        _add_external_forces(s, velocity)
+        s.run(4)
+        s.run(3)
+        sims.append(s)
+    a, b = sims
+    assert_bitwise(a.positions, b.positions, "positions")
+    assert_bitwise(a.velocities, b.velocities, "velocities")
+    np.testing.assert_array_equal(a.times, b.times)
+    # the external forces did act
+    c = Simulation(dt=0.004, eps=5e-5, algorithm=algorithm)
+    c.add_IC(ic)
+    c.run(7)
+    assert not np.array_equal(a.velocities[-1], c.velocities[-1])
+
+
+def test_external_callback_error_propagates(ctx):
+    ic = workloads.plummer(200, seed=37)
+    calls = []
+
+    def bad(pos, args):
+        calls.append(1)
+        if len(calls) == 3:
+            raise ValueError("external force failed")
+        return 0.0 * pos
+
+    sim = Simulation(dt=0.004, eps=5e-5, algorithm='direct')
+    sim.add_IC(ic)
+    sim.add_external_force(bad)
+    with pytest.raises(ValueError, match="external force failed"):
+        sim.run(5)
+    assert len(calls) == 3
+    # the context stays usable
+    x, v = ctx.run(ic["pos"], ic["vel"], ic["mass"], 0.004, 5e-5, 2)
+    assert np.all(np.isfinite(x))
 
 
 def test_simulation_continue_equals_long_run(ctx):

@@ -144,7 +144,7 @@ def test_simulation_tree_device_vs_host(ctx):
     a = Simulation(dt=0.005, eps=5e-5, algorithm='tree')
     a.add_IC(ic)
     a.run(4)
-    b = Simulation(dt=0.005, eps=5e-5, algorithm='tree')
+    b = Simulation(dt=0.005, eps=5e-5, algorithm='tree', device_loop=False)
     b.add_IC(ic)
     b.add_external_force(lambda pos, args: np.zeros_like(pos))
     b.run(4)
