@@ -207,10 +207,15 @@ __global__ __launch_bounds__(kBlock) void k_rs_hist(const u64* __restrict__ key,
     h[threadIdx.x] = 0;
     __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * kRsChunk;
+    u32 d[kRsItems];
+#pragma unroll
     for (int it = 0; it < kRsItems; it++) {
         const int64_t i = base + (int64_t)it * kBlock + threadIdx.x;
-        if (i < n) atomicAdd(&h[(key[i] >> shift) & 255u], 1u);
+        d[it] = i < n ? (u32)((key[i] >> shift) & 255u) : 256u;
     }
+#pragma unroll
+    for (int it = 0; it < kRsItems; it++)
+        if (d[it] < 256u) atomicAdd(&h[d[it]], 1u);
     __syncthreads();
     hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
 }
@@ -241,38 +246,79 @@ __global__ __launch_bounds__(kBlock) void k_rs_scan(u32* __restrict__ hist, int 
     if (threadIdx.x == 0) totals[blockIdx.x] = carry;
 }
 
+// Stable scatter of one 8-bit digit.  Every element's output position is its digit's
+// base (all elements of smaller digits) + the elements of the same digit in earlier
+// blocks (hist, scanned) + its rank inside the block (wave ballots, then waves in
+// order).  STAGE (one key word only): the block's elements are first placed in LDS in
+// output order and then written out by consecutive threads, so each digit's run of the
+// block leaves as contiguous stores; otherwise every element is stored directly.
+template <bool STAGE>
 __global__ __launch_bounds__(kBlock) void k_rs_scatter(
     const u64* __restrict__ kin, u64* __restrict__ kout, const u32* __restrict__ iin,
     u32* __restrict__ iout, int64_t n, int words, int wsel, int shift,
     const u32* __restrict__ hist, int nblocks, const u32* __restrict__ totals)
 {
-    __shared__ u32 gbase[256];
+    __shared__ u32 gbase[256];   // running output position per digit
+    __shared__ u32 gbase0[256];  // this block's first output position per digit
+    __shared__ u32 lstart[256];  // this block's first LDS slot per digit (STAGE)
     __shared__ u32 wcount[kBlock / 64][256];
     __shared__ u32 woff[kBlock / 64][256];
+    __shared__ u64 stage_k[STAGE ? kRsChunk : 1];
+    __shared__ u32 stage_i[STAGE ? kRsChunk : 1];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int dd = threadIdx.x;  // one digit per thread in the per-digit steps
     // digit bases: exclusive scan of the 256 digit totals
     {
-        const u32 tot = totals[threadIdx.x];
-        gbase[threadIdx.x] = tot;
+        const u32 tot = totals[dd];
+        gbase[dd] = tot;
         __syncthreads();
         for (int off = 1; off < 256; off <<= 1) {
-            const u32 t = threadIdx.x >= off ? gbase[threadIdx.x - off] : 0u;
+            const u32 t = dd >= off ? gbase[dd - off] : 0u;
             __syncthreads();
-            gbase[threadIdx.x] += t;
+            gbase[dd] += t;
             __syncthreads();
         }
-        const u32 base = gbase[threadIdx.x] - tot;
+        const u32 first = gbase[dd] - tot + hist[(int64_t)dd * nblocks + blockIdx.x];
         __syncthreads();
-        gbase[threadIdx.x] = base + hist[(int64_t)threadIdx.x * nblocks + blockIdx.x];
+        gbase[dd] = first;
+        gbase0[dd] = first;
     }
-    for (int w = 0; w < kBlock / 64; w++) wcount[w][threadIdx.x] = 0;
-    __syncthreads();
+    if (STAGE) {
+        // this block's count of each digit, then their exclusive scan
+        const u32 h0 = hist[(int64_t)dd * nblocks + blockIdx.x];
+        const u32 h1 = (int)blockIdx.x + 1 < nblocks
+                           ? hist[(int64_t)dd * nblocks + blockIdx.x + 1]
+                           : totals[dd];
+        const u32 cnt = h1 - h0;
+        lstart[dd] = cnt;
+        __syncthreads();
+        for (int off = 1; off < 256; off <<= 1) {
+            const u32 t = dd >= off ? lstart[dd - off] : 0u;
+            __syncthreads();
+            lstart[dd] += t;
+            __syncthreads();
+        }
+        lstart[dd] -= cnt;
+    }
+    for (int w = 0; w < kBlock / 64; w++) wcount[w][dd] = 0;
     const int64_t base = (int64_t)blockIdx.x * kRsChunk;
+    // every item's digit word and index is loaded up front, so that the loads are in
+    // flight together instead of one round trip per ranking step
+    u64 kw[kRsItems];
+    u32 id[kRsItems];
+#pragma unroll
+    for (int it = 0; it < kRsItems; it++) {
+        const int64_t i = base + (int64_t)it * kBlock + threadIdx.x;
+        kw[it] = i < n ? kin[(int64_t)wsel * n + i] : 0ull;
+        id[it] = i < n ? iin[i] : 0u;
+    }
+    __syncthreads();
     const u64 lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
     for (int it = 0; it < kRsItems; it++) {
         const int64_t i = base + (int64_t)it * kBlock + threadIdx.x;
         const bool valid = i < n;
-        const u32 d = valid ? (u32)((kin[(int64_t)wsel * n + i] >> shift) & 255u) : 0u;
+        const u32 d = valid ? (u32)((kw[it] >> shift) & 255u) : 0u;
         u64 peers = __ballot(valid);
         for (int b = 0; b < 8; b++) {
             const u64 m = __ballot((d >> b) & 1u);
@@ -282,7 +328,6 @@ __global__ __launch_bounds__(kBlock) void k_rs_scatter(
         if (valid && rank == 0) wcount[wave][d] = (u32)__popcll(peers);
         __syncthreads();
         {
-            const int dd = threadIdx.x;
             u32 run = gbase[dd];
             for (int w = 0; w < kBlock / 64; w++) {
                 woff[w][dd] = run;
@@ -294,10 +339,28 @@ __global__ __launch_bounds__(kBlock) void k_rs_scatter(
         __syncthreads();
         if (valid) {
             const u32 dst = woff[wave][d] + rank;
-            for (int w = 0; w < words; w++) kout[(int64_t)w * n + dst] = kin[(int64_t)w * n + i];
-            iout[dst] = iin[i];
+            if (STAGE) {
+                const u32 slot = lstart[d] + (dst - gbase0[d]);
+                stage_k[slot] = kw[it];
+                stage_i[slot] = id[it];
+            } else {
+                for (int w = 0; w < words; w++)
+                    kout[(int64_t)w * n + dst] = (w == wsel) ? kw[it] : kin[(int64_t)w * n + i];
+                iout[dst] = id[it];
+            }
         }
         __syncthreads();
+    }
+    if (STAGE) {
+        // LDS slot e holds the block's e-th element in output order
+        const int64_t m = (n - base) < kRsChunk ? (n - base) : kRsChunk;
+        for (int64_t e = threadIdx.x; e < m; e += kBlock) {
+            const u64 k = stage_k[e];
+            const u32 d = (u32)((k >> shift) & 255u);
+            const u32 dst = gbase0[d] + (u32)(e - lstart[d]);
+            kout[dst] = k;
+            iout[dst] = stage_i[e];
+        }
     }
 }
 
@@ -913,10 +976,16 @@ static int radix_sort(gh_ctx* c, DevBuf* keys, DevBuf* idx, DevBuf& hist, int64_
             hipLaunchKernelGGL(k_rs_scan, dim3(256), dim3(kBlock), 0, s, hist.as<u32>(), nblocks,
                                hist.as<u32>() + (int64_t)256 * nblocks);
             GH_TRY(check_launch(c, "k_rs_scan"));
-            hipLaunchKernelGGL(k_rs_scatter, dim3(nblocks), dim3(kBlock), 0, s, kin,
-                               keys[sel ^ 1].as<u64>(), idx[sel].as<u32>(), idx[sel ^ 1].as<u32>(),
-                               n, words, w, shift, hist.as<u32>(), nblocks,
-                               hist.as<u32>() + (int64_t)256 * nblocks);
+            if (words == 1)
+                hipLaunchKernelGGL(k_rs_scatter<true>, dim3(nblocks), dim3(kBlock), 0, s, kin,
+                                   keys[sel ^ 1].as<u64>(), idx[sel].as<u32>(),
+                                   idx[sel ^ 1].as<u32>(), n, words, w, shift, hist.as<u32>(),
+                                   nblocks, hist.as<u32>() + (int64_t)256 * nblocks);
+            else
+                hipLaunchKernelGGL(k_rs_scatter<false>, dim3(nblocks), dim3(kBlock), 0, s, kin,
+                                   keys[sel ^ 1].as<u64>(), idx[sel].as<u32>(),
+                                   idx[sel ^ 1].as<u32>(), n, words, w, shift, hist.as<u32>(),
+                                   nblocks, hist.as<u32>() + (int64_t)256 * nblocks);
             GH_TRY(check_launch(c, "k_rs_scatter"));
             sel ^= 1;
         }
