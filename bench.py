@@ -134,6 +134,8 @@ def bench_tree(ctx, torch, steps, warmup, lite=False):
     m = torch.tensor(g["mass"], dtype=torch.float64, device=dev).contiguous()
     torch.cuda.synchronize()  # library kernels run on the context's own stream
     FP32, FP64, TREE = _native.GH_FP32, _native.GH_FP64, _native.GH_TREE
+    if not lite:
+        ke0, pe0, _ = ctx.energy(g["pos"], g["vel"], g["mass"], eps)
     ctx.run_dev(x.data_ptr(), v.data_ptr(), m.data_ptr(), n, dt, eps, warmup,
                 algorithm=TREE, theta=theta, precision=FP32)
     ctx.sync()
@@ -161,6 +163,12 @@ def bench_tree(ctx, torch, steps, warmup, lite=False):
     # force accuracy on a 16384-particle subsample against the FP64 direct sum
     pos = x.cpu().numpy()
     mass = g["mass"]
+    # total-energy drift over the warmup and timed steps (FP64 pair sums)
+    ke1, pe1, _ = ctx.energy(pos, v.cpu().numpy(), mass, eps)
+    G = 4.398600412921223e-09 / 1.022712165045695e-03  # (km/s)^2 kpc / Msun
+    e0, e1 = ke0 + G * pe0, ke1 + G * pe1
+    out["energy_rel_drift"] = abs(e1 - e0) / abs(e0)
+    out["energy_drift_steps"] = warmup + steps
     rng = np.random.default_rng(16384)
     sub = rng.choice(n, 16384, replace=False)
     acc_t = torch.empty((n, 3), dtype=torch.float64, device=dev)
@@ -180,6 +188,7 @@ def bench_tree(ctx, torch, steps, warmup, lite=False):
     _, cnt = ctx.tree_interactions(pos, mass, eps, theta)
     out["interactions_per_target"] = float(cnt.mean())
     out["walk_achieved_tflops"] = 20.0 * cnt.sum() / (out["walk_ms"] * 1e-3) / 1e12
+    out["walk_roofline_frac"] = out["walk_achieved_tflops"] / PEAK_FP32_TFLOPS
     # opening angle trade-off: step time and force error at smaller theta
     sweep = []
     for th in (0.7, 0.5, 0.4, 0.3):

@@ -10,6 +10,8 @@
 //                 broadcasts, FP32 sums per tile and an FP32 two-sum (hi, lo) across
 //                 tiles; FP64 out.  Sources may be split over grid.y when there are too
 //                 few targets to fill 256 CUs; the splits are added in split order.
+#include <cmath>
+
 #include "gh_internal.h"
 
 namespace gh {
@@ -442,15 +444,30 @@ void launch_f32(bool guard, dim3 grid, hipStream_t s, const float4* src, int64_t
 
 // Symmetric (pair-once) kernel for the self field: particles per lane, or 0 for the
 // one-sided kernel.  Block pairs must number a few thousand to fill 256 CUs.
+// The pair-once kernel keeps one FP32 partial field per block: 12 B x np x (np / block)
+// of scratch, which grows as np^2 (12.9 GB at np = 2^20).  Beyond this size the
+// one-sided kernel (no scratch, 1.45x the VALU work per pair) is used instead.
+constexpr double kSymScratchMax = 16.0 * (1 << 30);
+
 int sym_ipt(gh_ctx* c, int64_t np)
 {
     const int forced = c->tune[GH_TUNE_DIRECT_SELF];
     if (forced < 0) return 0;
-    if (forced == 1 || forced == 2 || forced == 4) return forced;
-    if (np >= 65536) return 4;
-    if (np >= 32768) return 2;
-    if (np >= 16384) return 1;
-    return 0;
+    int ipt = 0;
+    if (forced == 1 || forced == 2 || forced == 4)
+        ipt = forced;
+    else if (np >= 65536)
+        ipt = 4;
+    else if (np >= 32768)
+        ipt = 2;
+    else if (np >= 16384)
+        ipt = 1;
+    if (ipt > 0) {
+        const double blk = (double)kBlock * ipt;
+        const double np_pad = std::ceil((double)np / blk) * blk;
+        if (12.0 * np_pad * (np_pad / blk) > kSymScratchMax) return 0;
+    }
+    return ipt;
 }
 
 template <int IPT>
