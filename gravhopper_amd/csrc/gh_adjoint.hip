@@ -53,6 +53,7 @@ __global__ __launch_bounds__(kBlock) void k_vjp_sym_f32(const float4* __restrict
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t ibase = (int64_t)I * BLK + wave * 64 * IPT + lane;
     const int64_t jbase = (int64_t)J * BLK;
+    eps2 = in_vgpr(eps2);
 
     float xk[IPT], yk[IPT], zk[IPT], mk[IPT], ux[IPT], uy[IPT], uz[IPT];
     float Ax[IPT], Ay[IPT], Az[IPT], B[IPT], Cx[IPT], Cy[IPT], Cz[IPT], L[IPT];

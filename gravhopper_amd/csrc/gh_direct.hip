@@ -137,6 +137,7 @@ __global__ __launch_bounds__(kBlock) void k_direct_f32(const float4* __restrict_
     const int64_t j_begin = (int64_t)blockIdx.y * j_per_split;
     const int64_t j_end = min(np_pad, j_begin + j_per_split);
 
+    eps2 = in_vgpr(eps2);
     float xi[IPT], yi[IPT], zi[IPT];
     float hx[IPT], hy[IPT], hz[IPT], lx[IPT], ly[IPT], lz[IPT];
 #pragma unroll
@@ -235,6 +236,7 @@ __device__ __forceinline__ void sym_item(const float4* __restrict__ src, int64_t
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t ibase = (int64_t)I * BLK + wave * 64 * IPT + lane;
     const int64_t jbase = (int64_t)J * BLK;
+    eps2 = in_vgpr(eps2);
 
     float xi[IPT], yi[IPT], zi[IPT], mi[IPT], ax[IPT], ay[IPT], az[IPT];
 #pragma unroll

@@ -106,6 +106,19 @@ int zero_dev(gh_ctx* c, double* p, int64_t n, hipStream_t s);
 inline hipStream_t pick_stream(gh_ctx* c, void* s) {
     return s ? static_cast<hipStream_t>(s) : c->stream;
 }
+// A wave-uniform value held in a vector register.  On gfx950 a wave64 VALU instruction
+// that reads a scalar register issues about every 4 cycles, one that reads only vector
+// registers or literals about every 2.2 (measured with independent FMA, MUL and SUB
+// chains), so loop-invariant scalars that the pair loops use are copied once into a
+// VGPR.  The empty asm makes the copy opaque to the compiler, which would otherwise
+// fold the scalar back into the instructions.
+__device__ __forceinline__ float in_vgpr(float x)
+{
+    float r;
+    asm volatile("" : "=v"(r) : "0"(x));
+    return r;
+}
+
 inline unsigned blocks_for(int64_t n, int per_block) {
     return static_cast<unsigned>((n + per_block - 1) / per_block);
 }

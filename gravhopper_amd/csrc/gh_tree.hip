@@ -753,6 +753,7 @@ __device__ __forceinline__ void walk_unit_f32(const char* __restrict__ nodes,
                                               int64_t unit, int lane)
 {
     const int64_t base = unit * 64 * WPW + lane;
+    eps2 = in_vgpr(eps2);
     int64_t t[WPW];
     float x[WPW], y[WPW], z[WPW], fx[WPW], fy[WPW], fz[WPW];
     double ax[WPW], ay[WPW], az[WPW];
