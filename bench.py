@@ -210,6 +210,19 @@ def bench_tree(ctx, torch, steps, warmup, lite=False):
                       "force_rel_err_median": float(np.median(e)),
                       "force_rel_err_p99": float(np.percentile(e, 99))})
     out["theta_sweep"] = sweep
+    # the FP64 walk (the default precision of Simulation): the reference's accepted
+    # nodes, FP64 monopoles
+    xs = torch.tensor(pos, dtype=torch.float64, device=dev)
+    vs = torch.tensor(g["vel"], dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    ctx.run_dev(xs.data_ptr(), vs.data_ptr(), m.data_ptr(), n, dt, eps, 1,
+                algorithm=TREE, theta=theta, precision=FP64)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.run_dev(xs.data_ptr(), vs.data_ptr(), m.data_ptr(), n, dt, eps, 3,
+                algorithm=TREE, theta=theta, precision=FP64)
+    ctx.sync()
+    out["fp64_ms_per_step"] = (time.perf_counter() - t0) / 3 * 1e3
     # the FP32 direct sum at the same N, one evaluation
     ctx.timing_enable(True)
     ctx.direct_dev(x.data_ptr(), m.data_ptr(), n, eps, FP32, acc_t.data_ptr())

@@ -887,16 +887,38 @@ __global__ void k_zero_queue(u32* __restrict__ q)
     if (threadIdx.x < kXcds) q[threadIdx.x] = 0u;
 }
 
-// FP64 in the reference's operation order for the opening test and the monopole
-// (gravhopper/_jbgrav.c:496-524), so the accepted node set equals the reference's.
+// FP64 walk: the reference's opening test decided exactly, the monopole to a few ulp.
+//  - Opening test (gravhopper/_jbgrav.c:496-502): the reference accepts an internal
+//    node when fl(size / fl(sqrt(d2))) < theta, with d2 = ((dx^2 + dy^2) + dz^2) formed
+//    in its operation order.  d2 is formed the same way here.  With R2 = (size/theta)^2
+//    the real-number test is d2 > R2, and the two roundings of the reference move q by
+//    less than 3 ulp, so d2 > R2 (1 + 1e-13) accepts and d2 < R2 (1 - 1e-13) opens
+//    exactly as the reference does.  Only a lane between the two bounds (rare) takes the
+//    reference's sqrt and division.  c_lo/c_hi = (1 -/+ 1e-13) / theta^2 (+inf for
+//    theta <= 0: nothing is accepted, as size/dist < theta never holds).
+//  - Leaves are accepted without the test (:502).
+//  - Monopole (:506-524): inv = (1/d)/sqrt(d), d = |com - x|^2 + eps^2, 0 if d == 0, here
+//    as r^3 with r = 1/sqrt(d) from v_rsq_f64 and two Newton steps (a few ulp), which
+//    is far inside the FP64 walk's tolerance against the reference (1e-10), set by
+//    moment rounding and summation order anyway.
+// counts (nullable) receives the accepted-node count per target.
+__device__ __forceinline__ double rsqrt_f64(double d)
+{
+    double r = __builtin_amdgcn_rsq(d);
+    const double h = 0.5 * d;
+#pragma unroll
+    for (int k = 0; k < 2; k++) r = r * __builtin_fma(-h, r * r, 1.5);
+    return r;
+}
+
 __global__ __launch_bounds__(kBlock) void k_walk_f64(const NodeD* __restrict__ nodes,
                                                      const int* __restrict__ skip,
                                                      const int* __restrict__ leaf,
                                                      int64_t n_nodes,
                                                      const double* __restrict__ tgt,
                                                      const u32* __restrict__ order, int64_t nt,
-                                                     double eps, double theta,
-                                                     double* __restrict__ acc,
+                                                     double eps, double theta, double c_lo,
+                                                     double c_hi, double* __restrict__ acc,
                                                      int* __restrict__ counts,
                                                      int64_t per_xcd)
 {
@@ -920,34 +942,43 @@ __global__ __launch_bounds__(kBlock) void k_walk_f64(const NodeD* __restrict__ n
     while (node < n_nodes) {
         const NodeD nd = nodes[node];
         const int sk = skip[node];
-        const bool is_leaf = leaf[node] >= 0;
         const bool active = (my_next == node);
-        bool open = false;
-        if (active) {
-            double dist = 0.0;
-            dist = dist + (nd.cx - x) * (nd.cx - x);
-            dist = dist + (nd.cy - y) * (nd.cy - y);
-            dist = dist + (nd.cz - z) * (nd.cz - z);
-            dist = sqrt(dist);
-            open = !(is_leaf || ((nd.size / dist) < theta));
-            if (!open) {
-                const double dx = nd.mx - x, dy = nd.my - y, dz = nd.mz - z;
-                double d2 = dx * dx;
-                d2 = d2 + dy * dy;
-                d2 = d2 + dz * dz;
-                const double d = d2 + eps2;
-                const double inv = (d == 0.0) ? 0.0 : 1.0 / d / sqrt(d);
-                ax = ax + dx * nd.mass * inv;
-                ay = ay + dy * nd.mass * inv;
-                az = az + dz * nd.mass * inv;
-                cnt++;
-                my_next = sk;
-            } else {
-                my_next = node + 1;
+        bool take = active;
+        int next = node + 1;
+        if (leaf[node] < 0) {  // internal node (a uniform test): opening test
+            const double ddx = nd.cx - x, ddy = nd.cy - y, ddz = nd.cz - z;
+            double d2 = ddx * ddx;
+            d2 = d2 + ddy * ddy;
+            d2 = d2 + ddz * ddz;
+            const double s2 = nd.size * nd.size;
+            bool far = d2 > s2 * c_hi;
+            const bool unsure = active && !far && !(d2 < s2 * c_lo);
+            if (__ballot(unsure) != 0ull) {
+                if (unsure) far = (nd.size / sqrt(d2)) < theta;
             }
+            const bool opens = active && !far;
+            take = active && far;
+            if (opens) my_next = node + 1;
+            next = (__ballot(opens) != 0ull) ? node + 1 : sk;
         }
-        const bool any_open = __ballot(active && open) != 0ull;
-        node = __builtin_amdgcn_readfirstlane(any_open ? node + 1 : sk);
+        if (take) {
+            const double dx = nd.mx - x, dy = nd.my - y, dz = nd.mz - z;
+            double d2 = dx * dx;
+            d2 = d2 + dy * dy;
+            d2 = d2 + dz * dz;
+            const double d = d2 + eps2;
+            double inv = 0.0;
+            if (d != 0.0) {
+                const double r = rsqrt_f64(d);
+                inv = r * r * r;
+            }
+            ax = ax + dx * nd.mass * inv;
+            ay = ay + dy * nd.mass * inv;
+            az = az + dz * nd.mass * inv;
+            cnt++;
+            my_next = sk;
+        }
+        node = __builtin_amdgcn_readfirstlane(next);
     }
     if (live) {
         acc[3 * t] = ax;
@@ -956,6 +987,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_f64(const NodeD* __restrict__ n
         if (counts) counts[t] = cnt;
     }
 }
+
 
 // ------------------------------------------------------------------------------
 // host orchestration
@@ -1150,9 +1182,13 @@ static int walk(gh_ctx* c, const double* tgt, bool self, int64_t nt, double eps,
     }
     if (precision == GH_FP64) {
         const int64_t per = (blocks_for(nt, kBlock) + kXcds - 1) / kXcds;
+        const double inf = __builtin_huge_val();
+        const double c_lo = theta > 0.0 ? (1.0 - 1e-13) / (theta * theta) : inf;
+        const double c_hi = theta > 0.0 ? (1.0 + 1e-13) / (theta * theta) : inf;
         hipLaunchKernelGGL(k_walk_f64, dim3((unsigned)(per * kXcds)), dim3(kBlock), 0, s,
                            tb->nodes64.as<NodeD>(), tb->skip.as<int>(), tb->leaf.as<int>(),
-                           tb->n_nodes, tgt, order, nt, eps, theta, d_acc, counts, per);
+                           tb->n_nodes, tgt, order, nt, eps, theta, c_lo, c_hi, d_acc, counts,
+                           per);
         return check_launch(c, "k_walk_f64");
     }
     const float eps2 = (float)(eps * eps);
