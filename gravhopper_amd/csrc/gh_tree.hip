@@ -67,6 +67,7 @@ struct TreeBufs {
     DevBuf keys[2];  // W words per element, word-major: keys[w * n + i]
     DevBuf idx[2];
     DevBuf hist, bbox_part, box, flags, delta, off, prefix, block_sums;
+    DevBuf spos;  // (x, y, z, m) of the particles in sorted order
     DevBuf nodes64, nodes32, skip, level, branch, leaf;
     DevBuf tkeys[2], tidx[2];  // Morton order of position targets (tree_force_position)
     DevBuf walk_queue;  // 8 unit counters of the dynamically scheduled FP32 walk
@@ -588,22 +589,27 @@ __host__ __device__ __forceinline__ int64_t node_end(const u64* __restrict__ key
     return lo;
 }
 
+// Gathers the particles into sorted order: the moment inputs (m, m x, m y, m z) for the
+// prefix sums, and (x, y, z, m) for the node fill, which then reads them contiguously.
 __global__ __launch_bounds__(kBlock) void k_moment_inputs(const double* __restrict__ pos,
                                                           const double* __restrict__ mass,
                                                           const u32* __restrict__ perm,
-                                                          int64_t n, DD4* __restrict__ v)
+                                                          int64_t n, DD4* __restrict__ v,
+                                                          double4* __restrict__ spos)
 {
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     const u32 p = perm[i];
     const double m = mass[p];
+    const double x = pos[3 * p], y = pos[3 * p + 1], z = pos[3 * p + 2];
     DD4 r;
     r.h[0] = m;
-    r.h[1] = m * pos[3 * p];
-    r.h[2] = m * pos[3 * p + 1];
-    r.h[3] = m * pos[3 * p + 2];
+    r.h[1] = m * x;
+    r.h[2] = m * y;
+    r.h[3] = m * z;
     for (int k = 0; k < 4; k++) r.l[k] = 0.0;
     v[i] = r;
+    spos[i] = make_double4(x, y, z, m);
 }
 
 __device__ __forceinline__ double dd_diff(const DD4& a, const DD4& b, int k)
@@ -612,12 +618,11 @@ __device__ __forceinline__ double dd_diff(const DD4& a, const DD4& b, int k)
 }
 
 struct FillArgs {
-    const double* pos;
-    const double* mass;
     const u32* perm;
     const unsigned char* dp;
     const long long* off;  // n+1 entries
     const u64* keys;  // sorted keys, word-major
+    const double4* spos;  // sorted (x, y, z, m)
     const double* box;
     const DD4* pref;  // n+1 entries
     double theta;
@@ -644,10 +649,11 @@ __device__ void emit_node(const FillArgs& a, int64_t n, int64_t id, int lv, int 
     d.cz = cz;
     d.size = size;
     if (is_leaf) {
-        d.mass = a.mass[p];
-        d.mx = a.pos[3 * p];
-        d.my = a.pos[3 * p + 1];
-        d.mz = a.pos[3 * p + 2];
+        const double4 q = a.spos[s];
+        d.mass = q.w;
+        d.mx = q.x;
+        d.my = q.y;
+        d.mz = q.z;
     } else {
         const DD4 hi = a.pref[e + 1], lo = a.pref[s];
         const double m = dd_diff(hi, lo, 0);
@@ -686,7 +692,8 @@ __global__ __launch_bounds__(kBlock) void k_fill_nodes(FillArgs a, int64_t n)
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     const u32 p = a.perm[i];
-    const double x = a.pos[3 * p], y = a.pos[3 * p + 1], z = a.pos[3 * p + 2];
+    const double4 q = a.spos[i];
+    const double x = q.x, y = q.y, z = q.z;
     const int lprev = (i == 0) ? -1 : (int)a.dp[i - 1] - 1;
     const int lcur = (int)a.dp[i] - 1;
     const int leaf_lv = (lprev > lcur ? lprev : lcur) + 1;
@@ -1057,6 +1064,7 @@ static int build_tree(gh_ctx* c, const double* d_pos, const double* d_mass, int6
     GH_TRY(ensure(c, tb->off, sizeof(long long) * (size_t)(2 * n + 2)));
     GH_TRY(ensure(c, tb->block_sums, sizeof(DD4) * (size_t)(nb + 1)));
     GH_TRY(ensure(c, tb->prefix, sizeof(DD4) * (size_t)(2 * n + 1)));
+    GH_TRY(ensure(c, tb->spos, sizeof(double4) * (size_t)n));
     long long* cnt = tb->off.as<long long>() + (n + 1);
     long long* off = tb->off.as<long long>();
     DD4* vin = tb->prefix.as<DD4>() + (n + 1);
@@ -1093,7 +1101,7 @@ static int build_tree(gh_ctx* c, const double* d_pos, const double* d_mass, int6
         GH_TRY(scan_exclusive<OpI64>(c, cnt, n, off, tb->block_sums.as<long long>(), s));
         // 6. moments: exclusive double-double prefix sums of (m, m x, m y, m z)
         hipLaunchKernelGGL(k_moment_inputs, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s,
-                           d_pos, d_mass, perm, n, vin);
+                           d_pos, d_mass, perm, n, vin, tb->spos.as<double4>());
         GH_TRY(check_launch(c, "k_moment_inputs"));
         GH_TRY(scan_exclusive<OpDD4>(c, vin, n, pref, tb->block_sums.as<DD4>(), s));
         GH_HIP(c, hipMemcpyAsync(tb->readback, tb->flags.p, 2 * sizeof(int),
@@ -1129,12 +1137,11 @@ static int build_tree(gh_ctx* c, const double* d_pos, const double* d_mass, int6
     }
 
     FillArgs fa;
-    fa.pos = d_pos;
-    fa.mass = d_mass;
     fa.perm = perm;
     fa.dp = tb->delta.as<unsigned char>();
     fa.off = off;
     fa.keys = tb->keys[tb->key_sel].as<u64>();
+    fa.spos = tb->spos.as<double4>();
     fa.box = tb->box.as<double>();
     fa.pref = pref;
     fa.theta = theta;
@@ -1263,7 +1270,7 @@ void tree_free(gh_ctx* c)
     TreeBufs* tb = c->tree;
     DevBuf* bufs[] = {&tb->keys[0], &tb->keys[1], &tb->idx[0],    &tb->idx[1],  &tb->hist,
                       &tb->bbox_part, &tb->box,   &tb->flags,     &tb->delta,   &tb->off,
-                      &tb->prefix,  &tb->block_sums, &tb->nodes64, &tb->nodes32,
+                      &tb->prefix,  &tb->spos,      &tb->block_sums, &tb->nodes64, &tb->nodes32,
                       &tb->skip,    &tb->level,   &tb->branch,    &tb->leaf,    &tb->tkeys[0],
                       &tb->tkeys[1], &tb->tidx[0], &tb->tidx[1],  &tb->walk_queue};
     for (DevBuf* b : bufs)
