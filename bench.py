@@ -121,6 +121,42 @@ def cpu_tree_baseline(n_targets=131072):
                       "(%.1f s measured)" % (n, n_targets, n, t2 - t0)}
 
 
+def bench_config1(ctx):
+    """The reference's CPU-runnable case: Plummer N=2000 (b=1 pc, 1e6 Msun), eps=0.05 pc,
+    dt=0.005 Myr, 400 FP64 DKD steps with the direct sum, end to end through gh_run with
+    the full history, as Simulation.run does it."""
+    from gravhopper_amd import _native, workloads
+    ic = workloads.plummer(2000, b=1e-3, mass=1e6, seed=20260001)
+    ctx.run(ic["pos"], ic["vel"], ic["mass"], 0.005, 5e-5, 4, history=True)  # warm
+    t0 = time.perf_counter()
+    ctx.run(ic["pos"], ic["vel"], ic["mass"], 0.005, 5e-5, 400, precision=_native.GH_FP64,
+            history=True)
+    el = time.perf_counter() - t0
+    return {"workload": "Plummer N=2000, eps=0.05 pc, dt=0.005 Myr, 400 FP64 DKD steps, "
+                        "direct sum, history of 400 snapshots",
+            "seconds": el, "ms_per_step": el / 400 * 1e3}
+
+
+def cpu_config1_baseline(n_forces=40):
+    """The reference's C direct sum (built from its source) for the N=2000 case, on one
+    core: n_forces force evaluations, scaled to the run's 400 (the leapfrog itself is a
+    few numpy passes per step, negligible at this N)."""
+    from gravhopper_amd import workloads
+    from oracle import oracle as O
+    ref = O.ref_module()
+    if ref is None:
+        return None
+    ic = workloads.plummer(2000, b=1e-3, mass=1e6, seed=20260001)
+    t0 = time.perf_counter()
+    for _ in range(n_forces):
+        ref.direct_summation(ic["pos"], ic["mass"], 5e-5)
+    el = time.perf_counter() - t0
+    return {"value": el / n_forces * 400, "unit": "s per 400-step run (extrapolated)",
+            "cores": 1, "kind": "reference",
+            "sample": "gravhopper/_jbgrav.c direct_summation, Plummer N=2000, %d force "
+                      "evaluations (%.1f s), scaled to 400" % (n_forces, el)}
+
+
 def bench_tree(ctx, torch, steps, warmup, lite=False):
     """Barnes-Hut step at N = 1048576: Hernquist halo + exponential disk, theta = 0.7,
     eps = 0.05 kpc, dt = 0.1 Myr, FP32 monopoles, FP64 state."""
@@ -490,6 +526,9 @@ def main():
             "final_pos_diff_vs_fp64_kernel_kpc_max": float(dx.max()),
         })
         del x64, v64
+    if not args.no_extras:
+        log("bench: N=2000 FP64 run of 400 steps")
+        extras["config1"] = bench_config1(ctx)
     if not args.no_extras and not args.no_tree:
         log("bench: tree step at N=1M")
         extras["tree"] = bench_tree(ctx, torch, args.tree_steps, 1)
@@ -503,6 +542,14 @@ def main():
     if not args.no_cpu_baseline:
         log("bench: cpu baseline")
         result["cpu_baseline"] = cpu_baseline()
+        if not args.no_extras:
+            log("bench: cpu baseline, reference direct sum at N=2000")
+            c1 = cpu_config1_baseline()
+            if c1 is not None:
+                result["cpu_baseline"]["config1"] = c1
+                if "config1" in extras:
+                    extras["config1"]["speedup_over_cpu_reference"] = (
+                        c1["value"] / extras["config1"]["seconds"])
         if not args.no_extras and not args.no_tree:
             log("bench: cpu baseline, reference tree at N=1M")
             tb = cpu_tree_baseline()
