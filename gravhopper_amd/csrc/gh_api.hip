@@ -31,11 +31,17 @@ int ensure(gh_ctx* c, DevBuf& b, size_t bytes)
         // stream a _dev caller passed in (growth is rare, so wait for the whole device)
         GH_HIP(c, hipDeviceSynchronize());
         GH_HIP(c, hipFree(b.p));
+        c->arena_bytes -= b.cap;
         b.p = nullptr;
         b.cap = 0;
     }
     size_t want = bytes + bytes / 8;  // some headroom for slowly growing inputs
     want = (want + 255) & ~size_t(255);
+    const int limit_mb = c->tune[GH_TUNE_ALLOC_LIMIT_MB];
+    if (limit_mb > 0 && c->arena_bytes + want > ((size_t)limit_mb << 20))
+        return fail(c, GH_ENOMEM,
+                    "device arena limit of %d MiB reached: %zu bytes requested, %zu held",
+                    limit_mb, want, c->arena_bytes);
     hipError_t e = hipMalloc(&b.p, want);
     if (e != hipSuccess) {
         b.p = nullptr;
@@ -43,6 +49,7 @@ int ensure(gh_ctx* c, DevBuf& b, size_t bytes)
                     hipGetErrorString(e));
     }
     b.cap = want;
+    c->arena_bytes += want;
     return 0;
 }
 

@@ -43,6 +43,7 @@ struct gh_ctx {
     gh::DevBuf xs, vs, xh, hist_p, hist_v, scratch, counters;
     gh::TreeBufs* tree = nullptr;
     int tune[GH_TUNE_COUNT] = {0};  // gh_set_tuning values, 0 = automatic
+    size_t arena_bytes = 0;         // device memory held by the context's arenas
     gh::DevBuf work;                // block-pair list of the pair-once direct kernel
     gh::DevBuf eps_blk;             // its per-block squared softening
     int sym_nb = -1, sym_ipt = -1;
