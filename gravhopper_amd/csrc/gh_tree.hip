@@ -22,8 +22,9 @@
 //                     level max(delta_{i-1}, delta_i) + 1, so one scan of
 //                     max(0, delta_i - delta_{i-1}) + 1 numbers the whole tree; a node's
 //                     range ends at the last j whose key shares the node's level count
-//                     of leading digits (exponential search over the sorted keys) and
-//                     its skip pointer is the first node of position end+1
+//                     of leading digits (search over delta and its group minima,
+//                     gh_search.h) and its skip pointer is the first node of position
+//                     end+1
 //   6. moments        node mass and first moment as differences of double-double prefix
 //                     sums over the sorted particles (error far below one FP64 ulp)
 //   7. walk           stackless pre-order walk with skip pointers.  A wavefront walks 64
@@ -35,6 +36,7 @@
 #include <algorithm>
 
 #include "gh_internal.h"
+#include "gh_search.h"
 
 namespace gh {
 
@@ -370,13 +372,20 @@ __global__ __launch_bounds__(kBlock) void k_rs_scatter(
 // ------------------------------------------------------------------------------
 // dp[i] = delta_i + 1 (delta_i = shared leading levels of sorted keys i and i+1), with
 // dp[n-1] = 0; flags[0] counts pairs equal over all words, flags[1] counts those at
-// identical positions (both rare, so their atomics cost nothing).
+// identical positions (both rare, so their atomics cost nothing).  Also the minima of
+// dp over aligned groups of 16 positions (m1) and 256 positions (m2, one per
+// workgroup), which the node fill searches; positions from n on count as 0 there.
+// Each lane's 16-lane group lies in its own wavefront, so m1 is a 16-lane reduction.
+static_assert(kBlock == 256, "k_delta writes one m2 entry per workgroup of 256");
 __global__ __launch_bounds__(kBlock) void k_delta(const u64* __restrict__ keys, int64_t n,
                                                   int words, const u32* __restrict__ perm,
                                                   const double* __restrict__ pos,
                                                   unsigned char* __restrict__ dp,
+                                                  unsigned char* __restrict__ m1,
+                                                  unsigned char* __restrict__ m2,
                                                   int* __restrict__ flags)
 {
+    __shared__ int wave_min[kBlock / 64];
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     int m = 0;  // this lane's delta + 1; 0 for the sentinel and for lanes past the end
     if (i < n - 1) {
@@ -399,6 +408,17 @@ __global__ __launch_bounds__(kBlock) void k_delta(const u64* __restrict__ keys, 
         m = delta + 1;
     }
     if (i < n) dp[i] = (unsigned char)m;
+    int g = m;
+    for (int o = 1; o < 16; o <<= 1) g = min(g, __shfl_xor(g, o));
+    if ((threadIdx.x & 15) == 0) m1[i >> 4] = (unsigned char)g;
+    for (int o = 16; o < 64; o <<= 1) g = min(g, __shfl_xor(g, o));
+    if ((threadIdx.x & 63) == 0) wave_min[threadIdx.x >> 6] = g;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int b = wave_min[0];
+        for (int k = 1; k < kBlock / 64; k++) b = min(b, wave_min[k]);
+        m2[blockIdx.x] = (unsigned char)b;
+    }
 }
 
 // cnt[i] = internal nodes starting at sorted position i, plus its leaf
@@ -425,13 +445,19 @@ __device__ __forceinline__ void dd_add(double& h, double& l, double bh, double b
     l = e - (h - s);
 }
 
+// A scan operation: the element type T, its identity and sum, and the input type In
+// with the function that turns an input element into a T.
 struct OpI64 {
     typedef long long T;
+    typedef long long In;
     __device__ static T zero() { return 0; }
     __device__ static T add(T a, T b) { return a + b; }
+    __device__ static T load(const In* in, int64_t i) { return in[i]; }
 };
+// Moments of sorted particles (x, y, z, m): (m, m x, m y, m z) as double-double.
 struct OpDD4 {
     typedef DD4 T;
+    typedef double4 In;
     __device__ static T zero()
     {
         DD4 z;
@@ -443,6 +469,17 @@ struct OpDD4 {
         for (int k = 0; k < 4; k++) dd_add(a.h[k], a.l[k], b.h[k], b.l[k]);
         return a;
     }
+    __device__ static T load(const In* in, int64_t i)
+    {
+        const double4 q = in[i];
+        DD4 r;
+        r.h[0] = q.w;
+        r.h[1] = q.w * q.x;
+        r.h[2] = q.w * q.y;
+        r.h[3] = q.w * q.z;
+        for (int k = 0; k < 4; k++) r.l[k] = 0.0;
+        return r;
+    }
 };
 
 constexpr int kScanItems = 4;
@@ -451,7 +488,7 @@ constexpr int kScanChunk = kBlock * kScanItems;
 // in[0..n) -> out[0..n] exclusive (out[0] = 0, out[n] = total); phase 1 writes the
 // block-local exclusive prefix and the block totals.
 template <class Op>
-__global__ __launch_bounds__(kBlock) void k_scan_local(const typename Op::T* __restrict__ in,
+__global__ __launch_bounds__(kBlock) void k_scan_local(const typename Op::In* __restrict__ in,
                                                        int64_t n, typename Op::T* __restrict__ out,
                                                        typename Op::T* __restrict__ sums)
 {
@@ -462,7 +499,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_local(const typename Op::T* __r
     T run = Op::zero();
     for (int k = 0; k < kScanItems; k++) {
         v[k] = run;
-        if (b + k < n) run = Op::add(run, in[b + k]);
+        if (b + k < n) run = Op::add(run, Op::load(in, b + k));
     }
     sh[threadIdx.x] = run;
     __syncthreads();
@@ -522,7 +559,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_add(typename Op::T* __restrict_
 
 // out must hold n+1 elements and tmp at least blocks+1 elements
 template <class Op>
-static int scan_exclusive(gh_ctx* c, const typename Op::T* in, int64_t n, typename Op::T* out,
+static int scan_exclusive(gh_ctx* c, const typename Op::In* in, int64_t n, typename Op::T* out,
                           typename Op::T* tmp, hipStream_t s)
 {
     const int64_t nb = (n + kScanChunk - 1) / kScanChunk;
@@ -544,72 +581,37 @@ static int scan_exclusive(gh_ctx* c, const typename Op::T* in, int64_t n, typena
 // ------------------------------------------------------------------------------
 // 5./6. node ends, moments, node records
 // ------------------------------------------------------------------------------
-// The sorted keys a and b share their first L octant digits (L <= words * 21).
-__host__ __device__ __forceinline__ bool same_prefix(const u64* __restrict__ keys, int64_t n, int64_t a,
-                                            int64_t b, int L)
+// Levels 3 and up of the end search, each from the level below, by one workgroup.
+__global__ __launch_bounds__(kBlock) void k_search_levels(unsigned char* __restrict__ base,
+                                                          int64_t n, int levels)
 {
-    int w = 0;
-    for (; L >= kLevelsPerWord; L -= kLevelsPerWord, w++)
-        if (keys[w * n + a] != keys[w * n + b]) return false;
-    if (L == 0) return true;
-    return ((keys[w * n + a] ^ keys[w * n + b]) >> (63 - 3 * L)) == 0;
+    for (int k = 3; k < levels; k++) {
+        const unsigned char* lo = base + search_level_offset(n, k - 1);
+        unsigned char* hi = base + search_level_offset(n, k);
+        const int64_t lo_size = search_level_size(n, k - 1), size = search_level_size(n, k);
+        for (int64_t e = threadIdx.x; e < size; e += kBlock) {
+            int v = 255;
+            for (int j = 0; j < 16; j++) {
+                const int64_t q = 16 * e + j;
+                v = min(v, q < lo_size ? (int)lo[q] : 0);
+            }
+            hi[e] = (unsigned char)v;
+        }
+        __syncthreads();  // the next level reads this one (same workgroup)
+    }
 }
 
-// Last sorted position of the node of level L that starts at s: the particles sharing
-// the first L octant digits with s form the contiguous range [s, end].  Exponential
-// search from s (probes s+1, s+2, s+4, s+8 issued together, then the next four powers),
-// then bisection of the bracket, so a node of m particles costs about 2 log2(m) loads
-// and small nodes, the common case, only a few.
-__host__ __device__ __forceinline__ int64_t node_end(const u64* __restrict__ keys, int64_t n,
-                                            int64_t s, int L)
-{
-    int64_t lo = s, hi = n;  // prefix(lo) == prefix(s); hi == n or prefix(hi) differs
-    for (int k = 0; k < 62; k += 4) {
-        int64_t j[4];
-        bool same[4];
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-            j[u] = s + ((int64_t)1 << (k + u));
-            same[u] = j[u] < n && same_prefix(keys, n, s, j[u], L);
-        }
-        int u = 0;
-        while (u < 4 && same[u]) lo = j[u++];
-        if (u < 4) {
-            hi = j[u] < n ? j[u] : n;
-            break;
-        }
-    }
-    while (hi - lo > 1) {
-        const int64_t mid = lo + ((hi - lo) >> 1);
-        if (same_prefix(keys, n, s, mid, L))
-            lo = mid;
-        else
-            hi = mid;
-    }
-    return lo;
-}
-
-// Gathers the particles into sorted order: the moment inputs (m, m x, m y, m z) for the
-// prefix sums, and (x, y, z, m) for the node fill, which then reads them contiguously.
+// Gathers the particles into sorted order as (x, y, z, m), read contiguously by the
+// moment prefix sums and the node fill.
 __global__ __launch_bounds__(kBlock) void k_moment_inputs(const double* __restrict__ pos,
                                                           const double* __restrict__ mass,
                                                           const u32* __restrict__ perm,
-                                                          int64_t n, DD4* __restrict__ v,
-                                                          double4* __restrict__ spos)
+                                                          int64_t n, double4* __restrict__ spos)
 {
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     const u32 p = perm[i];
-    const double m = mass[p];
-    const double x = pos[3 * p], y = pos[3 * p + 1], z = pos[3 * p + 2];
-    DD4 r;
-    r.h[0] = m;
-    r.h[1] = m * x;
-    r.h[2] = m * y;
-    r.h[3] = m * z;
-    for (int k = 0; k < 4; k++) r.l[k] = 0.0;
-    v[i] = r;
-    spos[i] = make_double4(x, y, z, m);
+    spos[i] = make_double4(pos[3 * p], pos[3 * p + 1], pos[3 * p + 2], mass[p]);
 }
 
 __device__ __forceinline__ double dd_diff(const DD4& a, const DD4& b, int k)
@@ -620,8 +622,8 @@ __device__ __forceinline__ double dd_diff(const DD4& a, const DD4& b, int k)
 struct FillArgs {
     const u32* perm;
     const unsigned char* dp;
+    EndSearch es;
     const long long* off;  // n+1 entries
-    const u64* keys;  // sorted keys, word-major
     const double4* spos;  // sorted (x, y, z, m)
     const double* box;
     const DD4* pref;  // n+1 entries
@@ -687,6 +689,11 @@ __device__ void emit_node(const FillArgs& a, int64_t n, int64_t id, int lv, int 
     a.nodes32[id] = f;
 }
 
+// One thread per sorted particle i: the internal nodes starting at i (levels lprev+1 ..
+// lcur) and the leaf of i (level lcur+1, or lprev+1 when no internal node starts
+// here).  The centre recurrence runs in three phases: down to the first node of i, the
+// internal nodes one level apart, then the leaf, so that the lanes of a wavefront write
+// their k-th node in the same pass rather than one pass per distinct level.
 __global__ __launch_bounds__(kBlock) void k_fill_nodes(FillArgs a, int64_t n)
 {
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -696,24 +703,25 @@ __global__ __launch_bounds__(kBlock) void k_fill_nodes(FillArgs a, int64_t n)
     const double x = q.x, y = q.y, z = q.z;
     const int lprev = (i == 0) ? -1 : (int)a.dp[i - 1] - 1;
     const int lcur = (int)a.dp[i] - 1;
-    const int leaf_lv = (lprev > lcur ? lprev : lcur) + 1;
+    const int n_int = lcur > lprev ? lcur - lprev : 0;
+    const int first = lprev + 1;  // level of the first node starting at i
     const int64_t base = a.off[i];
     double cx = a.box[0], cy = a.box[1], cz = a.box[2], size = a.box[3];
     int br = -1;
-    for (int lv = 0; lv <= leaf_lv; lv++) {
-        if (lv > lprev && lv <= lcur) {
-            const int64_t e = node_end(a.keys, n, i, lv);
-            emit_node(a, n, base + (lv - lprev - 1), lv, br, cx, cy, cz, size, i, e, false, p);
-        }
-        if (lv == leaf_lv) {
-            const int64_t id = base + (lcur > lprev ? lcur - lprev : 0);
-            emit_node(a, n, id, lv, br, cx, cy, cz, size, i, i, true, p);
-            break;
-        }
+    for (int lv = 0; lv < first; lv++) {
         const double h = 0.5 * size;
         br = descend(x, y, z, cx, cy, cz, h);
         size = h;
     }
+    for (int k = 0; k < n_int; k++) {
+        const int lv = first + k;
+        const int64_t e = node_end_dp(a.es, n, i, lv);
+        emit_node(a, n, base + k, lv, br, cx, cy, cz, size, i, e, false, p);
+        const double h = 0.5 * size;
+        br = descend(x, y, z, cx, cy, cz, h);
+        size = h;
+    }
+    emit_node(a, n, base + n_int, first + n_int, br, cx, cy, cz, size, i, i, true, p);
 }
 
 // ------------------------------------------------------------------------------
@@ -1056,18 +1064,19 @@ static int build_tree(gh_ctx* c, const double* d_pos, const double* d_mass, int6
 
     GH_TRY(ensure(c, tb->idx[0], sizeof(u32) * (size_t)n));
     GH_TRY(ensure(c, tb->idx[1], sizeof(u32) * (size_t)n));
-    GH_TRY(ensure(c, tb->delta, (size_t)n));
+    GH_TRY(ensure(c, tb->delta, (size_t)search_buffer_bytes(n)));
+    unsigned char* const dp = tb->delta.as<unsigned char>();
+    const int search_lv = search_levels(n);
     GH_TRY(ensure(c, tb->flags, sizeof(int) * 4));
     if (!tb->readback)
         GH_HIP(c, hipHostMalloc((void**)&tb->readback, 4 * sizeof(long long), hipHostMallocDefault));
     const int64_t nb = (n + kScanChunk - 1) / kScanChunk;
     GH_TRY(ensure(c, tb->off, sizeof(long long) * (size_t)(2 * n + 2)));
     GH_TRY(ensure(c, tb->block_sums, sizeof(DD4) * (size_t)(nb + 1)));
-    GH_TRY(ensure(c, tb->prefix, sizeof(DD4) * (size_t)(2 * n + 1)));
+    GH_TRY(ensure(c, tb->prefix, sizeof(DD4) * (size_t)(n + 1)));
     GH_TRY(ensure(c, tb->spos, sizeof(double4) * (size_t)n));
     long long* cnt = tb->off.as<long long>() + (n + 1);
     long long* off = tb->off.as<long long>();
-    DD4* vin = tb->prefix.as<DD4>() + (n + 1);
     DD4* pref = tb->prefix.as<DD4>();
     long long total = 0;
     // Keys with `words` words, sort, shared levels, numbering and moments, then one
@@ -1090,7 +1099,8 @@ static int build_tree(gh_ctx* c, const double* d_pos, const double* d_mass, int6
         // 4. shared levels
         hipLaunchKernelGGL(k_delta, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s,
                            tb->keys[tb->key_sel].as<u64>(), n, words,
-                           tb->idx[tb->key_sel].as<u32>(), d_pos, tb->delta.as<unsigned char>(),
+                           tb->idx[tb->key_sel].as<u32>(), d_pos, dp,
+                           dp + search_level_offset(n, 1), dp + search_level_offset(n, 2),
                            tb->flags.as<int>());
         GH_TRY(check_launch(c, "k_delta"));
         const u32* perm = tb->idx[tb->key_sel].as<u32>();
@@ -1101,9 +1111,10 @@ static int build_tree(gh_ctx* c, const double* d_pos, const double* d_mass, int6
         GH_TRY(scan_exclusive<OpI64>(c, cnt, n, off, tb->block_sums.as<long long>(), s));
         // 6. moments: exclusive double-double prefix sums of (m, m x, m y, m z)
         hipLaunchKernelGGL(k_moment_inputs, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s,
-                           d_pos, d_mass, perm, n, vin, tb->spos.as<double4>());
+                           d_pos, d_mass, perm, n, tb->spos.as<double4>());
         GH_TRY(check_launch(c, "k_moment_inputs"));
-        GH_TRY(scan_exclusive<OpDD4>(c, vin, n, pref, tb->block_sums.as<DD4>(), s));
+        GH_TRY(scan_exclusive<OpDD4>(c, tb->spos.as<double4>(), n, pref,
+                                     tb->block_sums.as<DD4>(), s));
         GH_HIP(c, hipMemcpyAsync(tb->readback, tb->flags.p, 2 * sizeof(int),
                                  hipMemcpyDeviceToHost, s));
         GH_HIP(c, hipMemcpyAsync(tb->readback + 1, off + n, sizeof(long long),
@@ -1138,9 +1149,9 @@ static int build_tree(gh_ctx* c, const double* d_pos, const double* d_mass, int6
 
     FillArgs fa;
     fa.perm = perm;
-    fa.dp = tb->delta.as<unsigned char>();
+    fa.dp = dp;
+    fa.es = end_search(dp, n);
     fa.off = off;
-    fa.keys = tb->keys[tb->key_sel].as<u64>();
     fa.spos = tb->spos.as<double4>();
     fa.box = tb->box.as<double>();
     fa.pref = pref;
@@ -1152,6 +1163,10 @@ static int build_tree(gh_ctx* c, const double* d_pos, const double* d_mass, int6
     fa.branch = tb->branch.as<signed char>();
     fa.leaf = tb->leaf.as<int>();
     fa.what = what;
+    if (search_lv > 3) {
+        hipLaunchKernelGGL(k_search_levels, dim3(1), dim3(kBlock), 0, s, dp, n, search_lv);
+        GH_TRY(check_launch(c, "k_search_levels"));
+    }
     hipLaunchKernelGGL(k_fill_nodes, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, fa, n);
     GH_TRY(check_launch(c, "k_fill_nodes"));
     c->last_tree_nodes = total;
