@@ -15,7 +15,9 @@
 //   3. radix sort     stable LSD sort of (key words, index), 8-bit digits
 //                     (k_rs_hist / k_rs_scan / k_rs_scatter, wave-ballot ranking)
 //   4. k_delta        delta_i = number of leading levels shared by sorted keys i, i+1;
-//                     keys equal over all W words -> more words (or coincident error)
+//                     keys equal over all W words -> more words (or coincident error);
+//                     the same pass writes the node counts, the end-search levels and
+//                     the particles in sorted order
 //   5. nodes          in depth-first pre-order (children in branch order) the nodes
 //                     whose particle range starts at sorted position i are the internal
 //                     nodes of levels (delta_{i-1}, delta_i] followed by the leaf of i at
@@ -45,7 +47,7 @@ typedef unsigned int u32;
 
 constexpr int kLevelsPerWord = 21;
 constexpr int kMaxWords = 8;  // 168 levels
-constexpr int kRsItems = 16;
+constexpr int kRsItems = 8;
 constexpr int kRsChunk = kBlock * kRsItems;  // elements per radix-sort workgroup
 
 // FP64 node record of the walk and the dump
@@ -68,7 +70,7 @@ struct DD4 {
 struct TreeBufs {
     DevBuf keys[2];  // W words per element, word-major: keys[w * n + i]
     DevBuf idx[2];
-    DevBuf hist, bbox_part, box, flags, delta, off, prefix, block_sums;
+    DevBuf hist, bbox_part, box, flags, delta, off, prefix, block_sums, block_counts;
     DevBuf spos;  // (x, y, z, m) of the particles in sorted order
     DevBuf nodes64, nodes32, skip, level, branch, leaf;
     DevBuf tkeys[2], tidx[2];  // Morton order of position targets (tree_force_position)
@@ -370,70 +372,110 @@ __global__ __launch_bounds__(kBlock) void k_rs_scatter(
 // ------------------------------------------------------------------------------
 // 4. shared levels of adjacent sorted keys
 // ------------------------------------------------------------------------------
-// dp[i] = delta_i + 1 (delta_i = shared leading levels of sorted keys i and i+1), with
-// dp[n-1] = 0; flags[0] counts pairs equal over all words, flags[1] counts those at
-// identical positions (both rare, so their atomics cost nothing).  Also the minima of
-// dp over aligned groups of 16 positions (m1) and 256 positions (m2, one per
-// workgroup), which the node fill searches; positions from n on count as 0 there.
-// Each lane's 16-lane group lies in its own wavefront, so m1 is a 16-lane reduction.
-static_assert(kBlock == 256, "k_delta writes one m2 entry per workgroup of 256");
-__global__ __launch_bounds__(kBlock) void k_delta(const u64* __restrict__ keys, int64_t n,
-                                                  int words, const u32* __restrict__ perm,
-                                                  const double* __restrict__ pos,
-                                                  unsigned char* __restrict__ dp,
-                                                  unsigned char* __restrict__ m1,
-                                                  unsigned char* __restrict__ m2,
-                                                  int* __restrict__ flags)
+// delta_i + 1 for sorted positions i and i+1 (delta_i = their shared leading levels,
+// words * 21 when the keys are equal over all words), 0 for i = n-1.
+__device__ __forceinline__ int shared_levels_plus1(const u64* __restrict__ keys, int64_t n,
+                                                   int words, int64_t i)
+{
+    if (i >= n - 1) return 0;
+    int delta = words * kLevelsPerWord;
+    for (int w = 0; w < words; w++) {
+        const u64 a = keys[(int64_t)w * n + i], b = keys[(int64_t)w * n + i + 1];
+        if (a != b) {
+            const int lz = __clzll(a ^ b);  // >= 1: bit 63 is never used
+            delta = w * kLevelsPerWord + (lz - 1) / 3;
+            break;
+        }
+    }
+    return delta + 1;
+}
+
+struct DeltaArgs {
+    const u64* keys;    // sorted keys, word-major
+    const u32* perm;    // sorted position -> particle
+    const double* pos;  // particles (x, y, z)
+    const double* mass;
+    unsigned char* search;  // end-search buffer: dp = level 0, then levels 1, 2, ...
+    long long* cnt;         // nodes starting at each sorted position
+    double4* spos;          // sorted (x, y, z, m)
+    int* flags;             // [0] pairs equal over all words, [1] coincident pairs
+};
+
+// Per sorted position i, after the sort:
+//  - dp[i] = delta_i + 1 (dp[n-1] = 0); flags[0] counts pairs equal over all words,
+//    flags[1] those at identical positions (both rare, so their atomics cost nothing);
+//  - cnt[i] = internal nodes starting at i (levels delta_{i-1} + 1 .. delta_i) plus its
+//    leaf, the input of the node numbering scan;
+//  - spos[i] = (x, y, z, m) of the particle, read contiguously by the moment scan and the
+//    node fill;
+//  - the end-search levels 1 (minimum of dp over aligned groups of 16, a 16-lane
+//    reduction inside a wavefront) and 2 (groups of 256, one per workgroup), with
+//    positions from n on counting as 0.
+static_assert(kBlock == 256, "k_delta writes one level-2 entry per workgroup of 256");
+__global__ __launch_bounds__(kBlock) void k_delta(DeltaArgs a, int64_t n, int words)
 {
     __shared__ int wave_min[kBlock / 64];
+    __shared__ int lane_m[kBlock];
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    int m = 0;  // this lane's delta + 1; 0 for the sentinel and for lanes past the end
-    if (i < n - 1) {
-        int delta = words * kLevelsPerWord;
-        for (int w = 0; w < words; w++) {
-            const u64 a = keys[(int64_t)w * n + i], b = keys[(int64_t)w * n + i + 1];
-            if (a != b) {
-                const int lz = __clzll(a ^ b);  // >= 1: bit 63 is never used
-                delta = w * kLevelsPerWord + (lz - 1) / 3;
-                break;
-            }
+    const int m = i < n ? shared_levels_plus1(a.keys, n, words, i) : 0;
+    unsigned char* dp = a.search;
+    if (i < n) {
+        dp[i] = (unsigned char)m;
+        const u32 p = a.perm[i];
+        a.spos[i] = make_double4(a.pos[3 * p], a.pos[3 * p + 1], a.pos[3 * p + 2], a.mass[p]);
+        if (m == words * kLevelsPerWord + 1) {
+            atomicAdd(&a.flags[0], 1);
+            const u32 q = a.perm[i + 1];
+            if (a.pos[3 * p] == a.pos[3 * q] && a.pos[3 * p + 1] == a.pos[3 * q + 1] &&
+                a.pos[3 * p + 2] == a.pos[3 * q + 2])
+                atomicAdd(&a.flags[1], 1);
         }
-        if (delta == words * kLevelsPerWord) {
-            atomicAdd(&flags[0], 1);
-            const u32 p = perm[i], q = perm[i + 1];
-            if (pos[3 * p] == pos[3 * q] && pos[3 * p + 1] == pos[3 * q + 1] &&
-                pos[3 * p + 2] == pos[3 * q + 2])
-                atomicAdd(&flags[1], 1);
-        }
-        m = delta + 1;
     }
-    if (i < n) dp[i] = (unsigned char)m;
+    lane_m[threadIdx.x] = m;
+    // level 1 and 2 minima
     int g = m;
     for (int o = 1; o < 16; o <<= 1) g = min(g, __shfl_xor(g, o));
-    if ((threadIdx.x & 15) == 0) m1[i >> 4] = (unsigned char)g;
+    if ((threadIdx.x & 15) == 0) (dp + search_level_offset(n, 1))[i >> 4] = (unsigned char)g;
     for (int o = 16; o < 64; o <<= 1) g = min(g, __shfl_xor(g, o));
     if ((threadIdx.x & 63) == 0) wave_min[threadIdx.x >> 6] = g;
     __syncthreads();
+    // node counts: levels (delta_{i-1}, delta_i] plus the leaf
+    if (i < n) {
+        const int mprev = threadIdx.x ? lane_m[threadIdx.x - 1]
+                                      : (i ? shared_levels_plus1(a.keys, n, words, i - 1) : 0);
+        const int lprev = mprev - 1, lcur = m - 1;
+        a.cnt[i] = (long long)(lcur > lprev ? lcur - lprev : 0) + 1;
+    }
     if (threadIdx.x == 0) {
         int b = wave_min[0];
         for (int k = 1; k < kBlock / 64; k++) b = min(b, wave_min[k]);
-        m2[blockIdx.x] = (unsigned char)b;
+        (dp + search_level_offset(n, 2))[blockIdx.x] = (unsigned char)b;
     }
 }
 
-// cnt[i] = internal nodes starting at sorted position i, plus its leaf
-__global__ __launch_bounds__(kBlock) void k_node_counts(const unsigned char* __restrict__ dp,
-                                                        int64_t n, long long* __restrict__ cnt)
+// End-search levels 3 and up, each from the level below, by one workgroup (the levels
+// above 2 hold at most n / 4096 entries).
+__global__ __launch_bounds__(kBlock) void k_search_levels(unsigned char* __restrict__ dp,
+                                                          int64_t n, int levels)
 {
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const int lprev = (i == 0) ? -1 : (int)dp[i - 1] - 1;
-    const int lcur = (int)dp[i] - 1;
-    cnt[i] = (long long)(lcur > lprev ? lcur - lprev : 0) + 1;
+    for (int k = 3; k < levels; k++) {
+        const unsigned char* lo = dp + search_level_offset(n, k - 1);
+        unsigned char* hi = dp + search_level_offset(n, k);
+        const int64_t lo_size = search_level_size(n, k - 1), size = search_level_size(n, k);
+        for (int64_t e = threadIdx.x; e < size; e += kBlock) {
+            int v = 255;
+            for (int j = 0; j < 16; j++) {
+                const int64_t q = 16 * e + j;
+                v = min(v, q < lo_size ? (int)lo[q] : 0);
+            }
+            hi[e] = (unsigned char)v;
+        }
+        __syncthreads();  // the next level reads this one
+    }
 }
 
 // ------------------------------------------------------------------------------
-// generic three-phase scans (int64 sums and double-double 4-vectors)
+// generic two-phase scans (int64 sums and double-double 4-vectors)
 // ------------------------------------------------------------------------------
 __device__ __forceinline__ void dd_add(double& h, double& l, double bh, double bl)
 {
@@ -549,18 +591,26 @@ __global__ __launch_bounds__(kBlock) void k_scan_sums(typename Op::T* __restrict
     if (threadIdx.x == kBlock - 1) *total = sh[kBlock - 1];
 }
 
+// A scan kept in two parts, block-local exclusive prefixes and the exclusive prefix of
+// the block totals, and added where it is read (the one reader visits each element
+// about twice, so a separate pass adding them would cost more than it saves).
 template <class Op>
-__global__ __launch_bounds__(kBlock) void k_scan_add(typename Op::T* __restrict__ out, int64_t n,
-                                                     const typename Op::T* __restrict__ sums)
-{
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i < n) out[i] = Op::add(sums[i / kScanChunk], out[i]);
-}
+struct ScanView {
+    const typename Op::T* local;   // n+1 entries, local[n] = total
+    const typename Op::T* blocks;  // exclusive prefix of the block totals
+    int64_t n;
+    // element j (0 <= j <= n) of the exclusive scan
+    __device__ typename Op::T at(int64_t j) const
+    {
+        return j == n ? local[n] : Op::add(blocks[j / kScanChunk], local[j]);
+    }
+};
 
-// out must hold n+1 elements and tmp at least blocks+1 elements
+// Exclusive scan of in[0..n) in two parts (ScanView): out must hold n+1 elements and
+// tmp at least blocks+1 elements.
 template <class Op>
 static int scan_exclusive(gh_ctx* c, const typename Op::In* in, int64_t n, typename Op::T* out,
-                          typename Op::T* tmp, hipStream_t s)
+                          typename Op::T* tmp, hipStream_t s, ScanView<Op>* view)
 {
     const int64_t nb = (n + kScanChunk - 1) / kScanChunk;
     if (nb > 0) {
@@ -570,50 +620,15 @@ static int scan_exclusive(gh_ctx* c, const typename Op::In* in, int64_t n, typen
     }
     hipLaunchKernelGGL(k_scan_sums<Op>, dim3(1), dim3(kBlock), 0, s, tmp, nb, out + n);
     GH_TRY(check_launch(c, "k_scan_sums"));
-    if (n > 0) {
-        hipLaunchKernelGGL(k_scan_add<Op>, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, out,
-                           n, tmp);
-        GH_TRY(check_launch(c, "k_scan_add"));
-    }
+    view->local = out;
+    view->blocks = tmp;
+    view->n = n;
     return 0;
 }
 
 // ------------------------------------------------------------------------------
 // 5./6. node ends, moments, node records
 // ------------------------------------------------------------------------------
-// Levels 3 and up of the end search, each from the level below, by one workgroup.
-__global__ __launch_bounds__(kBlock) void k_search_levels(unsigned char* __restrict__ base,
-                                                          int64_t n, int levels)
-{
-    for (int k = 3; k < levels; k++) {
-        const unsigned char* lo = base + search_level_offset(n, k - 1);
-        unsigned char* hi = base + search_level_offset(n, k);
-        const int64_t lo_size = search_level_size(n, k - 1), size = search_level_size(n, k);
-        for (int64_t e = threadIdx.x; e < size; e += kBlock) {
-            int v = 255;
-            for (int j = 0; j < 16; j++) {
-                const int64_t q = 16 * e + j;
-                v = min(v, q < lo_size ? (int)lo[q] : 0);
-            }
-            hi[e] = (unsigned char)v;
-        }
-        __syncthreads();  // the next level reads this one (same workgroup)
-    }
-}
-
-// Gathers the particles into sorted order as (x, y, z, m), read contiguously by the
-// moment prefix sums and the node fill.
-__global__ __launch_bounds__(kBlock) void k_moment_inputs(const double* __restrict__ pos,
-                                                          const double* __restrict__ mass,
-                                                          const u32* __restrict__ perm,
-                                                          int64_t n, double4* __restrict__ spos)
-{
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const u32 p = perm[i];
-    spos[i] = make_double4(pos[3 * p], pos[3 * p + 1], pos[3 * p + 2], mass[p]);
-}
-
 __device__ __forceinline__ double dd_diff(const DD4& a, const DD4& b, int k)
 {
     return (a.h[k] - b.h[k]) + (a.l[k] - b.l[k]);
@@ -623,10 +638,10 @@ struct FillArgs {
     const u32* perm;
     const unsigned char* dp;
     EndSearch es;
-    const long long* off;  // n+1 entries
+    ScanView<OpI64> off;  // first node id of each sorted position, n+1 entries
     const double4* spos;  // sorted (x, y, z, m)
     const double* box;
-    const DD4* pref;  // n+1 entries
+    ScanView<OpDD4> pref;  // moment prefix sums, n+1 entries
     double theta;
     NodeD* nodes64;
     NodeF* nodes32;
@@ -657,14 +672,14 @@ __device__ void emit_node(const FillArgs& a, int64_t n, int64_t id, int lv, int 
         d.my = q.y;
         d.mz = q.z;
     } else {
-        const DD4 hi = a.pref[e + 1], lo = a.pref[s];
+        const DD4 hi = a.pref.at(e + 1), lo = a.pref.at(s);
         const double m = dd_diff(hi, lo, 0);
         d.mass = m;
         d.mx = dd_diff(hi, lo, 1) / m;
         d.my = dd_diff(hi, lo, 2) / m;
         d.mz = dd_diff(hi, lo, 3) / m;
     }
-    a.skip[id] = (int)a.off[e + 1];
+    a.skip[id] = (int)a.off.at(e + 1);
     if (a.what & (kFillF64 | kFillDump)) {
         a.nodes64[id] = d;
         a.leaf[id] = is_leaf ? (int)p : -1;
@@ -705,7 +720,7 @@ __global__ __launch_bounds__(kBlock) void k_fill_nodes(FillArgs a, int64_t n)
     const int lcur = (int)a.dp[i] - 1;
     const int n_int = lcur > lprev ? lcur - lprev : 0;
     const int first = lprev + 1;  // level of the first node starting at i
-    const int64_t base = a.off[i];
+    const int64_t base = a.off.at(i);
     double cx = a.box[0], cy = a.box[1], cz = a.box[2], size = a.box[3];
     int br = -1;
     for (int lv = 0; lv < first; lv++) {
@@ -1073,11 +1088,14 @@ static int build_tree(gh_ctx* c, const double* d_pos, const double* d_mass, int6
     const int64_t nb = (n + kScanChunk - 1) / kScanChunk;
     GH_TRY(ensure(c, tb->off, sizeof(long long) * (size_t)(2 * n + 2)));
     GH_TRY(ensure(c, tb->block_sums, sizeof(DD4) * (size_t)(nb + 1)));
+    GH_TRY(ensure(c, tb->block_counts, sizeof(long long) * (size_t)(nb + 1)));
     GH_TRY(ensure(c, tb->prefix, sizeof(DD4) * (size_t)(n + 1)));
     GH_TRY(ensure(c, tb->spos, sizeof(double4) * (size_t)n));
     long long* cnt = tb->off.as<long long>() + (n + 1);
     long long* off = tb->off.as<long long>();
     DD4* pref = tb->prefix.as<DD4>();
+    ScanView<OpI64> off_view;
+    ScanView<OpDD4> pref_view;
     long long total = 0;
     // Keys with `words` words, sort, shared levels, numbering and moments, then one
     // readback of (flags, node count).  Keys still equal over all words (rare) mean
@@ -1096,25 +1114,29 @@ static int build_tree(gh_ctx* c, const double* d_pos, const double* d_mass, int6
         GH_TRY(check_launch(c, "k_keys"));
         // 3. sort
         GH_TRY(radix_sort(c, tb->keys, tb->idx, tb->hist, n, words, 0, s, &tb->key_sel));
-        // 4. shared levels
-        hipLaunchKernelGGL(k_delta, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s,
-                           tb->keys[tb->key_sel].as<u64>(), n, words,
-                           tb->idx[tb->key_sel].as<u32>(), d_pos, dp,
-                           dp + search_level_offset(n, 1), dp + search_level_offset(n, 2),
-                           tb->flags.as<int>());
+        // 4./5. shared levels, end-search levels, node counts, sorted particles
+        DeltaArgs da;
+        da.keys = tb->keys[tb->key_sel].as<u64>();
+        da.perm = tb->idx[tb->key_sel].as<u32>();
+        da.pos = d_pos;
+        da.mass = d_mass;
+        da.search = dp;
+        da.cnt = cnt;
+        da.spos = tb->spos.as<double4>();
+        da.flags = tb->flags.as<int>();
+        hipLaunchKernelGGL(k_delta, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, da, n,
+                           words);
         GH_TRY(check_launch(c, "k_delta"));
-        const u32* perm = tb->idx[tb->key_sel].as<u32>();
+        if (search_lv > 3) {
+            hipLaunchKernelGGL(k_search_levels, dim3(1), dim3(kBlock), 0, s, dp, n, search_lv);
+            GH_TRY(check_launch(c, "k_search_levels"));
+        }
         // 5. numbering
-        hipLaunchKernelGGL(k_node_counts, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s,
-                           tb->delta.as<unsigned char>(), n, cnt);
-        GH_TRY(check_launch(c, "k_node_counts"));
-        GH_TRY(scan_exclusive<OpI64>(c, cnt, n, off, tb->block_sums.as<long long>(), s));
+        GH_TRY(scan_exclusive<OpI64>(c, cnt, n, off, tb->block_counts.as<long long>(), s,
+                                     &off_view));
         // 6. moments: exclusive double-double prefix sums of (m, m x, m y, m z)
-        hipLaunchKernelGGL(k_moment_inputs, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s,
-                           d_pos, d_mass, perm, n, tb->spos.as<double4>());
-        GH_TRY(check_launch(c, "k_moment_inputs"));
         GH_TRY(scan_exclusive<OpDD4>(c, tb->spos.as<double4>(), n, pref,
-                                     tb->block_sums.as<DD4>(), s));
+                                     tb->block_sums.as<DD4>(), s, &pref_view));
         GH_HIP(c, hipMemcpyAsync(tb->readback, tb->flags.p, 2 * sizeof(int),
                                  hipMemcpyDeviceToHost, s));
         GH_HIP(c, hipMemcpyAsync(tb->readback + 1, off + n, sizeof(long long),
@@ -1151,10 +1173,10 @@ static int build_tree(gh_ctx* c, const double* d_pos, const double* d_mass, int6
     fa.perm = perm;
     fa.dp = dp;
     fa.es = end_search(dp, n);
-    fa.off = off;
+    fa.off = off_view;
     fa.spos = tb->spos.as<double4>();
     fa.box = tb->box.as<double>();
-    fa.pref = pref;
+    fa.pref = pref_view;
     fa.theta = theta;
     fa.nodes64 = tb->nodes64.as<NodeD>();
     fa.nodes32 = tb->nodes32.as<NodeF>();
@@ -1163,10 +1185,6 @@ static int build_tree(gh_ctx* c, const double* d_pos, const double* d_mass, int6
     fa.branch = tb->branch.as<signed char>();
     fa.leaf = tb->leaf.as<int>();
     fa.what = what;
-    if (search_lv > 3) {
-        hipLaunchKernelGGL(k_search_levels, dim3(1), dim3(kBlock), 0, s, dp, n, search_lv);
-        GH_TRY(check_launch(c, "k_search_levels"));
-    }
     hipLaunchKernelGGL(k_fill_nodes, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, fa, n);
     GH_TRY(check_launch(c, "k_fill_nodes"));
     c->last_tree_nodes = total;
@@ -1285,7 +1303,8 @@ void tree_free(gh_ctx* c)
     TreeBufs* tb = c->tree;
     DevBuf* bufs[] = {&tb->keys[0], &tb->keys[1], &tb->idx[0],    &tb->idx[1],  &tb->hist,
                       &tb->bbox_part, &tb->box,   &tb->flags,     &tb->delta,   &tb->off,
-                      &tb->prefix,  &tb->spos,      &tb->block_sums, &tb->nodes64, &tb->nodes32,
+                      &tb->prefix,  &tb->spos,      &tb->block_sums, &tb->block_counts,
+                      &tb->nodes64, &tb->nodes32,
                       &tb->skip,    &tb->level,   &tb->branch,    &tb->leaf,    &tb->tkeys[0],
                       &tb->tkeys[1], &tb->tidx[0], &tb->tidx[1],  &tb->walk_queue};
     for (DevBuf* b : bufs)
