@@ -745,10 +745,11 @@ __global__ __launch_bounds__(kBlock) void k_fill_nodes(FillArgs a, int64_t n)
 // FP32 walk.  Targets: tgt (AoS f64) in the order given by order[] (nullable), results
 // written to acc[order[k]].
 //
-// A "unit" is 64 * WPW consecutive targets, walked by one wavefront as WPW independent
-// 64-lane walks whose node loads are issued together, so one walk's load latency
-// overlaps the others'.  The node index of a walk is wave-uniform, so node records and
-// skip pointers come through scalar loads.
+// A "unit" is lanes * WPW consecutive targets (lanes <= 64 targets per walk), walked by
+// one wavefront as WPW independent walks whose node loads are issued together, so one
+// walk's load latency overlaps the others'.  Lanes from `lanes` on stay idle.  The node
+// index of a walk is wave-uniform, so node records and skip pointers come through
+// scalar loads.
 //
 // Per visited node: a leaf (geo.w = -1, tested on the scalar unit) is accepted by every
 // lane waiting for it; an internal node gets the opening test in each waiting lane, and
@@ -769,6 +770,7 @@ __global__ __launch_bounds__(kBlock) void k_fill_nodes(FillArgs a, int64_t n)
 constexpr int kXcds = 8;
 constexpr int kFlushVisits = 64;
 constexpr int64_t kSortTargetsMin = 4096;  // position targets sorted from this many on
+constexpr int64_t kWalkWaves = 4096;       // walks in flight for small target counts
 __device__ __forceinline__ int64_t xcd_block(int64_t per_xcd)
 {
     return (int64_t)(blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
@@ -780,9 +782,10 @@ __device__ __forceinline__ void walk_unit_f32(const char* __restrict__ nodes,
                                               const double* __restrict__ tgt,
                                               const u32* __restrict__ order, int64_t nt,
                                               float eps2, double* __restrict__ acc,
-                                              int64_t unit, int lane)
+                                              int64_t unit, int lane, int lanes)
 {
-    const int64_t base = unit * 64 * WPW + lane;
+    const int64_t first = unit * lanes * WPW;  // the unit's first target
+    const int64_t base = first + lane;
     eps2 = in_vgpr(eps2);
     int64_t t[WPW];
     float x[WPW], y[WPW], z[WPW], fx[WPW], fy[WPW], fz[WPW];
@@ -790,8 +793,8 @@ __device__ __forceinline__ void walk_unit_f32(const char* __restrict__ nodes,
     int my_next[WPW], node[WPW];
 #pragma unroll
     for (int w = 0; w < WPW; w++) {
-        const int64_t k = base + w * 64;
-        const bool live = k < nt;
+        const int64_t k = base + w * lanes;
+        const bool live = lane < lanes && k < nt;
         t[w] = 0;
         x[w] = y[w] = z[w] = 0.f;
         if (live) {
@@ -804,7 +807,7 @@ __device__ __forceinline__ void walk_unit_f32(const char* __restrict__ nodes,
         ax[w] = ay[w] = az[w] = 0.0;
         my_next[w] = live ? 0 : 0x7fffffff;
         // a walk with no live lane never starts
-        node[w] = __builtin_amdgcn_readfirstlane((unit * 64 * WPW + w * 64 < nt) ? 0 : n_nodes);
+        node[w] = __builtin_amdgcn_readfirstlane((first + w * lanes < nt) ? 0 : n_nodes);
     }
     for (int it = 1;; it++) {
         bool more = false;
@@ -872,7 +875,7 @@ __device__ __forceinline__ void walk_unit_f32(const char* __restrict__ nodes,
     }
 #pragma unroll
     for (int w = 0; w < WPW; w++) {
-        if (base + w * 64 < nt) {
+        if (lane < lanes && base + w * lanes < nt) {
             acc[3 * t[w]] = ax[w] + (double)fx[w];
             acc[3 * t[w] + 1] = ay[w] + (double)fy[w];
             acc[3 * t[w] + 2] = az[w] + (double)fz[w];
@@ -887,14 +890,15 @@ __global__ __launch_bounds__(kBlock) void k_walk_f32(const NodeF* __restrict__ n
                                                      const u32* __restrict__ order, int64_t nt,
                                                      float eps2, double* __restrict__ acc,
                                                      int64_t n_units, int64_t per_xcd,
-                                                     u32* __restrict__ queue)
+                                                     u32* __restrict__ queue, int lanes)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const char* nb = (const char*)nodes;
     if (!DYN) {
         const int64_t unit = xcd_block(per_xcd) * (kBlock / 64) + wave;
         if (unit < n_units)
-            walk_unit_f32<WPW, GUARD>(nb, skip, n_nodes, tgt, order, nt, eps2, acc, unit, lane);
+            walk_unit_f32<WPW, GUARD>(nb, skip, n_nodes, tgt, order, nt, eps2, acc, unit, lane,
+                                      lanes);
         return;
     }
     const int home = (int)(blockIdx.x % kXcds);
@@ -907,7 +911,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_f32(const NodeF* __restrict__ n
             k = __builtin_amdgcn_readfirstlane(k);
             if (lo + (int64_t)k >= hi) break;
             walk_unit_f32<WPW, GUARD>(nb, skip, n_nodes, tgt, order, nt, eps2, acc,
-                                      lo + (int64_t)k, lane);
+                                      lo + (int64_t)k, lane, lanes);
         }
     }
 }
@@ -950,12 +954,14 @@ __global__ __launch_bounds__(kBlock) void k_walk_f64(const NodeD* __restrict__ n
                                                      double eps, double theta, double c_lo,
                                                      double c_hi, double* __restrict__ acc,
                                                      int* __restrict__ counts,
-                                                     int64_t per_xcd)
+                                                     int64_t per_xcd, int lanes)
 {
-    const int64_t blk = xcd_block(per_xcd);
-    if (blk * kBlock >= nt) return;  // whole workgroup idle
-    const int64_t k = blk * kBlock + threadIdx.x;
-    const bool live = k < nt;
+    // one walk of `lanes` consecutive targets per wavefront (lanes <= 64)
+    const int lane = threadIdx.x & 63;
+    const int64_t first = (xcd_block(per_xcd) * (kBlock / 64) + (threadIdx.x >> 6)) * lanes;
+    if (first >= nt) return;  // whole wavefront idle
+    const int64_t k = first + lane;
+    const bool live = lane < lanes && k < nt;
     int64_t t = 0;
     double x = 0.0, y = 0.0, z = 0.0;
     if (live) {
@@ -1220,15 +1226,23 @@ static int walk(gh_ctx* c, const double* tgt, bool self, int64_t nt, double eps,
         GH_TRY(radix_sort(c, tb->tkeys, tb->tidx, tb->hist, nt, 1, 24, s, &sel));
         order = tb->tidx[sel].as<u32>();
     }
+    // Targets per walk: 64 (a full wavefront) when there are enough targets to fill the
+    // GPU; fewer targets give each walk fewer lanes, so that about kWalkWaves walks run
+    // side by side (a walk's length grows with the spread of its targets, and a few
+    // long walks would leave most of the GPU idle).
+    const int lanes = (int)std::min<int64_t>(64, std::max<int64_t>(1, (nt + kWalkWaves - 1) /
+                                                                          kWalkWaves));
+    const int waves_per_block = kBlock / 64;
     if (precision == GH_FP64) {
-        const int64_t per = (blocks_for(nt, kBlock) + kXcds - 1) / kXcds;
+        const int64_t n_walks = (nt + lanes - 1) / lanes;
+        const int64_t per = (blocks_for(n_walks, waves_per_block) + kXcds - 1) / kXcds;
         const double inf = __builtin_huge_val();
         const double c_lo = theta > 0.0 ? (1.0 - 1e-13) / (theta * theta) : inf;
         const double c_hi = theta > 0.0 ? (1.0 + 1e-13) / (theta * theta) : inf;
         hipLaunchKernelGGL(k_walk_f64, dim3((unsigned)(per * kXcds)), dim3(kBlock), 0, s,
                            tb->nodes64.as<NodeD>(), tb->skip.as<int>(), tb->leaf.as<int>(),
                            tb->n_nodes, tgt, order, nt, eps, theta, c_lo, c_hi, d_acc, counts,
-                           per);
+                           per, lanes);
         return check_launch(c, "k_walk_f64");
     }
     const float eps2 = (float)(eps * eps);
@@ -1247,8 +1261,7 @@ static int walk(gh_ctx* c, const double* tgt, bool self, int64_t nt, double eps,
     const NodeF* nd = tb->nodes32.as<NodeF>();
     const int* sk = tb->skip.as<int>();
     const int T = (int)tb->n_nodes;
-    const int waves_per_block = kBlock / 64;
-    const int64_t n_units = (nt + 64 * wpw - 1) / (64 * wpw);
+    const int64_t n_units = (nt + (int64_t)lanes * wpw - 1) / ((int64_t)lanes * wpw);
     const int64_t nblk = (n_units + waves_per_block - 1) / waves_per_block;
     const int64_t per = (nblk + kXcds - 1) / kXcds;
     u32* q = nullptr;
@@ -1264,7 +1277,7 @@ static int walk(gh_ctx* c, const double* tgt, bool self, int64_t nt, double eps,
     }
 #define GH_WALK32(W, G, D)                                                                 \
     hipLaunchKernelGGL((k_walk_f32<W, G, D>), grid, dim3(kBlock), 0, s, nd, sk, T, tgt,  \
-                       order, nt, eps2, d_acc, n_units, per, q)
+                       order, nt, eps2, d_acc, n_units, per, q, lanes)
     if (wpw == 2) {
         if (dyn) {
             if (guard) GH_WALK32(2, true, true); else GH_WALK32(2, false, true);

@@ -171,3 +171,32 @@ def test_walks_per_wave_bitwise(ctx):
         ctx.set_tuning(_native.TUNE_WALK_SCHEDULE, 0)
     for o in out[1:]:
         assert_bitwise(out[0], o)
+
+
+def test_fp64_walk_independent_of_target_grouping(ctx):
+    # The FP64 walk sums each target's accepted nodes in tree order, whatever the other
+    # targets of its wavefront: a target's force is bit-identical among 300 targets
+    # (5 per walk), among 20000 (sorted, 5 per walk) or as a particle of the self walk
+    # (Morton order, 64 per walk).
+    g = workloads.galaxy(n_halo=150000, n_disk=150000, seed=86)
+    pos, mass = g["pos"], g["mass"]
+    rng = np.random.default_rng(87)
+    pick = rng.choice(len(mass), 20000, replace=False)
+    fpos = pos[pick]
+    self_acc = ctx.tree(pos, mass, 0.05, 0.7, FP64)
+    all_pos = ctx.tree(pos, mass, 0.05, 0.7, FP64, force_pos=fpos)
+    few_pos = ctx.tree(pos, mass, 0.05, 0.7, FP64, force_pos=fpos[:300])
+    assert_bitwise(all_pos[:300], few_pos, "300 vs 20000 targets")
+    assert_bitwise(all_pos, self_acc[pick], "position targets vs self walk")
+
+
+def test_fp32_walk_few_targets(ctx, oracle):
+    # few position targets are spread over many short walks (down to one target each)
+    rng = np.random.default_rng(88)
+    pos = rng.normal(size=(20000, 3))
+    mass = rng.uniform(size=20000)
+    for nt in (1, 7, 300):
+        fpos = rng.normal(size=(nt, 3)) * 2
+        want = oracle.tree(pos, mass, 0.01, 0.7, force_pos=fpos)
+        got = ctx.tree(pos, mass, 0.01, 0.7, FP32, force_pos=fpos)
+        assert np.median(rel_err(got, want)) < FP32_TREE_MEDIAN, nt
