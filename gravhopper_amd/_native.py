@@ -69,7 +69,7 @@ SIGNATURES = {
 
 # gh_set_tuning keys
 TUNE_DIRECT_SELF, TUNE_DIRECT_IPT, TUNE_WALKS_PER_WAVE, TUNE_RUN_GRAPH = 0, 1, 2, 3
-TUNE_WALK_SCHEDULE, TUNE_ALLOC_LIMIT_MB = 4, 5
+TUNE_WALK_SCHEDULE, TUNE_ALLOC_LIMIT_MB, TUNE_DIRECT_F64_TPW = 4, 5, 6
 
 # kernel classes of gh_timing_read
 T_DIRECT_F32, T_DIRECT_F64, T_TREE_BUILD, T_TREE_WALK, T_UPDATE, T_VJP = range(6)

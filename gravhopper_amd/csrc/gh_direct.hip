@@ -5,6 +5,9 @@
 //                 :299-353 (targets), so the result is bit-identical to the reference.
 //                 Built with -ffp-contract=off; double '/' and sqrt are the compiler's
 //                 correctly rounded expansions.
+//  k_direct_f64_spread  the same sums for fewer than 32768 targets: a target's terms
+//                 are computed by several lanes, then added by one lane in the same
+//                 order, so the results are the same bits.
 //  k_direct_f32   FP32 pair forces with v_rsq_f32, IPT targets per lane, 256-source
 //                 tiles staged in LDS as float4 (x, y, z, m) and read as wave-wide
 //                 broadcasts, FP32 sums per tile and an FP32 two-sum (hi, lo) across
@@ -109,6 +112,105 @@ __global__ __launch_bounds__(kBlock) void k_direct_f64(const double4* __restrict
         acc[3 * i + 1] = f1;
         acc[3 * i + 2] = f2;
     }
+}
+
+// The same sums for few targets, when one lane per target would leave most of the GPU
+// idle (each lane's serial loop over all sources, with an IEEE division and square root
+// per pair, is then the whole run time).  TPW targets per wavefront, P = 64 / TPW lanes
+// per target: per chunk of P sources, the P lanes of a target compute one term each
+// (the same expressions as k_direct_f64), and the target's first lane then adds the P
+// terms in ascending source order.  Every target's sum is formed term by term in the
+// reference's order, so the results are bit-identical to k_direct_f64's.
+template <bool SELF, int TPW>
+__global__ __launch_bounds__(kBlock) void k_direct_f64_spread(const double4* __restrict__ src,
+                                                              int64_t np,
+                                                              const double* __restrict__ tgt,
+                                                              int64_t nt, double eps2,
+                                                              double* __restrict__ acc)
+{
+    constexpr int P = 64 / TPW;
+    __shared__ double terms[kBlock / 64][3][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int g = lane / P, q = lane % P;  // target of this lane in the wave, source slot
+    const int64_t i = ((int64_t)blockIdx.x * (kBlock / 64) + wave) * TPW + g;
+    const bool live = i < nt;
+    double xi = 0.0, yi = 0.0, zi = 0.0;
+    if (live) {
+        xi = tgt[3 * i];
+        yi = tgt[3 * i + 1];
+        zi = tgt[3 * i + 2];
+    }
+    double f0 = 0.0, f1 = 0.0, f2 = 0.0;  // the sums, kept by lane q == 0
+    for (int64_t base = 0; base < np; base += P) {
+        const int64_t j = base + q;
+        double c0 = 0.0, c1 = 0.0, c2 = 0.0;
+        if (live && j < np) {
+            const double4 s = src[j];
+            const double dx = s.x - xi, dy = s.y - yi, dz = s.z - zi;
+            double d2 = dx * dx;
+            d2 = d2 + dy * dy;
+            d2 = d2 + dz * dz;
+            const double d = d2 + eps2;
+            double inv;
+            if (SELF) {
+                inv = 1.0 / d / sqrt(d);
+            } else {
+                inv = (d == 0.0) ? 0.0 : 1.0 / d / sqrt(d);
+            }
+            c0 = s.w * dx * inv;
+            c1 = s.w * dy * inv;
+            c2 = s.w * dz * inv;
+        }
+        terms[wave][0][lane] = c0;
+        terms[wave][1][lane] = c1;
+        terms[wave][2][lane] = c2;
+        __syncthreads();
+        if (live && q == 0) {
+            // the chunk's terms in order, without the target itself (at most one term):
+            // two branch-free runs, so that the LDS reads can be issued ahead
+            const int cnt = (int)((np - base) < P ? (np - base) : P);
+            const int skip = (SELF && i >= base && i < base + cnt) ? (int)(i - base) : cnt;
+            const double* r0 = &terms[wave][0][lane];
+            const double* r1 = &terms[wave][1][lane];
+            const double* r2 = &terms[wave][2][lane];
+#pragma unroll 8
+            for (int k = 0; k < skip; k++) {
+                f0 = f0 + r0[k];
+                f1 = f1 + r1[k];
+                f2 = f2 + r2[k];
+            }
+#pragma unroll 8
+            for (int k = skip + 1; k < cnt; k++) {
+                f0 = f0 + r0[k];
+                f1 = f1 + r1[k];
+                f2 = f2 + r2[k];
+            }
+        }
+        __syncthreads();
+    }
+    if (live && q == 0) {
+        acc[3 * i] = f0;
+        acc[3 * i + 1] = f1;
+        acc[3 * i + 2] = f2;
+    }
+}
+
+// Targets per wavefront for the FP64 sums: 64 (k_direct_f64) from kF64SpreadMax
+// targets on; fewer targets are spread so that at most about kF64Waves wavefronts run,
+// with at least 2 targets per wavefront.  Measured on Plummer spheres (kernel ms for
+// 1/2/4/8/16/32/64 targets per wavefront): N=2000 0.112/0.097/0.143/0.241/0.45/0.69/
+// 0.416; N=16384 5.56/3.02/1.90/1.95/3.62/5.62/3.44; N=65536 87/45.5/25.4/18.2/19.6/
+// 24.8/13.9.
+constexpr int64_t kF64Waves = 4096;
+constexpr int64_t kF64SpreadMax = 32768;
+static int f64_targets_per_wave(const gh_ctx* c, int64_t nt)
+{
+    const int t = c->tune[GH_TUNE_DIRECT_F64_TPW];
+    if (t == 1 || t == 2 || t == 4 || t == 8 || t == 16 || t == 32 || t == 64) return t;
+    if (nt >= kF64SpreadMax) return 64;
+    int tpw = 2;
+    while (tpw < 64 && (int64_t)tpw * kF64Waves < nt) tpw *= 2;
+    return tpw;
 }
 
 // ------------------------------------------------------------------------------
@@ -613,13 +715,36 @@ int direct_dev(gh_ctx* c, const double* d_pos, const double* d_mass, int64_t np,
         GH_TRY(check_launch(c, "k_pack_src64"));
         const double eps2 = eps * eps;
         TimedScope ts(c, s, GH_T_DIRECT_F64);
-        if (self)
-            hipLaunchKernelGGL(k_direct_f64<true>, dim3(blocks_for(nt, kBlock)), dim3(kBlock),
-                               0, s, src, np, tgt, nt, eps2, d_acc);
-        else
-            hipLaunchKernelGGL(k_direct_f64<false>, dim3(blocks_for(nt, kBlock)),
-                               dim3(kBlock), 0, s, src, np, tgt, nt, eps2, d_acc);
-        return check_launch(c, "k_direct_f64");
+        const int tpw = f64_targets_per_wave(c, nt);
+        if (tpw == 64) {
+            if (self)
+                hipLaunchKernelGGL(k_direct_f64<true>, dim3(blocks_for(nt, kBlock)),
+                                   dim3(kBlock), 0, s, src, np, tgt, nt, eps2, d_acc);
+            else
+                hipLaunchKernelGGL(k_direct_f64<false>, dim3(blocks_for(nt, kBlock)),
+                                   dim3(kBlock), 0, s, src, np, tgt, nt, eps2, d_acc);
+            return check_launch(c, "k_direct_f64");
+        }
+        const dim3 grid(blocks_for((nt + tpw - 1) / tpw, kBlock / 64));
+#define GH_SPREAD(T)                                                                      \
+    do {                                                                                  \
+        if (self)                                                                         \
+            hipLaunchKernelGGL((k_direct_f64_spread<true, T>), grid, dim3(kBlock), 0, s,  \
+                               src, np, tgt, nt, eps2, d_acc);                            \
+        else                                                                              \
+            hipLaunchKernelGGL((k_direct_f64_spread<false, T>), grid, dim3(kBlock), 0, s, \
+                               src, np, tgt, nt, eps2, d_acc);                            \
+    } while (0)
+        switch (tpw) {
+        case 1: GH_SPREAD(1); break;
+        case 2: GH_SPREAD(2); break;
+        case 4: GH_SPREAD(4); break;
+        case 8: GH_SPREAD(8); break;
+        case 16: GH_SPREAD(16); break;
+        default: GH_SPREAD(32); break;
+        }
+#undef GH_SPREAD
+        return check_launch(c, "k_direct_f64_spread");
     }
     if (precision != GH_FP32) return fail(c, GH_EARG, "unknown precision %d", precision);
     const int sym = self ? sym_ipt(c, np) : 0;

@@ -215,19 +215,23 @@ int gh_timing_read(gh_ctx *ctx, int kernel_class, double *total_ms, int64_t *lau
  *   GH_TUNE_WALK_SCHEDULE  FP32 tree walk: 0 automatic (static), -1 static (one unit of
  *                        targets per wavefront), 1 dynamic (wavefronts take units from
  *                        per-XCD atomic queues)
+ *   GH_TUNE_DIRECT_F64_TPW  FP64 direct sum: 0 automatic, 1/2/4/8/16/32/64 targets per
+ *                        wavefront (64: one target per lane; fewer: the terms of a
+ *                        target are computed by 64/value lanes and then added in order)
  *   GH_TUNE_ALLOC_LIMIT_MB  cap, in MiB, on the device memory the context's arenas may
  *                        hold (0: no cap); a call that would exceed it fails with
  *                        GH_ENOMEM and leaves the context usable
- * The direct-sum keys change results only by FP32 rounding; the walk keys and
- * GH_TUNE_RUN_GRAPH do not change results; GH_TUNE_ALLOC_LIMIT_MB only decides
- * whether a call may allocate. */
+ * The FP32 direct-sum keys change results only by FP32 rounding; GH_TUNE_DIRECT_F64_TPW,
+ * the walk keys and GH_TUNE_RUN_GRAPH do not change results; GH_TUNE_ALLOC_LIMIT_MB
+ * only decides whether a call may allocate. */
 #define GH_TUNE_DIRECT_SELF 0
 #define GH_TUNE_DIRECT_IPT 1
 #define GH_TUNE_WALKS_PER_WAVE 2
 #define GH_TUNE_RUN_GRAPH 3
 #define GH_TUNE_WALK_SCHEDULE 4
 #define GH_TUNE_ALLOC_LIMIT_MB 5
-#define GH_TUNE_COUNT 6
+#define GH_TUNE_DIRECT_F64_TPW 6
+#define GH_TUNE_COUNT 7
 int gh_set_tuning(gh_ctx *ctx, int key, int value);
 
 /* FP64 diagnostics (no reference counterpart): kinetic energy sum 0.5 m v^2

@@ -148,3 +148,22 @@ def test_fp32_self_kernels_agree_unequal_masses(ctx):
     ctx.set_tuning(_native.TUNE_DIRECT_SELF, 0)
     for ipt in (1, 4):
         assert np.median(rel_err(out[ipt], out[-1])) < 1e-6
+
+
+@pytest.mark.parametrize("tpw", [1, 2, 4, 8, 16, 32, 64])
+def test_fp64_direct_targets_per_wave_bitwise(ctx, oracle, tpw):
+    # every spread of a target's terms over lanes adds them in the reference's order
+    rng = np.random.default_rng(300 + tpw)
+    n = 777
+    pos = rng.standard_cauchy(size=(n, 3))
+    mass = rng.uniform(0.1, 3.0, size=n)
+    fpos = np.concatenate([pos[:9], rng.normal(size=(40, 3))])
+    try:
+        ctx.set_tuning(_native.TUNE_DIRECT_F64_TPW, tpw)
+        for eps in (0.0, 1e-3):
+            assert_bitwise(ctx.direct(pos, mass, eps, FP64), oracle.direct(pos, mass, eps),
+                           "self eps=%g" % eps)
+            assert_bitwise(ctx.direct(pos, mass, eps, FP64, force_pos=fpos),
+                           oracle.direct_position(pos, mass, fpos, eps), "position eps=%g" % eps)
+    finally:
+        ctx.set_tuning(_native.TUNE_DIRECT_F64_TPW, 0)
