@@ -51,6 +51,24 @@ def traffic_from_profiles(name):
         return None
 
 
+def host_description():
+    """CPU model and the cores this process may use (the baselines run on one of them)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count()
+    return {"cpu_model": model, "usable_cores": usable, "baseline_cores": 1}
+
+
 def cpu_baseline(budget_s=30.0):
     """The reference's own C direct sum (gravhopper/_jbgrav.c, built from its source
     into oracle/_ref) on one host core; falls back to the oracle restatement."""
@@ -541,7 +559,9 @@ def main():
     result["extras"] = extras
     if not args.no_cpu_baseline:
         log("bench: cpu baseline")
+        host = host_description()  # before the baselines pin this process to one core
         result["cpu_baseline"] = cpu_baseline()
+        result["cpu_baseline"]["host"] = host
         if not args.no_extras:
             log("bench: cpu baseline, reference direct sum at N=2000")
             c1 = cpu_config1_baseline()
