@@ -62,6 +62,8 @@ struct NodeF {
 };
 typedef float f32x8 __attribute__((ext_vector_type(8)));  // a NodeF as one load
 static_assert(sizeof(NodeF) == sizeof(f32x8), "NodeF is 32 bytes");
+typedef double f64x8 __attribute__((ext_vector_type(8)));  // a NodeD as one load
+static_assert(sizeof(NodeD) == sizeof(f64x8), "NodeD is 64 bytes");
 
 struct DD4 {
     double h[4], l[4];
@@ -974,31 +976,35 @@ __global__ __launch_bounds__(kBlock) void k_walk_f64(const NodeD* __restrict__ n
     double ax = 0.0, ay = 0.0, az = 0.0;
     int cnt = 0;
     int my_next = live ? 0 : 0x7fffffff;
+    const int n_nodes32 = (int)n_nodes;  // below 2^31 (checked by the build)
     int node = 0;
-    while (node < n_nodes) {
-        const NodeD nd = nodes[node];
+    while (node < n_nodes32) {
+        // the whole record (one 64-byte scalar load), its leaf flag and skip pointer,
+        // issued together
+        const f64x8 nd = *(const f64x8*)(nodes + node);
         const int sk = skip[node];
+        const bool is_leaf = leaf[node] >= 0;
         const bool active = (my_next == node);
         bool take = active;
         int next = node + 1;
-        if (leaf[node] < 0) {  // internal node (a uniform test): opening test
-            const double ddx = nd.cx - x, ddy = nd.cy - y, ddz = nd.cz - z;
+        if (!is_leaf) {  // internal node (a uniform test): opening test
+            const double ddx = nd[0] - x, ddy = nd[1] - y, ddz = nd[2] - z;
             double d2 = ddx * ddx;
             d2 = d2 + ddy * ddy;
             d2 = d2 + ddz * ddz;
-            const double s2 = nd.size * nd.size;
+            const double s2 = nd[3] * nd[3];
             bool far = d2 > s2 * c_hi;
             const bool unsure = active && !far && !(d2 < s2 * c_lo);
-            if (__ballot(unsure) != 0ull) {
-                if (unsure) far = (nd.size / sqrt(d2)) < theta;
+            if (__builtin_amdgcn_uicmp((u32)unsure, 0u, 33) != 0ull) {
+                if (unsure) far = (nd[3] / sqrt(d2)) < theta;
             }
             const bool opens = active && !far;
             take = active && far;
             if (opens) my_next = node + 1;
-            next = (__ballot(opens) != 0ull) ? node + 1 : sk;
+            next = (__builtin_amdgcn_uicmp((u32)opens, 0u, 33) != 0ull) ? node + 1 : sk;
         }
         if (take) {
-            const double dx = nd.mx - x, dy = nd.my - y, dz = nd.mz - z;
+            const double dx = nd[4] - x, dy = nd[5] - y, dz = nd[6] - z;
             double d2 = dx * dx;
             d2 = d2 + dy * dy;
             d2 = d2 + dz * dz;
@@ -1008,9 +1014,10 @@ __global__ __launch_bounds__(kBlock) void k_walk_f64(const NodeD* __restrict__ n
                 const double r = rsqrt_f64(d);
                 inv = r * r * r;
             }
-            ax = ax + dx * nd.mass * inv;
-            ay = ay + dy * nd.mass * inv;
-            az = az + dz * nd.mass * inv;
+            const double mi = nd[7] * inv;
+            ax = ax + dx * mi;
+            ay = ay + dy * mi;
+            az = az + dz * mi;
             cnt++;
             my_next = sk;
         }
