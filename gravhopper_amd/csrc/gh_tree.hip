@@ -29,10 +29,11 @@
 //                     end+1
 //   6. moments        node mass and first moment as differences of double-double prefix
 //                     sums over the sorted particles (error far below one FP64 ulp)
-//   7. walk           stackless pre-order walk with skip pointers.  A wavefront walks 64
-//                     targets in lockstep: every lane keeps its own next node, the wave
-//                     visits the smallest of them (next = n+1 if any active lane opened
-//                     n, else skip[n]), so each lane accepts exactly the nodes the
+//   7. walk           stackless pre-order walk with skip pointers.  A wavefront walks up
+//                     to 64 targets in lockstep: the wave visits nodes in increasing
+//                     order (next = n+1 if any waiting lane opened n, else skip[n]); a
+//                     lane waits for every node from the end of the last subtree it
+//                     accepted on, so each lane accepts exactly the nodes the
 //                     reference's recursive walk (:487-541) accepts, while the node index
 //                     stays wave-uniform and node records arrive through scalar loads.
 #include <algorithm>
@@ -792,7 +793,7 @@ __device__ __forceinline__ void walk_unit_f32(const char* __restrict__ nodes,
     int64_t t[WPW];
     float x[WPW], y[WPW], z[WPW], fx[WPW], fy[WPW], fz[WPW];
     double ax[WPW], ay[WPW], az[WPW];
-    int my_next[WPW], node[WPW];
+    int my_skip[WPW], node[WPW];
 #pragma unroll
     for (int w = 0; w < WPW; w++) {
         const int64_t k = base + w * lanes;
@@ -807,7 +808,7 @@ __device__ __forceinline__ void walk_unit_f32(const char* __restrict__ nodes,
         }
         fx[w] = fy[w] = fz[w] = 0.f;
         ax[w] = ay[w] = az[w] = 0.0;
-        my_next[w] = live ? 0 : 0x7fffffff;
+        my_skip[w] = live ? 0 : 0x7fffffff;
         // a walk with no live lane never starts
         node[w] = __builtin_amdgcn_readfirstlane((first + w * lanes < nt) ? 0 : n_nodes);
     }
@@ -829,24 +830,18 @@ __device__ __forceinline__ void walk_unit_f32(const char* __restrict__ nodes,
 #pragma unroll
         for (int w = 0; w < WPW; w++) {
             if (node[w] >= n_nodes) continue;  // this walk is done
-            // Opening test in every lane waiting for this node (my_next == node; the
-            // other lanes accepted an ancestor).  A leaf has geo.w = -1, so it is
-            // always far, i.e. accepted.  Lane masks come straight from the compares
-            // (predicates 32: equal, 2: ordered greater).
+            // Opening test in every lane waiting for this node: node >= my_skip, the end
+            // of the last subtree the lane accepted (nodes are visited in increasing
+            // order, so a lane that opened this node's ancestors is waiting, one that
+            // accepted an ancestor is not, and opening needs no update).  A leaf has
+            // geo.w = -1, so it is always far, i.e. accepted.  Lane masks come straight
+            // from the compares (predicates 37: unsigned <=, 2: ordered greater).
             const float cdx = nd[w][0] - x[w], cdy = nd[w][1] - y[w], cdz = nd[w][2] - z[w];
             const float d2 = __builtin_fmaf(cdx, cdx, __builtin_fmaf(cdy, cdy, cdz * cdz));
-            const u64 waiting = __builtin_amdgcn_uicmp((u32)my_next[w], (u32)node[w], 32);
+            const u64 waiting = __builtin_amdgcn_uicmp((u32)my_skip[w], (u32)node[w], 37);
             const u64 far = __builtin_amdgcn_fcmpf(d2, nd[w][3], 2);
             const u64 opened = waiting & ~far;  // the wave descends if any lane opens
-            const bool take = (my_next[w] == node[w]) && (d2 > nd[w][3]);
-            // my_next = node + 1 in the lanes of `opened`: one select on the lane mask
-            {
-                int r;
-                asm volatile("v_cndmask_b32_e64 %0, %1, %2, %3"
-                             : "=v"(r)
-                             : "v"(my_next[w]), "v"(node[w] + 1), "s"(opened));
-                my_next[w] = r;
-            }
+            const bool take = ((u32)my_skip[w] <= (u32)node[w]) && (d2 > nd[w][3]);
             int next = opened ? node[w] + 1 : sk[w];
             // uniform by construction; readfirstlane lets the compiler keep it scalar
             next = __builtin_amdgcn_readfirstlane(next);
@@ -861,7 +856,7 @@ __device__ __forceinline__ void walk_unit_f32(const char* __restrict__ nodes,
                 fx[w] = __builtin_fmaf(dx, sc, fx[w]);
                 fy[w] = __builtin_fmaf(dy, sc, fy[w]);
                 fz[w] = __builtin_fmaf(dz, sc, fz[w]);
-                my_next[w] = sk[w];
+                my_skip[w] = sk[w];
             }
             node[w] = next;
         }
@@ -975,7 +970,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_f64(const NodeD* __restrict__ n
     const double eps2 = eps * eps;
     double ax = 0.0, ay = 0.0, az = 0.0;
     int cnt = 0;
-    int my_next = live ? 0 : 0x7fffffff;
+    int my_skip = live ? 0 : 0x7fffffff;  // as in the FP32 walk: waiting iff node >= my_skip
     const int n_nodes32 = (int)n_nodes;  // below 2^31 (checked by the build)
     int node = 0;
     while (node < n_nodes32) {
@@ -984,7 +979,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_f64(const NodeD* __restrict__ n
         const f64x8 nd = *(const f64x8*)(nodes + node);
         const int sk = skip[node];
         const bool is_leaf = leaf[node] >= 0;
-        const bool active = (my_next == node);
+        const bool active = (u32)my_skip <= (u32)node;
         bool take = active;
         int next = node + 1;
         if (!is_leaf) {  // internal node (a uniform test): opening test
@@ -1000,7 +995,6 @@ __global__ __launch_bounds__(kBlock) void k_walk_f64(const NodeD* __restrict__ n
             }
             const bool opens = active && !far;
             take = active && far;
-            if (opens) my_next = node + 1;
             next = (__builtin_amdgcn_uicmp((u32)opens, 0u, 33) != 0ull) ? node + 1 : sk;
         }
         if (take) {
@@ -1019,7 +1013,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_f64(const NodeD* __restrict__ n
             ay = ay + dy * mi;
             az = az + dz * mi;
             cnt++;
-            my_next = sk;
+            my_skip = sk;
         }
         node = __builtin_amdgcn_readfirstlane(next);
     }
