@@ -62,6 +62,20 @@ def test_topology_bitwise(ctx, oracle, case):
     compare_topology(ctx.tree_dump(pos, mass, eps), oracle.tree_dump(pos, mass, eps), name)
 
 
+@pytest.mark.parametrize("n", [4096, 4097, 65536, 70000])
+def test_topology_bitwise_search_levels(ctx, oracle, n):
+    # node ends come from a search over group minima whose number of levels grows with
+    # n (3 up to 4096 particles, 4 up to 65536, 5 beyond): sizes on either side of the
+    # steps, clustered so that large nodes reach the upper levels
+    rng = np.random.default_rng(n)
+    pos = np.concatenate([rng.normal(size=(n // 2, 3)) * 1e-3,
+                          rng.standard_cauchy(size=(n - n // 2, 3))])
+    mass = rng.uniform(1, 2, n)
+    print("n", n, flush=True)
+    compare_topology(ctx.tree_dump(pos, mass, 1e-3), oracle.tree_dump(pos, mass, 1e-3),
+                     "n=%d" % n)
+
+
 @pytest.mark.parametrize("case", tree_cases(), ids=lambda c: c[0])
 def test_fp64_walk_matches_reference(ctx, oracle, case):
     name, pos, mass, eps = case
