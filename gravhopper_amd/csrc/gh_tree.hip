@@ -256,39 +256,52 @@ __global__ __launch_bounds__(kBlock) void k_rs_scan(u32* __restrict__ hist, int 
 
 // Stable scatter of one 8-bit digit.  Every element's output position is its digit's
 // base (all elements of smaller digits) + the elements of the same digit in earlier
-// blocks (hist, scanned) + its rank inside the block (wave ballots, then waves in
-// order).  STAGE (one key word only): the block's elements are first placed in LDS in
-// output order and then written out by consecutive threads, so each digit's run of the
-// block leaves as contiguous stores; otherwise every element is stored directly.
+// blocks (hist, scanned) + its rank inside the block.  Each wavefront of the block owns
+// a contiguous range of 64 * kRsItems elements and ranks them by itself (ballots per
+// 64 elements, running per-digit counts in a row of LDS that only this wavefront
+// touches), so the block synchronises only to add the counts of the earlier
+// wavefronts.  STAGE (one key word only): the block's elements are first placed in LDS
+// in output order and then written out by consecutive threads, so each digit's run of
+// the block leaves as contiguous stores; otherwise every element is stored directly.
 template <bool STAGE>
 __global__ __launch_bounds__(kBlock) void k_rs_scatter(
     const u64* __restrict__ kin, u64* __restrict__ kout, const u32* __restrict__ iin,
     u32* __restrict__ iout, int64_t n, int words, int wsel, int shift,
     const u32* __restrict__ hist, int nblocks, const u32* __restrict__ totals)
 {
-    __shared__ u32 gbase[256];   // running output position per digit
-    __shared__ u32 gbase0[256];  // this block's first output position per digit
-    __shared__ u32 lstart[256];  // this block's first LDS slot per digit (STAGE)
-    __shared__ u32 wcount[kBlock / 64][256];
-    __shared__ u32 woff[kBlock / 64][256];
+    constexpr int kWaves = kBlock / 64;
+    __shared__ u32 gbase0[256];          // this block's first output position per digit
+    __shared__ u32 lstart[256];          // this block's first LDS slot per digit (STAGE)
+    __shared__ u32 wcount[kWaves][256];  // per-wave running counts, then wave offsets
     __shared__ u64 stage_k[STAGE ? kRsChunk : 1];
     __shared__ u32 stage_i[STAGE ? kRsChunk : 1];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int dd = threadIdx.x;  // one digit per thread in the per-digit steps
-    // digit bases: exclusive scan of the 256 digit totals
+    const int64_t base = (int64_t)blockIdx.x * kRsChunk;
+    const int64_t wbase = base + (int64_t)wave * 64 * kRsItems;  // this wave's range
+    // every item's digit word and index is loaded up front, so that the loads are in
+    // flight together while the digit bases are formed
+    u64 kw[kRsItems];
+    u32 id[kRsItems];
+#pragma unroll
+    for (int it = 0; it < kRsItems; it++) {
+        const int64_t i = wbase + it * 64 + lane;
+        kw[it] = i < n ? kin[(int64_t)wsel * n + i] : 0ull;
+        id[it] = i < n ? iin[i] : 0u;
+    }
+    // digit bases: exclusive scan of the 256 digit totals, plus earlier blocks
     {
         const u32 tot = totals[dd];
-        gbase[dd] = tot;
+        gbase0[dd] = tot;
         __syncthreads();
         for (int off = 1; off < 256; off <<= 1) {
-            const u32 t = dd >= off ? gbase[dd - off] : 0u;
+            const u32 t = dd >= off ? gbase0[dd - off] : 0u;
             __syncthreads();
-            gbase[dd] += t;
+            gbase0[dd] += t;
             __syncthreads();
         }
-        const u32 first = gbase[dd] - tot + hist[(int64_t)dd * nblocks + blockIdx.x];
+        const u32 first = gbase0[dd] - tot + hist[(int64_t)dd * nblocks + blockIdx.x];
         __syncthreads();
-        gbase[dd] = first;
         gbase0[dd] = first;
     }
     if (STAGE) {
@@ -308,23 +321,14 @@ __global__ __launch_bounds__(kBlock) void k_rs_scatter(
         }
         lstart[dd] -= cnt;
     }
-    for (int w = 0; w < kBlock / 64; w++) wcount[w][dd] = 0;
-    const int64_t base = (int64_t)blockIdx.x * kRsChunk;
-    // every item's digit word and index is loaded up front, so that the loads are in
-    // flight together instead of one round trip per ranking step
-    u64 kw[kRsItems];
-    u32 id[kRsItems];
-#pragma unroll
-    for (int it = 0; it < kRsItems; it++) {
-        const int64_t i = base + (int64_t)it * kBlock + threadIdx.x;
-        kw[it] = i < n ? kin[(int64_t)wsel * n + i] : 0ull;
-        id[it] = i < n ? iin[i] : 0u;
-    }
+    for (int w = 0; w < kWaves; w++) wcount[w][dd] = 0u;
     __syncthreads();
+    // rank inside the wave: earlier items of this wave, then earlier lanes of this item
     const u64 lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    u32 dig[kRsItems], rank[kRsItems];
 #pragma unroll
     for (int it = 0; it < kRsItems; it++) {
-        const int64_t i = base + (int64_t)it * kBlock + threadIdx.x;
+        const int64_t i = wbase + it * 64 + lane;
         const bool valid = i < n;
         const u32 d = valid ? (u32)((kw[it] >> shift) & 255u) : 0u;
         u64 peers = __ballot(valid);
@@ -332,34 +336,42 @@ __global__ __launch_bounds__(kBlock) void k_rs_scatter(
             const u64 m = __ballot((d >> b) & 1u);
             peers &= ((d >> b) & 1u) ? m : ~m;
         }
-        const u32 rank = (u32)__popcll(peers & lt);
-        if (valid && rank == 0) wcount[wave][d] = (u32)__popcll(peers);
-        __syncthreads();
-        {
-            u32 run = gbase[dd];
-            for (int w = 0; w < kBlock / 64; w++) {
-                woff[w][dd] = run;
-                run += wcount[w][dd];
-                wcount[w][dd] = 0;
-            }
-            gbase[dd] = run;
+        // every lane reads its digit's count before the leader of the digit adds this
+        // item's peers (LDS operations of one wavefront take effect in program order)
+        const u32 before = wcount[wave][d];
+        if (valid && (peers & lt) == 0ull) wcount[wave][d] = before + (u32)__popcll(peers);
+        dig[it] = d;
+        rank[it] = before + (u32)__popcll(peers & lt);
+    }
+    __syncthreads();
+    {
+        u32 run = 0u;
+        for (int w = 0; w < kWaves; w++) {
+            const u32 t = wcount[w][dd];
+            wcount[w][dd] = run;
+            run += t;
         }
-        __syncthreads();
-        if (valid) {
-            const u32 dst = woff[wave][d] + rank;
-            if (STAGE) {
-                const u32 slot = lstart[d] + (dst - gbase0[d]);
-                stage_k[slot] = kw[it];
-                stage_i[slot] = id[it];
-            } else {
-                for (int w = 0; w < words; w++)
-                    kout[(int64_t)w * n + dst] = (w == wsel) ? kw[it] : kin[(int64_t)w * n + i];
-                iout[dst] = id[it];
-            }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < kRsItems; it++) {
+        const int64_t i = wbase + it * 64 + lane;
+        if (i >= n) continue;
+        const u32 d = dig[it];
+        const u32 inblk = wcount[wave][d] + rank[it];  // rank among the block's digit d
+        if (STAGE) {
+            const u32 slot = lstart[d] + inblk;
+            stage_k[slot] = kw[it];
+            stage_i[slot] = id[it];
+        } else {
+            const u32 dst = gbase0[d] + inblk;
+            for (int w = 0; w < words; w++)
+                kout[(int64_t)w * n + dst] = (w == wsel) ? kw[it] : kin[(int64_t)w * n + i];
+            iout[dst] = id[it];
         }
-        __syncthreads();
     }
     if (STAGE) {
+        __syncthreads();
         // LDS slot e holds the block's e-th element in output order
         const int64_t m = (n - base) < kRsChunk ? (n - base) : kRsChunk;
         for (int64_t e = threadIdx.x; e < m; e += kBlock) {
