@@ -941,19 +941,19 @@ __global__ void k_zero_queue(u32* __restrict__ q)
 //    theta <= 0: nothing is accepted, as size/dist < theta never holds).
 //  - Leaves are accepted without the test (:502).
 //  - Monopole (:506-524): inv = (1/d)/sqrt(d), d = |com - x|^2 + eps^2, 0 if d == 0, here
-//    as r^3 with r = 1/sqrt(d) from v_rsq_f64 and two Newton steps (a few ulp), which
-//    is far inside the FP64 walk's tolerance against the reference (1e-10), set by
-//    moment rounding and summation order anyway.
-// counts (nullable) receives the accepted-node count per target.
+//    as r^3 with r = 1/sqrt(d) from v_rsq_f64 and one Newton step.  Measured on gfx950
+//    over 2^20 values of d in [1e-30, 1e30] and [1, 4): v_rsq_f64 alone is within
+//    2.5e8 ulp, one step within 19.4 ulp (4e-15), two within 1.7 ulp.  One step is far
+//    inside the FP64 walk's tolerance against the reference (1e-10), which moment
+//    rounding and summation order set anyway, and 4 % faster than two.
+// COUNT: counts receives the accepted-node count per target.
 __device__ __forceinline__ double rsqrt_f64(double d)
 {
-    double r = __builtin_amdgcn_rsq(d);
-    const double h = 0.5 * d;
-#pragma unroll
-    for (int k = 0; k < 2; k++) r = r * __builtin_fma(-h, r * r, 1.5);
-    return r;
+    const double r = __builtin_amdgcn_rsq(d);
+    return r * __builtin_fma(-0.5 * d, r * r, 1.5);
 }
 
+template <bool COUNT>
 __global__ __launch_bounds__(kBlock) void k_walk_f64(const NodeD* __restrict__ nodes,
                                                      const int* __restrict__ skip,
                                                      const int* __restrict__ leaf,
@@ -1024,7 +1024,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_f64(const NodeD* __restrict__ n
             ax = ax + dx * mi;
             ay = ay + dy * mi;
             az = az + dz * mi;
-            cnt++;
+            if (COUNT) cnt++;
             my_skip = sk;
         }
         node = __builtin_amdgcn_readfirstlane(next);
@@ -1033,7 +1033,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_f64(const NodeD* __restrict__ n
         acc[3 * t] = ax;
         acc[3 * t + 1] = ay;
         acc[3 * t + 2] = az;
-        if (counts) counts[t] = cnt;
+        if (COUNT) counts[t] = cnt;
     }
 }
 
@@ -1252,10 +1252,16 @@ static int walk(gh_ctx* c, const double* tgt, bool self, int64_t nt, double eps,
         const double inf = __builtin_huge_val();
         const double c_lo = theta > 0.0 ? (1.0 - 1e-13) / (theta * theta) : inf;
         const double c_hi = theta > 0.0 ? (1.0 + 1e-13) / (theta * theta) : inf;
-        hipLaunchKernelGGL(k_walk_f64, dim3((unsigned)(per * kXcds)), dim3(kBlock), 0, s,
-                           tb->nodes64.as<NodeD>(), tb->skip.as<int>(), tb->leaf.as<int>(),
-                           tb->n_nodes, tgt, order, nt, eps, theta, c_lo, c_hi, d_acc, counts,
-                           per, lanes);
+        if (counts)
+            hipLaunchKernelGGL(k_walk_f64<true>, dim3((unsigned)(per * kXcds)), dim3(kBlock),
+                               0, s, tb->nodes64.as<NodeD>(), tb->skip.as<int>(),
+                               tb->leaf.as<int>(), tb->n_nodes, tgt, order, nt, eps, theta,
+                               c_lo, c_hi, d_acc, counts, per, lanes);
+        else
+            hipLaunchKernelGGL(k_walk_f64<false>, dim3((unsigned)(per * kXcds)), dim3(kBlock),
+                               0, s, tb->nodes64.as<NodeD>(), tb->skip.as<int>(),
+                               tb->leaf.as<int>(), tb->n_nodes, tgt, order, nt, eps, theta,
+                               c_lo, c_hi, d_acc, counts, per, lanes);
         return check_launch(c, "k_walk_f64");
     }
     const float eps2 = (float)(eps * eps);
