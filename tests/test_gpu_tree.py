@@ -214,3 +214,20 @@ def test_fp32_walk_few_targets(ctx, oracle):
         want = oracle.tree(pos, mass, 0.01, 0.7, force_pos=fpos)
         got = ctx.tree(pos, mass, 0.01, 0.7, FP32, force_pos=fpos)
         assert np.median(rel_err(got, want)) < FP32_TREE_MEDIAN, nt
+
+
+def test_fp32_walk_zero_softening(ctx, oracle):
+    # eps = 0: the walk's guarded form, where a zero separation (a particle's own leaf,
+    # or a target on a particle) gives zero force, as in the reference's tree walk
+    rng = np.random.default_rng(89)
+    pos = rng.normal(size=(6000, 3))
+    mass = rng.uniform(1, 2, 6000)
+    want = oracle.tree(pos, mass, 0.0, 0.7)
+    got = ctx.tree(pos, mass, 0.0, 0.7, FP32)
+    assert np.all(np.isfinite(got))
+    assert np.median(rel_err(got, want)) < FP32_TREE_MEDIAN
+    fpos = np.concatenate([pos[:50], rng.normal(size=(250, 3))])
+    want = oracle.tree(pos, mass, 0.0, 0.7, force_pos=fpos)
+    got = ctx.tree(pos, mass, 0.0, 0.7, FP32, force_pos=fpos)
+    assert np.all(np.isfinite(got))
+    assert np.median(rel_err(got, want)) < FP32_TREE_MEDIAN
