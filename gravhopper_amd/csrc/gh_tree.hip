@@ -786,6 +786,10 @@ constexpr int kXcds = 8;
 constexpr int kFlushVisits = 64;
 constexpr int64_t kSortTargetsMin = 4096;  // position targets sorted from this many on
 constexpr int64_t kWalkWaves = 4096;       // walks in flight for small target counts
+// Threads per walk workgroup: one wavefront, so that the hardware refills a SIMD's slot
+// as soon as its walk ends rather than when the slowest of four walks does (1.5 %
+// faster than 256-thread workgroups at N=2^20, FP32 and FP64).
+constexpr int kWalkBlock = 64;
 __device__ __forceinline__ int64_t xcd_block(int64_t per_xcd)
 {
     return (int64_t)(blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
@@ -904,7 +908,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_f32(const NodeF* __restrict__ n
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const char* nb = (const char*)nodes;
     if (!DYN) {
-        const int64_t unit = xcd_block(per_xcd) * (kBlock / 64) + wave;
+        const int64_t unit = xcd_block(per_xcd) * (kWalkBlock / 64) + wave;
         if (unit < n_units)
             walk_unit_f32<WPW, GUARD>(nb, skip, n_nodes, tgt, order, nt, eps2, acc, unit, lane,
                                       lanes);
@@ -967,7 +971,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_f64(const NodeD* __restrict__ n
 {
     // one walk of `lanes` consecutive targets per wavefront (lanes <= 64)
     const int lane = threadIdx.x & 63;
-    const int64_t first = (xcd_block(per_xcd) * (kBlock / 64) + (threadIdx.x >> 6)) * lanes;
+    const int64_t first = (xcd_block(per_xcd) * (kWalkBlock / 64) + (threadIdx.x >> 6)) * lanes;
     if (first >= nt) return;  // whole wavefront idle
     const int64_t k = first + lane;
     const bool live = lane < lanes && k < nt;
@@ -1245,7 +1249,7 @@ static int walk(gh_ctx* c, const double* tgt, bool self, int64_t nt, double eps,
     // long walks would leave most of the GPU idle).
     const int lanes = (int)std::min<int64_t>(64, std::max<int64_t>(1, (nt + kWalkWaves - 1) /
                                                                           kWalkWaves));
-    const int waves_per_block = kBlock / 64;
+    const int waves_per_block = kWalkBlock / 64;
     if (precision == GH_FP64) {
         const int64_t n_walks = (nt + lanes - 1) / lanes;
         const int64_t per = (blocks_for(n_walks, waves_per_block) + kXcds - 1) / kXcds;
@@ -1253,12 +1257,12 @@ static int walk(gh_ctx* c, const double* tgt, bool self, int64_t nt, double eps,
         const double c_lo = theta > 0.0 ? (1.0 - 1e-13) / (theta * theta) : inf;
         const double c_hi = theta > 0.0 ? (1.0 + 1e-13) / (theta * theta) : inf;
         if (counts)
-            hipLaunchKernelGGL(k_walk_f64<true>, dim3((unsigned)(per * kXcds)), dim3(kBlock),
+            hipLaunchKernelGGL(k_walk_f64<true>, dim3((unsigned)(per * kXcds)), dim3(kWalkBlock),
                                0, s, tb->nodes64.as<NodeD>(), tb->skip.as<int>(),
                                tb->leaf.as<int>(), tb->n_nodes, tgt, order, nt, eps, theta,
                                c_lo, c_hi, d_acc, counts, per, lanes);
         else
-            hipLaunchKernelGGL(k_walk_f64<false>, dim3((unsigned)(per * kXcds)), dim3(kBlock),
+            hipLaunchKernelGGL(k_walk_f64<false>, dim3((unsigned)(per * kXcds)), dim3(kWalkBlock),
                                0, s, tb->nodes64.as<NodeD>(), tb->skip.as<int>(),
                                tb->leaf.as<int>(), tb->n_nodes, tgt, order, nt, eps, theta,
                                c_lo, c_hi, d_acc, counts, per, lanes);
@@ -1295,7 +1299,7 @@ static int walk(gh_ctx* c, const double* tgt, bool self, int64_t nt, double eps,
         grid = dim3((unsigned)(nblk < resident ? nblk : resident));
     }
 #define GH_WALK32(W, G, D)                                                                 \
-    hipLaunchKernelGGL((k_walk_f32<W, G, D>), grid, dim3(kBlock), 0, s, nd, sk, T, tgt,  \
+    hipLaunchKernelGGL((k_walk_f32<W, G, D>), grid, dim3(kWalkBlock), 0, s, nd, sk, T, tgt, \
                        order, nt, eps2, d_acc, n_units, per, q, lanes)
     if (wpw == 2) {
         if (dyn) {
