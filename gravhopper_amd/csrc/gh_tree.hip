@@ -73,7 +73,7 @@ struct DD4 {
 struct TreeBufs {
     DevBuf keys[2];  // W words per element, word-major: keys[w * n + i]
     DevBuf idx[2];
-    DevBuf hist, bbox_part, box, flags, delta, off, prefix, block_sums, block_counts;
+    DevBuf hist, bbox_part, box, flags, delta, off, prefix, block_sums, block_counts, packed;
     DevBuf spos;  // (x, y, z, m) of the particles in sorted order
     DevBuf nodes64, nodes32, skip, level, branch, leaf;
     DevBuf tkeys[2], tidx[2];  // Morton order of position targets (tree_force_position)
@@ -180,15 +180,21 @@ __device__ __forceinline__ int descend(double x, double y, double z, double& cx,
     return (int)gx | ((int)gy << 1) | ((int)gz << 2);
 }
 
+// packed (nullable) receives (x, y, z, m) of each particle in input order, so that the
+// gather into sorted order after the sort fetches one 32-byte record per particle
+// rather than two scattered pieces (position and mass).
 __global__ __launch_bounds__(kBlock) void k_keys(const double* __restrict__ pos, int64_t n,
                                                  const double* __restrict__ box, int words,
                                                  u64* __restrict__ keys, u32* __restrict__ idx,
-                                                 int* __restrict__ flags)
+                                                 int* __restrict__ flags,
+                                                 const double* __restrict__ mass,
+                                                 double4* __restrict__ packed)
 {
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (flags && i < 4) flags[i] = 0;
     if (i >= n) return;
     const double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
+    if (packed) packed[i] = make_double4(x, y, z, mass[i]);
     double cx = box[0], cy = box[1], cz = box[2];
     double size = box[3];
     for (int w = 0; w < words; w++) {
@@ -407,9 +413,8 @@ __device__ __forceinline__ int shared_levels_plus1(const u64* __restrict__ keys,
 
 struct DeltaArgs {
     const u64* keys;    // sorted keys, word-major
-    const u32* perm;    // sorted position -> particle
-    const double* pos;  // particles (x, y, z)
-    const double* mass;
+    const u32* perm;         // sorted position -> particle
+    const double4* packed;   // particles (x, y, z, m), input order
     unsigned char* search;  // end-search buffer: dp = level 0, then levels 1, 2, ...
     long long* cnt;         // nodes starting at each sorted position
     double4* spos;          // sorted (x, y, z, m)
@@ -437,13 +442,12 @@ __global__ __launch_bounds__(kBlock) void k_delta(DeltaArgs a, int64_t n, int wo
     if (i < n) {
         dp[i] = (unsigned char)m;
         const u32 p = a.perm[i];
-        a.spos[i] = make_double4(a.pos[3 * p], a.pos[3 * p + 1], a.pos[3 * p + 2], a.mass[p]);
+        const double4 r = a.packed[p];
+        a.spos[i] = r;
         if (m == words * kLevelsPerWord + 1) {
             atomicAdd(&a.flags[0], 1);
-            const u32 q = a.perm[i + 1];
-            if (a.pos[3 * p] == a.pos[3 * q] && a.pos[3 * p + 1] == a.pos[3 * q + 1] &&
-                a.pos[3 * p + 2] == a.pos[3 * q + 2])
-                atomicAdd(&a.flags[1], 1);
+            const double4 o = a.packed[a.perm[i + 1]];
+            if (r.x == o.x && r.y == o.y && r.z == o.z) atomicAdd(&a.flags[1], 1);
         }
     }
     lane_m[threadIdx.x] = m;
@@ -1118,6 +1122,7 @@ static int build_tree(gh_ctx* c, const double* d_pos, const double* d_mass, int6
     GH_TRY(ensure(c, tb->block_counts, sizeof(long long) * (size_t)(nb + 1)));
     GH_TRY(ensure(c, tb->prefix, sizeof(DD4) * (size_t)(n + 1)));
     GH_TRY(ensure(c, tb->spos, sizeof(double4) * (size_t)n));
+    GH_TRY(ensure(c, tb->packed, sizeof(double4) * (size_t)n));
     long long* cnt = tb->off.as<long long>() + (n + 1);
     long long* off = tb->off.as<long long>();
     DD4* pref = tb->prefix.as<DD4>();
@@ -1137,7 +1142,8 @@ static int build_tree(gh_ctx* c, const double* d_pos, const double* d_mass, int6
         // 2. keys (the first thread also clears the flags read after k_delta)
         hipLaunchKernelGGL(k_keys, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, d_pos, n,
                            tb->box.as<double>(), words, tb->keys[0].as<u64>(),
-                           tb->idx[0].as<u32>(), tb->flags.as<int>());
+                           tb->idx[0].as<u32>(), tb->flags.as<int>(), d_mass,
+                           tb->packed.as<double4>());
         GH_TRY(check_launch(c, "k_keys"));
         // 3. sort
         GH_TRY(radix_sort(c, tb->keys, tb->idx, tb->hist, n, words, 0, s, &tb->key_sel));
@@ -1145,8 +1151,7 @@ static int build_tree(gh_ctx* c, const double* d_pos, const double* d_mass, int6
         DeltaArgs da;
         da.keys = tb->keys[tb->key_sel].as<u64>();
         da.perm = tb->idx[tb->key_sel].as<u32>();
-        da.pos = d_pos;
-        da.mass = d_mass;
+        da.packed = tb->packed.as<double4>();
         da.search = dp;
         da.cnt = cnt;
         da.spos = tb->spos.as<double4>();
@@ -1242,7 +1247,7 @@ static int walk(gh_ctx* c, const double* tgt, bool self, int64_t nt, double eps,
         }
         hipLaunchKernelGGL(k_keys, dim3(blocks_for(nt, kBlock)), dim3(kBlock), 0, s, tgt, nt,
                            tb->box.as<double>(), 1, tb->tkeys[0].as<u64>(), tb->tidx[0].as<u32>(),
-                           (int*)nullptr);
+                           (int*)nullptr, (const double*)nullptr, (double4*)nullptr);
         GH_TRY(check_launch(c, "k_keys"));
         int sel = 0;
         GH_TRY(radix_sort(c, tb->tkeys, tb->tidx, tb->hist, nt, 1, 24, s, &sel));
@@ -1345,6 +1350,7 @@ void tree_free(gh_ctx* c)
     DevBuf* bufs[] = {&tb->keys[0], &tb->keys[1], &tb->idx[0],    &tb->idx[1],  &tb->hist,
                       &tb->bbox_part, &tb->box,   &tb->flags,     &tb->delta,   &tb->off,
                       &tb->prefix,  &tb->spos,      &tb->block_sums, &tb->block_counts,
+                      &tb->packed,
                       &tb->nodes64, &tb->nodes32,
                       &tb->skip,    &tb->level,   &tb->branch,    &tb->leaf,    &tb->tkeys[0],
                       &tb->tkeys[1], &tb->tidx[0], &tb->tidx[1],  &tb->walk_queue};
