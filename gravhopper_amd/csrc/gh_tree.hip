@@ -37,6 +37,7 @@
 //                     reference's recursive walk (:487-541) accepts, while the node index
 //                     stays wave-uniform and node records arrive through scalar loads.
 #include <algorithm>
+#include <cmath>
 
 #include "gh_internal.h"
 #include "gh_search.h"
@@ -52,6 +53,8 @@ constexpr int kRsItems = 8;
 constexpr int kRsChunk = kBlock * kRsItems;  // elements per radix-sort workgroup
 
 // FP64 node record of the walk and the dump
+// FP64 record; a leaf carries its size with the sign bit set, so that the FP64 walk
+// reads leaf-ness from the record itself
 struct NodeD {
     double cx, cy, cz, size;
     double mx, my, mz, mass;  // centre of mass, mass
@@ -683,7 +686,7 @@ __device__ void emit_node(const FillArgs& a, int64_t n, int64_t id, int lv, int 
     d.cx = cx;
     d.cy = cy;
     d.cz = cz;
-    d.size = size;
+    d.size = is_leaf ? -size : size;
     if (is_leaf) {
         const double4 q = a.spos[s];
         d.mass = q.w;
@@ -968,7 +971,6 @@ __device__ __forceinline__ double rsqrt_f64(double d)
 template <bool COUNT>
 __global__ __launch_bounds__(kBlock) void k_walk_f64(const NodeD* __restrict__ nodes,
                                                      const int* __restrict__ skip,
-                                                     const int* __restrict__ leaf,
                                                      int64_t n_nodes,
                                                      const double* __restrict__ tgt,
                                                      const u32* __restrict__ order, int64_t nt,
@@ -998,11 +1000,11 @@ __global__ __launch_bounds__(kBlock) void k_walk_f64(const NodeD* __restrict__ n
     const int n_nodes32 = (int)n_nodes;  // below 2^31 (checked by the build)
     int node = 0;
     while (node < n_nodes32) {
-        // the whole record (one 64-byte scalar load), its leaf flag and skip pointer,
-        // issued together
+        // the whole record (one 64-byte scalar load) and the skip pointer, issued
+        // together; a leaf's size has its sign bit set
         const f64x8 nd = *(const f64x8*)(nodes + node);
         const int sk = skip[node];
-        const bool is_leaf = leaf[node] >= 0;
+        const bool is_leaf = __double_as_longlong(nd[3]) < 0;
         const bool active = (u32)my_skip <= (u32)node;
         bool take = active;
         int next = node + 1;
@@ -1269,12 +1271,12 @@ static int walk(gh_ctx* c, const double* tgt, bool self, int64_t nt, double eps,
         if (counts)
             hipLaunchKernelGGL(k_walk_f64<true>, dim3((unsigned)(per * kXcds)), dim3(kWalkBlock),
                                0, s, tb->nodes64.as<NodeD>(), tb->skip.as<int>(),
-                               tb->leaf.as<int>(), tb->n_nodes, tgt, order, nt, eps, theta,
+                               tb->n_nodes, tgt, order, nt, eps, theta,
                                c_lo, c_hi, d_acc, counts, per, lanes);
         else
             hipLaunchKernelGGL(k_walk_f64<false>, dim3((unsigned)(per * kXcds)), dim3(kWalkBlock),
                                0, s, tb->nodes64.as<NodeD>(), tb->skip.as<int>(),
-                               tb->leaf.as<int>(), tb->n_nodes, tgt, order, nt, eps, theta,
+                               tb->n_nodes, tgt, order, nt, eps, theta,
                                c_lo, c_hi, d_acc, counts, per, lanes);
         return check_launch(c, "k_walk_f64");
     }
@@ -1402,7 +1404,7 @@ extern "C" int gh_tree_dump(gh_ctx* c, const double* pos, const double* mass, in
         r.center[0] = nd[k].cx;
         r.center[1] = nd[k].cy;
         r.center[2] = nd[k].cz;
-        r.size = nd[k].size;
+        r.size = std::fabs(nd[k].size);
         r.mass = nd[k].mass;
         r.com[0] = nd[k].mx;
         r.com[1] = nd[k].my;
