@@ -810,6 +810,7 @@ template <int WPW, bool GUARD>
 __device__ __forceinline__ void walk_unit_f32(const char* __restrict__ nodes,
                                               const int* __restrict__ skip, int n_nodes,
                                               const double* __restrict__ tgt,
+                                              const double4* __restrict__ stgt,
                                               const u32* __restrict__ order, int64_t nt,
                                               float eps2, double* __restrict__ acc,
                                               int64_t unit, int lane, int lanes)
@@ -829,9 +830,16 @@ __device__ __forceinline__ void walk_unit_f32(const char* __restrict__ nodes,
         x[w] = y[w] = z[w] = 0.f;
         if (live) {
             t[w] = order ? (int64_t)order[k] : k;
-            x[w] = (float)tgt[3 * t[w]];
-            y[w] = (float)tgt[3 * t[w] + 1];
-            z[w] = (float)tgt[3 * t[w] + 2];
+            if (stgt) {  // self walk: the particles in walk order, read contiguously
+                const double4 q = stgt[k];
+                x[w] = (float)q.x;
+                y[w] = (float)q.y;
+                z[w] = (float)q.z;
+            } else {
+                x[w] = (float)tgt[3 * t[w]];
+                y[w] = (float)tgt[3 * t[w] + 1];
+                z[w] = (float)tgt[3 * t[w] + 2];
+            }
         }
         fx[w] = fy[w] = fz[w] = 0.f;
         ax[w] = ay[w] = az[w] = 0.0;
@@ -911,6 +919,7 @@ template <int WPW, bool GUARD, bool DYN>
 __global__ __launch_bounds__(kBlock) void k_walk_f32(const NodeF* __restrict__ nodes,
                                                      const int* __restrict__ skip, int n_nodes,
                                                      const double* __restrict__ tgt,
+                                                     const double4* __restrict__ stgt,
                                                      const u32* __restrict__ order, int64_t nt,
                                                      float eps2, double* __restrict__ acc,
                                                      int64_t n_units, int64_t per_xcd,
@@ -921,8 +930,8 @@ __global__ __launch_bounds__(kBlock) void k_walk_f32(const NodeF* __restrict__ n
     if (!DYN) {
         const int64_t unit = xcd_block(per_xcd) * (kWalkBlock / 64) + wave;
         if (unit < n_units)
-            walk_unit_f32<WPW, GUARD>(nb, skip, n_nodes, tgt, order, nt, eps2, acc, unit, lane,
-                                      lanes);
+            walk_unit_f32<WPW, GUARD>(nb, skip, n_nodes, tgt, stgt, order, nt, eps2, acc, unit,
+                                      lane, lanes);
         return;
     }
     const int home = (int)(blockIdx.x % kXcds);
@@ -934,7 +943,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_f32(const NodeF* __restrict__ n
             if (lane == 0) k = atomicAdd(&queue[q], 1u);
             k = __builtin_amdgcn_readfirstlane(k);
             if (lo + (int64_t)k >= hi) break;
-            walk_unit_f32<WPW, GUARD>(nb, skip, n_nodes, tgt, order, nt, eps2, acc,
+            walk_unit_f32<WPW, GUARD>(nb, skip, n_nodes, tgt, stgt, order, nt, eps2, acc,
                                       lo + (int64_t)k, lane, lanes);
         }
     }
@@ -973,6 +982,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_f64(const NodeD* __restrict__ n
                                                      const int* __restrict__ skip,
                                                      int64_t n_nodes,
                                                      const double* __restrict__ tgt,
+                                                     const double4* __restrict__ stgt,
                                                      const u32* __restrict__ order, int64_t nt,
                                                      double eps, double theta, double c_lo,
                                                      double c_hi, double* __restrict__ acc,
@@ -989,9 +999,16 @@ __global__ __launch_bounds__(kBlock) void k_walk_f64(const NodeD* __restrict__ n
     double x = 0.0, y = 0.0, z = 0.0;
     if (live) {
         t = order ? (int64_t)order[k] : k;
-        x = tgt[3 * t];
-        y = tgt[3 * t + 1];
-        z = tgt[3 * t + 2];
+        if (stgt) {  // self walk: the particles in walk order, read contiguously
+            const double4 q = stgt[k];
+            x = q.x;
+            y = q.y;
+            z = q.z;
+        } else {
+            x = tgt[3 * t];
+            y = tgt[3 * t + 1];
+            z = tgt[3 * t + 2];
+        }
     }
     const double eps2 = eps * eps;
     double ax = 0.0, ay = 0.0, az = 0.0;
@@ -1240,8 +1257,10 @@ static int walk(gh_ctx* c, const double* tgt, bool self, int64_t nt, double eps,
     if (nt <= 0) return 0;
     TimedScope ts(c, s, GH_T_TREE_WALK);
     const u32* order = nullptr;
+    const double4* stgt = nullptr;  // self walks read the sorted particles
     if (self) {
         order = tb->idx[tb->key_sel].as<u32>();
+        stgt = tb->spos.as<double4>();
     } else if (nt >= kSortTargetsMin) {
         for (int k = 0; k < 2; k++) {
             GH_TRY(ensure(c, tb->tkeys[k], sizeof(u64) * (size_t)nt));
@@ -1271,12 +1290,12 @@ static int walk(gh_ctx* c, const double* tgt, bool self, int64_t nt, double eps,
         if (counts)
             hipLaunchKernelGGL(k_walk_f64<true>, dim3((unsigned)(per * kXcds)), dim3(kWalkBlock),
                                0, s, tb->nodes64.as<NodeD>(), tb->skip.as<int>(),
-                               tb->n_nodes, tgt, order, nt, eps, theta,
+                               tb->n_nodes, tgt, stgt, order, nt, eps, theta,
                                c_lo, c_hi, d_acc, counts, per, lanes);
         else
             hipLaunchKernelGGL(k_walk_f64<false>, dim3((unsigned)(per * kXcds)), dim3(kWalkBlock),
                                0, s, tb->nodes64.as<NodeD>(), tb->skip.as<int>(),
-                               tb->n_nodes, tgt, order, nt, eps, theta,
+                               tb->n_nodes, tgt, stgt, order, nt, eps, theta,
                                c_lo, c_hi, d_acc, counts, per, lanes);
         return check_launch(c, "k_walk_f64");
     }
@@ -1312,7 +1331,7 @@ static int walk(gh_ctx* c, const double* tgt, bool self, int64_t nt, double eps,
     }
 #define GH_WALK32(W, G, D)                                                                 \
     hipLaunchKernelGGL((k_walk_f32<W, G, D>), grid, dim3(kWalkBlock), 0, s, nd, sk, T, tgt, \
-                       order, nt, eps2, d_acc, n_units, per, q, lanes)
+                       stgt, order, nt, eps2, d_acc, n_units, per, q, lanes)
     if (wpw == 2) {
         if (dyn) {
             if (guard) GH_WALK32(2, true, true); else GH_WALK32(2, false, true);
