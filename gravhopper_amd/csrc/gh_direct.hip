@@ -5,9 +5,9 @@
 //                 :299-353 (targets), so the result is bit-identical to the reference.
 //                 Built with -ffp-contract=off; double '/' and sqrt are the compiler's
 //                 correctly rounded expansions.
-//  k_direct_f64_spread  the same sums for fewer than 32768 targets: a target's terms
-//                 are computed by several lanes, then added by one lane in the same
-//                 order, so the results are the same bits.
+//  k_direct_f64_spread  the same sums for up to 65536 targets: a target's terms
+//                 are computed by several lanes, then added in the same order, so the
+//                 results are the same bits.
 //  k_direct_f32   FP32 pair forces with v_rsq_f32, IPT targets per lane, 256-source
 //                 tiles staged in LDS as float4 (x, y, z, m) and read as wave-wide
 //                 broadcasts, FP32 sums per tile and an FP32 two-sum (hi, lo) across
@@ -118,9 +118,9 @@ __global__ __launch_bounds__(kBlock) void k_direct_f64(const double4* __restrict
 // idle (each lane's serial loop over all sources, with an IEEE division and square root
 // per pair, is then the whole run time).  TPW targets per wavefront, P = 64 / TPW lanes
 // per target: per chunk of P sources, the P lanes of a target compute one term each
-// (the same expressions as k_direct_f64), and the target's first lane then adds the P
-// terms in ascending source order.  Every target's sum is formed term by term in the
-// reference's order, so the results are bit-identical to k_direct_f64's.
+// (the same expressions as k_direct_f64), and the lanes then add the P terms in
+// ascending source order.  Every target's sum is formed term by term in the reference's
+// order, so the results are bit-identical to k_direct_f64's.
 template <bool SELF, int TPW>
 __global__ __launch_bounds__(kBlock) void k_direct_f64_spread(const double4* __restrict__ src,
                                                               int64_t np,
@@ -140,51 +140,48 @@ __global__ __launch_bounds__(kBlock) void k_direct_f64_spread(const double4* __r
         yi = tgt[3 * i + 1];
         zi = tgt[3 * i + 2];
     }
-    double f0 = 0.0, f1 = 0.0, f2 = 0.0;  // the sums, kept by lane q == 0
-    for (int64_t base = 0; base < np; base += P) {
-        const int64_t j = base + q;
-        double c0 = 0.0, c1 = 0.0, c2 = 0.0;
-        if (live && j < np) {
-            const double4 s = src[j];
-            const double dx = s.x - xi, dy = s.y - yi, dz = s.z - zi;
-            double d2 = dx * dx;
-            d2 = d2 + dy * dy;
-            d2 = d2 + dz * dz;
-            const double d = d2 + eps2;
-            double inv;
-            if (SELF) {
-                inv = 1.0 / d / sqrt(d);
-            } else {
-                inv = (d == 0.0) ? 0.0 : 1.0 / d / sqrt(d);
-            }
-            c0 = s.w * dx * inv;
-            c1 = s.w * dy * inv;
-            c2 = s.w * dz * inv;
+    // one term of source j, computed without branches (the source index is clamped, and
+    // the term is +0.0 for a source past np or for the target itself); adding +0.0 to a
+    // sum that starts at +0.0 never changes it, so these terms leave the reference's
+    // sums untouched
+    auto term = [&](int64_t j, double& c0, double& c1, double& c2) {
+        const double4 s = src[j < np ? j : np - 1];
+        const double dx = s.x - xi, dy = s.y - yi, dz = s.z - zi;
+        double d2 = dx * dx;
+        d2 = d2 + dy * dy;
+        d2 = d2 + dz * dz;
+        const double d = d2 + eps2;
+        double inv;
+        if (SELF) {
+            inv = 1.0 / d / sqrt(d);
+        } else {
+            inv = (d == 0.0) ? 0.0 : 1.0 / d / sqrt(d);
         }
+        const bool use = j < np && !(SELF && j == i);
+        c0 = use ? s.w * dx * inv : 0.0;
+        c1 = use ? s.w * dy * inv : 0.0;
+        c2 = use ? s.w * dz * inv : 0.0;
+    };
+    // Software pipelined: while a group adds the P terms of one chunk in source order,
+    // its lanes already compute the terms of the next chunk; both are in one branch-free
+    // block, so their dependent chains interleave.
+    double c0, c1, c2;
+    term(q, c0, c1, c2);
+    double f0 = 0.0, f1 = 0.0, f2 = 0.0;  // the sums (every lane of a group holds them)
+    const double* r0 = &terms[wave][0][g * P];
+    const double* r1 = &terms[wave][1][g * P];
+    const double* r2 = &terms[wave][2][g * P];
+    for (int64_t base = 0; base < np; base += P) {
         terms[wave][0][lane] = c0;
         terms[wave][1][lane] = c1;
         terms[wave][2][lane] = c2;
         __syncthreads();
-        if (live && q == 0) {
-            // the chunk's terms in order, without the target itself (at most one term):
-            // two branch-free runs, so that the LDS reads can be issued ahead
-            const int cnt = (int)((np - base) < P ? (np - base) : P);
-            const int skip = (SELF && i >= base && i < base + cnt) ? (int)(i - base) : cnt;
-            const double* r0 = &terms[wave][0][lane];
-            const double* r1 = &terms[wave][1][lane];
-            const double* r2 = &terms[wave][2][lane];
-#pragma unroll 8
-            for (int k = 0; k < skip; k++) {
-                f0 = f0 + r0[k];
-                f1 = f1 + r1[k];
-                f2 = f2 + r2[k];
-            }
-#pragma unroll 8
-            for (int k = skip + 1; k < cnt; k++) {
-                f0 = f0 + r0[k];
-                f1 = f1 + r1[k];
-                f2 = f2 + r2[k];
-            }
+        term(base + P + q, c0, c1, c2);
+#pragma unroll
+        for (int k = 0; k < P; k++) {
+            f0 = f0 + r0[k];
+            f1 = f1 + r1[k];
+            f2 = f2 + r2[k];
         }
         __syncthreads();
     }
@@ -195,21 +192,21 @@ __global__ __launch_bounds__(kBlock) void k_direct_f64_spread(const double4* __r
     }
 }
 
-// Targets per wavefront for the FP64 sums: 64 (k_direct_f64) from kF64SpreadMax
-// targets on; fewer targets are spread so that at most about kF64Waves wavefronts run,
-// with at least 2 targets per wavefront.  Measured on Plummer spheres (kernel ms for
-// 1/2/4/8/16/32/64 targets per wavefront): N=2000 0.112/0.097/0.143/0.241/0.45/0.69/
-// 0.416; N=16384 5.56/3.02/1.90/1.95/3.62/5.62/3.44; N=65536 87/45.5/25.4/18.2/19.6/
-// 24.8/13.9.
-constexpr int64_t kF64Waves = 4096;
-constexpr int64_t kF64SpreadMax = 32768;
+// Targets per wavefront for the FP64 sums: 64 (k_direct_f64) above kF64SpreadMax
+// targets; fewer targets are spread so that at most about kF64Waves wavefronts run,
+// with 2 to 16 targets per wavefront.  Measured on Plummer spheres (kernel ms for
+// 1/2/4/8/16/32/64 targets per wavefront): N=2000 0.071/0.064/0.086/0.13/0.25/0.43/
+// 0.413; N=16384 3.43/1.95/1.36/1.26/1.99/3.50/3.42; N=65536 48.3/27.8/19.0/15.7/13.6/
+// 16.5/13.8.
+constexpr int64_t kF64Waves = 2048;
+constexpr int64_t kF64SpreadMax = 65536;
 static int f64_targets_per_wave(const gh_ctx* c, int64_t nt)
 {
     const int t = c->tune[GH_TUNE_DIRECT_F64_TPW];
     if (t == 1 || t == 2 || t == 4 || t == 8 || t == 16 || t == 32 || t == 64) return t;
-    if (nt >= kF64SpreadMax) return 64;
+    if (nt > kF64SpreadMax) return 64;
     int tpw = 2;
-    while (tpw < 64 && (int64_t)tpw * kF64Waves < nt) tpw *= 2;
+    while (tpw < 16 && (int64_t)tpw * kF64Waves < nt) tpw *= 2;
     return tpw;
 }
 
