@@ -5,9 +5,9 @@
 //                 :299-353 (targets), so the result is bit-identical to the reference.
 //                 Built with -ffp-contract=off; double '/' and sqrt are the compiler's
 //                 correctly rounded expansions.
-//  k_direct_f64_spread  the same sums for up to 65536 targets: a target's terms
-//                 are computed by several lanes, then added in the same order, so the
-//                 results are the same bits.
+//  k_direct_f64_chain, k_direct_f64_spread  the same sums for up to 65536 targets:
+//                 a target's terms are computed by several lanes, then added in the
+//                 same order, so the results are the same bits.
 //  k_direct_f32   FP32 pair forces with v_rsq_f32, IPT targets per lane, 256-source
 //                 tiles staged in LDS as float4 (x, y, z, m) and read as wave-wide
 //                 broadcasts, FP32 sums per tile and an FP32 two-sum (hi, lo) across
@@ -192,18 +192,105 @@ __global__ __launch_bounds__(kBlock) void k_direct_f64_spread(const double4* __r
     }
 }
 
-// Targets per wavefront for the FP64 sums: 64 (k_direct_f64) above kF64SpreadMax
-// targets; fewer targets are spread so that at most about kF64Waves wavefronts run,
-// with 2 to 16 targets per wavefront.  Measured on Plummer spheres (kernel ms for
-// 1/2/4/8/16/32/64 targets per wavefront): N=2000 0.071/0.064/0.086/0.13/0.25/0.43/
-// 0.413; N=16384 3.43/1.95/1.36/1.26/1.99/3.50/3.42; N=65536 48.3/27.8/19.0/15.7/13.6/
-// 16.5/13.8.
+// The same sums with a lane of its own for each sum.  A one-wavefront workgroup takes
+// TPW targets, whose 3 TPW component sums are held by lanes 0 .. 3 TPW - 1.  Per chunk
+// of 64 sources, lane l computes the terms of source base + l for every target (the
+// same expressions as k_direct_f64, +0.0 for the target itself and for sources past
+// np) into LDS; lane L then adds the 64 terms of its sum in source order.  The next
+// chunk's terms are computed while the current chunk is added (two LDS buffers; a
+// single wavefront reads and writes LDS in program order, so one barrier per chunk
+// orders each buffer's writes after the reads two chunks earlier).
+template <bool SELF, int TPW>
+__global__ __launch_bounds__(64) void k_direct_f64_chain(const double4* __restrict__ src,
+                                                         int64_t np,
+                                                         const double* __restrict__ tgt,
+                                                         int64_t nt, double eps2,
+                                                         double* __restrict__ acc)
+{
+    constexpr int NS = 3 * TPW;  // sums per wavefront
+    // T[b][k][L]: terms 2k and 2k + 1 of sum L in chunk buffer b
+    __shared__ double2 T[2][32][NS];
+    const int lane = threadIdx.x;
+    const int64_t i0 = (int64_t)blockIdx.x * TPW;
+    double xt[TPW], yt[TPW], zt[TPW];
+#pragma unroll
+    for (int t = 0; t < TPW; t++) {
+        // a target past nt takes the last target's position; its sums are not stored
+        const int64_t i = i0 + t < nt ? i0 + t : nt - 1;
+        xt[t] = tgt[3 * i];
+        yt[t] = tgt[3 * i + 1];
+        zt[t] = tgt[3 * i + 2];
+    }
+    double c[TPW][3];
+    auto terms = [&](int64_t base) {
+        const int64_t j = base + lane;
+        const double4 s = src[j < np ? j : np - 1];
+#pragma unroll
+        for (int t = 0; t < TPW; t++) {
+            const double dx = s.x - xt[t], dy = s.y - yt[t], dz = s.z - zt[t];
+            double d2 = dx * dx;
+            d2 = d2 + dy * dy;
+            d2 = d2 + dz * dz;
+            const double d = d2 + eps2;
+            double inv;
+            if (SELF) {
+                inv = 1.0 / d / sqrt(d);
+            } else {
+                inv = (d == 0.0) ? 0.0 : 1.0 / d / sqrt(d);
+            }
+            const bool use = j < np && !(SELF && j == i0 + t);
+            c[t][0] = use ? s.w * dx * inv : 0.0;
+            c[t][1] = use ? s.w * dy * inv : 0.0;
+            c[t][2] = use ? s.w * dz * inv : 0.0;
+        }
+    };
+    terms(0);
+    const int L = lane < NS ? lane : NS - 1;
+    double f = 0.0;
+    int b = 0;
+    for (int64_t base = 0; base < np; base += 64) {
+        double* w = reinterpret_cast<double*>(&T[b][lane >> 1][0]) + (lane & 1);
+#pragma unroll
+        for (int t = 0; t < TPW; t++) {
+            w[2 * (3 * t)] = c[t][0];
+            w[2 * (3 * t + 1)] = c[t][1];
+            w[2 * (3 * t + 2)] = c[t][2];
+        }
+        __syncthreads();
+        terms(base + 64);
+#pragma unroll
+        for (int k = 0; k < 32; k++) {
+            const double2 v = T[b][k][L];
+            f = f + v.x;
+            f = f + v.y;
+        }
+        b ^= 1;
+    }
+    if (lane < NS && i0 + lane / 3 < nt) acc[3 * i0 + lane] = f;
+}
+
+// Kernel shape for the FP64 sums, as targets per wavefront: negative -v selects
+// k_direct_f64_chain with v targets, positive v k_direct_f64_spread with v targets
+// (64: k_direct_f64, one target per lane).  Automatic: the chain kernel up to
+// kF64ChainMax targets (1, 2 or 4 per wavefront by target count), then the spread
+// kernel up to kF64SpreadMax (2 to 16 targets, so that at most about kF64Waves
+// wavefronts run), then one target per lane.  Measured on Plummer spheres (kernel ms):
+//   targets per wavefront    chain 1/2/4/8/16          spread 1/2/4/8/16/32, plain
+//   N=2000                   0.034/0.032/0.040/0.062/0.14  0.071/0.064/0.086/0.13/0.25/0.43, 0.41
+//   N=16384                  1.31/1.02/0.99/1.52/2.23      3.43/1.95/1.36/1.26/1.99/3.50, 3.42
+//   N=65536                  18.6/14.4/13.8/17.0/26.9      48.3/27.8/19.0/15.7/13.6/16.5, 13.8
+//   N=262144                 -/-/208/254/393               plain 141
 constexpr int64_t kF64Waves = 2048;
+constexpr int64_t kF64ChainMax = 32768;
 constexpr int64_t kF64SpreadMax = 65536;
 static int f64_targets_per_wave(const gh_ctx* c, int64_t nt)
 {
     const int t = c->tune[GH_TUNE_DIRECT_F64_TPW];
     if (t == 1 || t == 2 || t == 4 || t == 8 || t == 16 || t == 32 || t == 64) return t;
+    if (t == -1 || t == -2 || t == -4 || t == -8 || t == -16) return t;
+    if (nt <= 1024) return -1;
+    if (nt <= 4096) return -2;
+    if (nt <= kF64ChainMax) return -4;
     if (nt > kF64SpreadMax) return 64;
     int tpw = 2;
     while (tpw < 16 && (int64_t)tpw * kF64Waves < nt) tpw *= 2;
@@ -738,6 +825,28 @@ int direct_dev(gh_ctx* c, const double* d_pos, const double* d_mass, int64_t np,
                 hipLaunchKernelGGL(k_direct_f64<false>, dim3(blocks_for(nt, kBlock)),
                                    dim3(kBlock), 0, s, src, np, tgt, nt, eps2, d_acc);
             return check_launch(c, "k_direct_f64");
+        }
+        if (tpw < 0) {
+            const int tc = -tpw;
+            const dim3 gc((unsigned)((nt + tc - 1) / tc));
+#define GH_CHAIN(T)                                                                       \
+    do {                                                                                  \
+        if (self)                                                                         \
+            hipLaunchKernelGGL((k_direct_f64_chain<true, T>), gc, dim3(64), 0, s,         \
+                               src, np, tgt, nt, eps2, d_acc);                            \
+        else                                                                              \
+            hipLaunchKernelGGL((k_direct_f64_chain<false, T>), gc, dim3(64), 0, s,        \
+                               src, np, tgt, nt, eps2, d_acc);                            \
+    } while (0)
+            switch (tc) {
+            case 1: GH_CHAIN(1); break;
+            case 2: GH_CHAIN(2); break;
+            case 4: GH_CHAIN(4); break;
+            case 8: GH_CHAIN(8); break;
+            default: GH_CHAIN(16); break;
+            }
+#undef GH_CHAIN
+            return check_launch(c, "k_direct_f64_chain");
         }
         const dim3 grid(blocks_for((nt + tpw - 1) / tpw, kBlock / 64));
 #define GH_SPREAD(T)                                                                      \

@@ -217,7 +217,10 @@ int gh_timing_read(gh_ctx *ctx, int kernel_class, double *total_ms, int64_t *lau
  *                        per-XCD atomic queues)
  *   GH_TUNE_DIRECT_F64_TPW  FP64 direct sum: 0 automatic, 1/2/4/8/16/32/64 targets per
  *                        wavefront (64: one target per lane; fewer: the terms of a
- *                        target are computed by 64/value lanes and then added in order)
+ *                        target are computed by 64/value lanes and then added in order),
+ *                        -1/-2/-4/-8/-16: that many targets per wavefront, every lane
+ *                        computing one source's terms for all of them and each of the
+ *                        3 x targets sums added in order by a lane of its own
  *   GH_TUNE_ALLOC_LIMIT_MB  cap, in MiB, on the device memory the context's arenas may
  *                        hold (0: no cap); a call that would exceed it fails with
  *                        GH_ENOMEM and leaves the context usable

@@ -150,9 +150,10 @@ def test_fp32_self_kernels_agree_unequal_masses(ctx):
         assert np.median(rel_err(out[ipt], out[-1])) < 1e-6
 
 
-@pytest.mark.parametrize("tpw", [1, 2, 4, 8, 16, 32, 64])
+@pytest.mark.parametrize("tpw", [1, 2, 4, 8, 16, 32, 64, -1, -2, -4, -8, -16])
 def test_fp64_direct_targets_per_wave_bitwise(ctx, oracle, tpw):
-    # every spread of a target's terms over lanes adds them in the reference's order
+    # every spread of a target's terms over lanes (positive: the spread kernel, negative:
+    # a lane per component sum) adds them in the reference's order
     rng = np.random.default_rng(300 + tpw)
     n = 777
     pos = rng.standard_cauchy(size=(n, 3))
@@ -165,5 +166,11 @@ def test_fp64_direct_targets_per_wave_bitwise(ctx, oracle, tpw):
                            "self eps=%g" % eps)
             assert_bitwise(ctx.direct(pos, mass, eps, FP64, force_pos=fpos),
                            oracle.direct_position(pos, mass, fpos, eps), "position eps=%g" % eps)
+            # fewer sources than one chunk, and fewer targets than one wavefront's
+            assert_bitwise(ctx.direct(pos[:5], mass[:5], eps, FP64),
+                           oracle.direct(pos[:5], mass[:5], eps), "5 particles eps=%g" % eps)
+            assert_bitwise(ctx.direct(pos[:70], mass[:70], eps, FP64, force_pos=fpos[:3]),
+                           oracle.direct_position(pos[:70], mass[:70], fpos[:3], eps),
+                           "3 targets eps=%g" % eps)
     finally:
         ctx.set_tuning(_native.TUNE_DIRECT_F64_TPW, 0)
