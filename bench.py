@@ -493,7 +493,7 @@ def main():
         "roofline": {"bound": "valu", "achieved": achieved_tflops, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved_tflops / PEAK_FP32_TFLOPS,
                      "traffic": traffic_from_profiles("direct_f32_n262144_traffic.json"),
-                     "kernel": "k_direct_f32", "kernel_avg_ms": k_avg_s * 1e3,
+                     "kernel": "k_direct_sym_f32<4,false>", "kernel_avg_ms": k_avg_s * 1e3,
                      "flop_per_pair": FLOP_PER_PAIR},
     }
     extras = {"update_kernel_avg_ms": u_ms / max(u_n, 1)}
