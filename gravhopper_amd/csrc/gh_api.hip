@@ -137,10 +137,12 @@ static int check_common(gh_ctx* c, int64_t np, int64_t nf, double eps, int preci
 }
 
 // One step: force at xh, then the fused kick/drift/half-drift.  slot != nullptr: the
-// history slot is read on the device (graph replay); otherwise hp/hv point at it.
+// history slot is read on the device (graph replay), and *slot_next becomes *slot + 1;
+// otherwise hp/hv point at the step's snapshot.
 static int one_step(gh_ctx* c, double* x, double* v, const double* m, int64_t np,
                     double dt, double eps, int algorithm, double theta, int precision,
-                    double* hp, double* hv, const long long* slot, hipStream_t s)
+                    double* hp, double* hv, const long long* slot, hipStream_t s,
+                    long long* slot_next = nullptr)
 {
     const int64_t n3 = 3 * np;
     double* xh = c->xh.as<double>();
@@ -155,13 +157,16 @@ static int one_step(gh_ctx* c, double* x, double* v, const double* m, int64_t np
         GH_TRY(zero_dev(c, acc, n3, s));
     }
     TimedScope ts(c, s, GH_T_UPDATE);
-    return kick_drift(c, acc, np, dt, xh, v, x, hp, hv, s, slot);
+    return kick_drift(c, acc, np, dt, xh, v, x, hp, hv, s, slot, nullptr, slot_next);
 }
 
 // nsteps DKD steps on device state; d_hist_* (nullable) receive (nsteps, n, 3).
-// Direct-sum runs replay one captured step as a HIP graph after a first eager step
-// (which sizes every buffer); the history slot then comes from a device counter.  The
-// tree's build reads its node count back to the host, so tree runs stay eager.
+// Direct-sum runs replay two captured steps as one HIP graph after a first eager step
+// (which sizes every buffer); an odd last step runs eagerly.  In the graph the history
+// slot comes from two device counters that the steps' update kernels advance in turn
+// (step A reads slot[0] and writes slot[1] = slot[0] + 1, step B the reverse), so a step
+// is two launches.  The tree's build reads its node count back to the host, so tree
+// runs stay eager.
 static int run_steps(gh_ctx* c, double* x, double* v, const double* m, int64_t np, double dt,
                      double eps, int algorithm, double theta, int precision, int64_t nsteps,
                      double* hist_p, double* hist_v, hipStream_t s)
@@ -187,8 +192,10 @@ static int run_steps(gh_ctx* c, double* x, double* v, const double* m, int64_t n
     GH_TRY(set_slot(c, slot, 1, 0, s));
     GH_HIP(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     int rc = one_step(c, x, v, m, np, dt, eps, algorithm, theta, precision, hist_p, hist_v,
-                      slot, s);
-    if (!rc) rc = set_slot(c, slot, 1, 1, s);
+                      slot, s, slot + 1);
+    if (!rc)
+        rc = one_step(c, x, v, m, np, dt, eps, algorithm, theta, precision, hist_p, hist_v,
+                      slot + 1, s, slot);
     hipGraph_t g = nullptr;
     const hipError_t ee = hipStreamEndCapture(s, &g);
     if (rc) {
@@ -202,8 +209,13 @@ static int run_steps(gh_ctx* c, double* x, double* v, const double* m, int64_t n
     (void)hipGraphDestroy(g);
     if (ie != hipSuccess)
         return fail(c, GH_EHIP, "step graph instantiation failed: %s", hipGetErrorString(ie));
-    for (int64_t st = 1; st < nsteps && !rc; st++)
+    int64_t st = 1;
+    for (; st + 1 < nsteps && !rc; st += 2)
         if (hipGraphLaunch(ex, s) != hipSuccess) rc = fail(c, GH_EHIP, "graph launch failed");
+    if (!rc && st < nsteps)
+        rc = one_step(c, x, v, m, np, dt, eps, algorithm, theta, precision,
+                      hist_p ? hist_p + st * n3 : nullptr, hist_v ? hist_v + st * n3 : nullptr,
+                      nullptr, s);
     // the executable graph may be destroyed once its launches are enqueued
     (void)hipStreamSynchronize(s);
     (void)hipGraphExecDestroy(ex);

@@ -201,7 +201,8 @@ __global__ __launch_bounds__(kBlock) void k_direct_f64_spread(const double4* __r
 // single wavefront reads and writes LDS in program order, so one barrier per chunk
 // orders each buffer's writes after the reads two chunks earlier).
 template <bool SELF, int TPW>
-__global__ __launch_bounds__(64) void k_direct_f64_chain(const double4* __restrict__ src,
+__global__ __launch_bounds__(64) void k_direct_f64_chain(const double* __restrict__ pos,
+                                                         const double* __restrict__ mass,
                                                          int64_t np,
                                                          const double* __restrict__ tgt,
                                                          int64_t nt, double eps2,
@@ -222,12 +223,16 @@ __global__ __launch_bounds__(64) void k_direct_f64_chain(const double4* __restri
         zt[t] = tgt[3 * i + 2];
     }
     double c[TPW][3];
+    // sources are read in place from the (n,3) positions and the masses: the chunks
+    // are small, and a packing pass would cost a launch per force evaluation
     auto terms = [&](int64_t base) {
         const int64_t j = base + lane;
-        const double4 s = src[j < np ? j : np - 1];
+        const int64_t jc = j < np ? j : np - 1;
+        const double sx = pos[3 * jc], sy = pos[3 * jc + 1], sz = pos[3 * jc + 2];
+        const double sw = mass[jc];
 #pragma unroll
         for (int t = 0; t < TPW; t++) {
-            const double dx = s.x - xt[t], dy = s.y - yt[t], dz = s.z - zt[t];
+            const double dx = sx - xt[t], dy = sy - yt[t], dz = sz - zt[t];
             double d2 = dx * dx;
             d2 = d2 + dy * dy;
             d2 = d2 + dz * dz;
@@ -239,9 +244,9 @@ __global__ __launch_bounds__(64) void k_direct_f64_chain(const double4* __restri
                 inv = (d == 0.0) ? 0.0 : 1.0 / d / sqrt(d);
             }
             const bool use = j < np && !(SELF && j == i0 + t);
-            c[t][0] = use ? s.w * dx * inv : 0.0;
-            c[t][1] = use ? s.w * dy * inv : 0.0;
-            c[t][2] = use ? s.w * dz * inv : 0.0;
+            c[t][0] = use ? sw * dx * inv : 0.0;
+            c[t][1] = use ? sw * dy * inv : 0.0;
+            c[t][2] = use ? sw * dz * inv : 0.0;
         }
     };
     terms(0);
@@ -809,34 +814,20 @@ int direct_dev(gh_ctx* c, const double* d_pos, const double* d_mass, int64_t np,
         return zero_dev(c, d_acc, 3 * nt, s);
     }
     if (precision == GH_FP64) {
-        GH_TRY(ensure(c, c->src64, sizeof(double4) * (size_t)np));
-        double4* src = c->src64.as<double4>();
-        hipLaunchKernelGGL(k_pack_src64, dim3(blocks_for(np, kBlock)), dim3(kBlock), 0, s,
-                           d_pos, d_mass, np, src);
-        GH_TRY(check_launch(c, "k_pack_src64"));
         const double eps2 = eps * eps;
-        TimedScope ts(c, s, GH_T_DIRECT_F64);
         const int tpw = f64_targets_per_wave(c, nt);
-        if (tpw == 64) {
-            if (self)
-                hipLaunchKernelGGL(k_direct_f64<true>, dim3(blocks_for(nt, kBlock)),
-                                   dim3(kBlock), 0, s, src, np, tgt, nt, eps2, d_acc);
-            else
-                hipLaunchKernelGGL(k_direct_f64<false>, dim3(blocks_for(nt, kBlock)),
-                                   dim3(kBlock), 0, s, src, np, tgt, nt, eps2, d_acc);
-            return check_launch(c, "k_direct_f64");
-        }
         if (tpw < 0) {
+            TimedScope ts(c, s, GH_T_DIRECT_F64);
             const int tc = -tpw;
             const dim3 gc((unsigned)((nt + tc - 1) / tc));
 #define GH_CHAIN(T)                                                                       \
     do {                                                                                  \
         if (self)                                                                         \
             hipLaunchKernelGGL((k_direct_f64_chain<true, T>), gc, dim3(64), 0, s,         \
-                               src, np, tgt, nt, eps2, d_acc);                            \
+                               d_pos, d_mass, np, tgt, nt, eps2, d_acc);                  \
         else                                                                              \
             hipLaunchKernelGGL((k_direct_f64_chain<false, T>), gc, dim3(64), 0, s,        \
-                               src, np, tgt, nt, eps2, d_acc);                            \
+                               d_pos, d_mass, np, tgt, nt, eps2, d_acc);                  \
     } while (0)
             switch (tc) {
             case 1: GH_CHAIN(1); break;
@@ -847,6 +838,21 @@ int direct_dev(gh_ctx* c, const double* d_pos, const double* d_mass, int64_t np,
             }
 #undef GH_CHAIN
             return check_launch(c, "k_direct_f64_chain");
+        }
+        GH_TRY(ensure(c, c->src64, sizeof(double4) * (size_t)np));
+        double4* src = c->src64.as<double4>();
+        hipLaunchKernelGGL(k_pack_src64, dim3(blocks_for(np, kBlock)), dim3(kBlock), 0, s,
+                           d_pos, d_mass, np, src);
+        GH_TRY(check_launch(c, "k_pack_src64"));
+        TimedScope ts(c, s, GH_T_DIRECT_F64);
+        if (tpw == 64) {
+            if (self)
+                hipLaunchKernelGGL(k_direct_f64<true>, dim3(blocks_for(nt, kBlock)),
+                                   dim3(kBlock), 0, s, src, np, tgt, nt, eps2, d_acc);
+            else
+                hipLaunchKernelGGL(k_direct_f64<false>, dim3(blocks_for(nt, kBlock)),
+                                   dim3(kBlock), 0, s, src, np, tgt, nt, eps2, d_acc);
+            return check_launch(c, "k_direct_f64");
         }
         const dim3 grid(blocks_for((nt + tpw - 1) / tpw, kBlock / 64));
 #define GH_SPREAD(T)                                                                      \

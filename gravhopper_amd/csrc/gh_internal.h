@@ -144,12 +144,14 @@ int direct_f32_packed(gh_ctx* c, const float4* src, int64_t np_pad, const float4
 
 // leapfrog update (gh_step.hip).  kick_drift: ext (nullable) is an extra acceleration
 // in km/s/Myr added to the N-body one, as a = acc*A + ext; slot (nullable) is the
-// history slot counter of graph-replayed runs.
+// history slot counter of graph-replayed runs, and slot_next (nullable) receives
+// *slot + 1.
 int half_drift(gh_ctx* c, const double* x, const double* v, int64_t n, double dt, double* xh,
                hipStream_t s);
 int kick_drift(gh_ctx* c, const double* acc, int64_t n, double dt, double* xh, double* v,
                double* x, double* hp, double* hv, hipStream_t s,
-               const long long* slot = nullptr, const double* ext = nullptr);
+               const long long* slot = nullptr, const double* ext = nullptr,
+               long long* slot_next = nullptr);
 
 // tree (gh_tree.hip)
 int tree_dev(gh_ctx* c, const double* d_pos, const double* d_mass, int64_t np,

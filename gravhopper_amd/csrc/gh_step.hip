@@ -23,6 +23,8 @@ __global__ __launch_bounds__(kBlock) void k_half_drift(const double* __restrict_
 
 // hist_p / hist_v (nullable): snapshot slot of this step, or, when slot != nullptr,
 // the base of the history, the slot index being read from *slot (graph replay).
+// slot_next (nullable, another word than slot): receives *slot + 1 for the next step,
+// so a replayed step needs no separate counter update.
 // ext (nullable): external acceleration in km/s/Myr, added as the reference adds the
 // summed external forces to the N-body acceleration (gravhopper.py:419-459); without
 // external forces the sum is with 0.0, as in the reference.
@@ -34,10 +36,12 @@ __global__ __launch_bounds__(kBlock) void k_kick_drift(const double* __restrict_
                                                        double* __restrict__ hist_p,
                                                        double* __restrict__ hist_v,
                                                        const long long* __restrict__ slot,
-                                                       const double* __restrict__ ext)
+                                                       const double* __restrict__ ext,
+                                                       long long* __restrict__ slot_next)
 {
     const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (k >= n3) return;
+    if (slot_next && k == 0) *slot_next = *slot + 1;
     if (slot) {
         const int64_t off = (int64_t)(*slot) * n3;
         if (hist_p) hist_p += off;
@@ -80,11 +84,12 @@ int half_drift(gh_ctx* c, const double* x, const double* v, int64_t n, double dt
 
 int kick_drift(gh_ctx* c, const double* acc, int64_t n, double dt, double* xh, double* v,
                double* x, double* hp, double* hv, hipStream_t s, const long long* slot,
-               const double* ext)
+               const double* ext, long long* slot_next)
 {
     if (n <= 0) return 0;
+    if (slot_next && !slot) return fail(c, GH_EARG, "slot_next needs slot");
     hipLaunchKernelGGL(k_kick_drift, dim3(blocks_for(3 * n, kBlock)), dim3(kBlock), 0, s, acc,
-                       3 * n, dt, xh, v, x, hp, hv, slot, ext);
+                       3 * n, dt, xh, v, x, hp, hv, slot, ext, slot_next);
     return check_launch(c, "k_kick_drift");
 }
 

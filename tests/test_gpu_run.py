@@ -148,19 +148,22 @@ def test_simulation_fp32_close_to_fp64(ctx):
     assert np.median(diff / disp) < 1e-5
 
 
+@pytest.mark.parametrize("nsteps", [7, 8])
 @pytest.mark.parametrize("prec", [_native.GH_FP64, _native.GH_FP32])
-def test_run_graph_replay_equals_eager(ctx, prec):
-    """Direct-sum runs replay a captured step as a HIP graph; every replayed step's
-    snapshot must equal the eagerly launched run, over several calls with host work and
+def test_run_graph_replay_equals_eager(ctx, prec, nsteps):
+    """Direct-sum runs replay two captured steps as a HIP graph (8 steps: an eager first
+    step, three replays and an eager last step); every replayed step's snapshot must
+    equal the eagerly launched run, over several calls with host work and
     synchronisation in between."""
     ic = workloads.plummer(20000 if prec == _native.GH_FP32 else 1500, seed=35)
     x, v = ic["pos"], ic["vel"]
     xe, ve = x, v
     for call in range(3):
         ctx.set_tuning(_native.TUNE_RUN_GRAPH, 0)
-        x, v, hp, hv = ctx.run(x, v, ic["mass"], 0.002, 1e-5, 7, precision=prec, history=True)
+        x, v, hp, hv = ctx.run(x, v, ic["mass"], 0.002, 1e-5, nsteps, precision=prec,
+                               history=True)
         ctx.set_tuning(_native.TUNE_RUN_GRAPH, -1)
-        xe, ve, hpe, hve = ctx.run(xe, ve, ic["mass"], 0.002, 1e-5, 7, precision=prec,
+        xe, ve, hpe, hve = ctx.run(xe, ve, ic["mass"], 0.002, 1e-5, nsteps, precision=prec,
                                    history=True)
         ctx.set_tuning(_native.TUNE_RUN_GRAPH, 0)
         assert_bitwise(hp, hpe, "call %d history pos" % call)
