@@ -1,6 +1,8 @@
 // Exported C ABI (include/gravhop.h): contexts, host-pointer wrappers, the
 // device-resident leapfrog loop and diagnostics.
+#include <algorithm>
 #include <cstring>
+#include <utility>
 
 #include "gh_internal.h"
 
@@ -167,10 +169,15 @@ static int one_step(gh_ctx* c, double* x, double* v, const double* m, int64_t np
 // (step A reads slot[0] and writes slot[1] = slot[0] + 1, step B the reverse), so a step
 // is two launches.  The tree's build reads its node count back to the host, so tree
 // runs stay eager.
+// out_p / out_v (nullable, host memory): when given, a replayed run copies its history
+// there itself, slice by slice on the copy stream while later steps still run, and
+// sets *copied; otherwise the caller copies the device history afterwards.
 static int run_steps(gh_ctx* c, double* x, double* v, const double* m, int64_t np, double dt,
                      double eps, int algorithm, double theta, int precision, int64_t nsteps,
-                     double* hist_p, double* hist_v, hipStream_t s)
+                     double* hist_p, double* hist_v, hipStream_t s, double* out_p = nullptr,
+                     double* out_v = nullptr, bool* copied = nullptr)
 {
+    if (copied) *copied = false;
     if (np == 0 || nsteps == 0) return 0;
     const int64_t n3 = 3 * np;
     GH_TRY(ensure(c, c->xh, sizeof(double) * (size_t)n3));
@@ -209,16 +216,62 @@ static int run_steps(gh_ctx* c, double* x, double* v, const double* m, int64_t n
     (void)hipGraphDestroy(g);
     if (ie != hipSuccess)
         return fail(c, GH_EHIP, "step graph instantiation failed: %s", hipGetErrorString(ie));
+    // history slices [a, b) of steps, each complete on s once its event has fired
+    const bool out = (out_p && hist_p) || (out_v && hist_v);
+    const int64_t slice = std::max<int64_t>(8, (nsteps + 15) / 16);
+    std::vector<std::pair<int64_t, int64_t>> slices;
+    std::vector<hipEvent_t> events;
+    int64_t marked = 0;
+    auto mark = [&](int64_t done) {
+        if (!out || rc || (done - marked < slice && done < nsteps)) return;
+        hipEvent_t e = nullptr;
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+            rc = fail(c, GH_EHIP, "history event creation failed");
+            return;
+        }
+        events.push_back(e);
+        if (hipEventRecord(e, s) != hipSuccess) {
+            rc = fail(c, GH_EHIP, "history event record failed");
+            return;
+        }
+        slices.emplace_back(marked, done);
+        marked = done;
+    };
+    mark(1);
     int64_t st = 1;
-    for (; st + 1 < nsteps && !rc; st += 2)
+    for (; st + 1 < nsteps && !rc; st += 2) {
         if (hipGraphLaunch(ex, s) != hipSuccess) rc = fail(c, GH_EHIP, "graph launch failed");
-    if (!rc && st < nsteps)
+        mark(st + 2);
+    }
+    if (!rc && st < nsteps) {
         rc = one_step(c, x, v, m, np, dt, eps, algorithm, theta, precision,
                       hist_p ? hist_p + st * n3 : nullptr, hist_v ? hist_v + st * n3 : nullptr,
                       nullptr, s);
+        mark(nsteps);
+    }
+    // every step is enqueued: copy the slices out as they complete (a copy to pageable
+    // memory holds the host, not the steps still queued on s)
+    hipStream_t cs = c->copy_stream;
+    for (size_t k = 0; k < slices.size() && !rc; k++) {
+        const int64_t a = slices[k].first, b = slices[k].second;
+        const size_t bytes = sizeof(double) * (size_t)(n3 * (b - a));
+        if (hipStreamWaitEvent(cs, events[k], 0) != hipSuccess)
+            rc = fail(c, GH_EHIP, "history copy wait failed");
+        if (!rc && out_p && hist_p &&
+            hipMemcpyAsync(out_p + a * n3, hist_p + a * n3, bytes, hipMemcpyDeviceToHost,
+                           cs) != hipSuccess)
+            rc = fail(c, GH_EHIP, "history copy failed");
+        if (!rc && out_v && hist_v &&
+            hipMemcpyAsync(out_v + a * n3, hist_v + a * n3, bytes, hipMemcpyDeviceToHost,
+                           cs) != hipSuccess)
+            rc = fail(c, GH_EHIP, "history copy failed");
+    }
+    if (out) (void)hipStreamSynchronize(cs);
     // the executable graph may be destroyed once its launches are enqueued
     (void)hipStreamSynchronize(s);
     (void)hipGraphExecDestroy(ex);
+    for (hipEvent_t e : events) (void)hipEventDestroy(e);
+    if (!rc && out && copied) *copied = marked == nsteps;
     return rc;
 }
 
@@ -470,11 +523,17 @@ int gh_run(gh_ctx* c, double* pos, double* vel, const double* mass, int64_t np, 
         if (hist_pos) GH_TRY(ensure(c, c->hist_p, per * (size_t)chunk));
         if (hist_vel) GH_TRY(ensure(c, c->hist_v, per * (size_t)chunk));
     }
+    if (hist && !c->copy_stream)
+        GH_HIP(c, hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
     for (int64_t s0 = 0; s0 < nsteps; s0 += chunk) {
         const int64_t s1 = (s0 + chunk < nsteps) ? s0 + chunk : nsteps;
+        bool copied = false;
         GH_TRY(run_steps(c, x, v, m, np, dt, eps, algorithm, theta, precision, s1 - s0,
                          hist_pos ? c->hist_p.as<double>() : nullptr,
-                         hist_vel ? c->hist_v.as<double>() : nullptr, s));
+                         hist_vel ? c->hist_v.as<double>() : nullptr, s,
+                         hist_pos ? hist_pos + s0 * n3 : nullptr,
+                         hist_vel ? hist_vel + s0 * n3 : nullptr, &copied));
+        if (copied) continue;
         if (hist_pos)
             GH_HIP(c, hipMemcpyAsync(hist_pos + s0 * n3, c->hist_p.p,
                                      sizeof(double) * (size_t)(n3 * (s1 - s0)),

@@ -170,3 +170,21 @@ def test_run_graph_replay_equals_eager(ctx, prec, nsteps):
         assert_bitwise(hv, hve, "call %d history vel" % call)
         assert_bitwise(x, xe)
         assert_bitwise(v, ve)
+
+
+def test_run_history_slices_equal_eager(ctx):
+    """A replayed run copies its history out in slices while later steps run; 101 steps
+    give 13 slices and an odd last step.  Every snapshot must equal the eager run's."""
+    ic = workloads.plummer(3000, seed=36)
+    try:
+        ctx.set_tuning(_native.TUNE_RUN_GRAPH, -1)
+        xe, ve, hpe, hve = ctx.run(ic["pos"], ic["vel"], ic["mass"], 0.003, 2e-5, 101,
+                                   history=True)
+    finally:
+        ctx.set_tuning(_native.TUNE_RUN_GRAPH, 0)
+    x, v, hp, hv = ctx.run(ic["pos"], ic["vel"], ic["mass"], 0.003, 2e-5, 101, history=True)
+    assert_bitwise(hp, hpe, "history pos")
+    assert_bitwise(hv, hve, "history vel")
+    assert_bitwise(x, xe, "pos")
+    assert_bitwise(v, ve, "vel")
+    assert_bitwise(hp[-1], x, "last snapshot")
