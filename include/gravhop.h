@@ -210,8 +210,8 @@ int gh_timing_read(gh_ctx *ctx, int kernel_class, double *total_ms, int64_t *lau
  *   GH_TUNE_DIRECT_IPT   FP32 one-sided kernel: 0 automatic, 1/2/4 targets per lane
  *   GH_TUNE_WALKS_PER_WAVE  FP32 tree walk: 0 automatic, 1 or 2 independent 64-target
  *                        walks per wavefront
- *   GH_TUNE_RUN_GRAPH    direct-sum gh_run / gh_run_dev: 0 replay a captured step as a
- *                        HIP graph (from 4 steps on), -1 launch every kernel eagerly
+ *   GH_TUNE_RUN_GRAPH    direct-sum gh_run / gh_run_dev: 0 replay two captured steps as
+ *                        one HIP graph (from 4 steps on), -1 launch every kernel eagerly
  *   GH_TUNE_WALK_SCHEDULE  FP32 tree walk: 0 automatic (static), -1 static (one unit of
  *                        targets per wavefront), 1 dynamic (wavefronts take units from
  *                        per-XCD atomic queues)
