@@ -407,6 +407,8 @@ def main():
                     help="run only the N=16384 adjoint benchmark (for profiling)")
     ap.add_argument("--only-ensemble", action="store_true",
                     help="run only the 128-sphere ensemble benchmark (for profiling)")
+    ap.add_argument("--only-config1", action="store_true",
+                    help="run only the N=2000 FP64 run with history (for profiling)")
     ap.add_argument("--tree-lite", action="store_true",
                     help="tree steps without the accuracy and comparison runs")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -441,6 +443,9 @@ def main():
     if args.only_ensemble:
         print(json.dumps({"ensemble": bench_ensemble(ctx, torch, args.ensemble_steps)}),
               flush=True)
+        return
+    if args.only_config1:
+        print(json.dumps({"config1": [bench_config1(ctx) for _ in range(3)]}), flush=True)
         return
     n = args.n
     ic = workloads.plummer(n, b=1e-3, mass=1e6, seed=SEED_DIRECT)
