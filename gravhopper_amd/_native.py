@@ -118,6 +118,21 @@ def _ptr(a):
     return None if a is None else ctypes.c_void_p(a.ctypes.data)
 
 
+def _history_arrays(nsteps, n, history, hist_out):
+    """(hist_pos, hist_vel) for a run: new arrays, the caller's (checked), or None."""
+    if hist_out is None:
+        if not history:
+            return None, None
+        return np.empty((nsteps, n, 3)), np.empty((nsteps, n, 3))
+    hp, hv = hist_out
+    for a in (hp, hv):
+        if not (isinstance(a, np.ndarray) and a.dtype == np.float64
+                and a.shape == (nsteps, n, 3) and a.flags.c_contiguous and a.flags.writeable):
+            raise ValueError("hist_out arrays must be writable C-contiguous float64 arrays "
+                             "of shape (%d, %d, 3)" % (nsteps, n))
+    return hp, hv
+
+
 class Context:
     """One device context: a HIP stream plus the library's device arenas."""
 
@@ -212,32 +227,34 @@ class Context:
 
     # -- device-resident leapfrog ----------------------------------------------
     def run(self, pos, vel, mass, dt, eps, nsteps, algorithm=GH_DIRECT, theta=0.7,
-            precision=GH_FP64, history=False):
+            precision=GH_FP64, history=False, hist_out=None):
         """Advances copies of pos/vel by nsteps DKD steps; returns (pos, vel) or
-        (pos, vel, hist_pos, hist_vel) with (nsteps, n, 3) snapshots."""
+        (pos, vel, hist_pos, hist_vel) with (nsteps, n, 3) snapshots.  hist_out: a pair
+        of arrays the snapshots are written into in place (implies history)."""
         pos = np.array(pos, dtype=np.float64, order="C", copy=True)
         vel = np.array(vel, dtype=np.float64, order="C", copy=True)
         mass = np.ascontiguousarray(mass, dtype=np.float64)
         n = len(mass)
-        hp = np.empty((nsteps, n, 3)) if history else None
-        hv = np.empty((nsteps, n, 3)) if history else None
+        history = history or hist_out is not None
+        hp, hv = _history_arrays(nsteps, n, history, hist_out)
         self.check(self.lib.gh_run(self.handle, _ptr(pos), _ptr(vel), _ptr(mass), n,
                                    float(dt), float(eps), int(algorithm), float(theta),
                                    int(precision), int(nsteps), _ptr(hp), _ptr(hv)))
         return (pos, vel, hp, hv) if history else (pos, vel)
 
     def run_ext(self, pos, vel, mass, dt, eps, nsteps, external, t0=0.0, pass_velocity=False,
-                algorithm=GH_DIRECT, theta=0.7, precision=GH_FP64, history=False):
+                algorithm=GH_DIRECT, theta=0.7, precision=GH_FP64, history=False,
+                hist_out=None):
         """gh_run with external accelerations: external(x_half, v, t) -> (n, 3) array in
         km/s/Myr, called once per step with copies of the half-step positions and (when
         pass_velocity) the pre-kick velocities; v is None otherwise.  An exception raised
-        by `external` stops the run and is re-raised here."""
+        by `external` stops the run and is re-raised here.  hist_out as for run."""
         pos = np.array(pos, dtype=np.float64, order="C", copy=True)
         vel = np.array(vel, dtype=np.float64, order="C", copy=True)
         mass = np.ascontiguousarray(mass, dtype=np.float64)
         n = len(mass)
-        hp = np.empty((nsteps, n, 3)) if history else None
-        hv = np.empty((nsteps, n, 3)) if history else None
+        history = history or hist_out is not None
+        hp, hv = _history_arrays(nsteps, n, history, hist_out)
         failure = []
 
         def trampoline(user, n_, x_ptr, v_ptr, t, acc_ptr):

@@ -205,11 +205,11 @@ class Simulation(object):
         t0 = self.timestep
         algo = _native.GH_DIRECT if self.params['algorithm'] == 'direct' else _native.GH_TREE
         prec = _native.PRECISIONS[self.params['precision']]
-        _, _, hp, hv = _native.default_context().run(
+        # the snapshots go straight into the history arrays (rows t0+1 .. t0+N)
+        _native.default_context().run(
             self._pos[t0], self._vel[t0], self._mass, self._dt(), self._eps(), N,
-            algorithm=algo, theta=self.theta, precision=prec, history=True)
-        self._pos[t0 + 1:t0 + N + 1] = hp
-        self._vel[t0 + 1:t0 + N + 1] = hv
+            algorithm=algo, theta=self.theta, precision=prec,
+            hist_out=(self._pos[t0 + 1:t0 + N + 1], self._vel[t0 + 1:t0 + N + 1]))
         dt = self._dt()
         for k in range(t0 + 1, t0 + N + 1):
             self._times[k] = self._times[k - 1] + dt
@@ -226,12 +226,11 @@ class Simulation(object):
         def external(xh, v, t):
             return self._extra_raw(xh, xh, t, v)
 
-        _, _, hp, hv = _native.default_context().run_ext(
+        _native.default_context().run_ext(
             self._pos[t0], self._vel[t0], self._mass, self._dt(), self._eps(), N, external,
             t0=self._times[t0], pass_velocity=pass_v, algorithm=algo, theta=self.theta,
-            precision=prec, history=True)
-        self._pos[t0 + 1:t0 + N + 1] = hp
-        self._vel[t0 + 1:t0 + N + 1] = hv
+            precision=prec,
+            hist_out=(self._pos[t0 + 1:t0 + N + 1], self._vel[t0 + 1:t0 + N + 1]))
         dt = self._dt()
         for k in range(t0 + 1, t0 + N + 1):
             self._times[k] = self._times[k - 1] + dt

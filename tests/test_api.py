@@ -115,3 +115,20 @@ def test_workload_generators_shapes_and_centring():
     assert np.all(np.isfinite(t['vel']))
     g = W.galaxy(n_halo=4096, n_disk=4096, seed=5)
     assert g['pos'].shape == (8192, 3) and np.all(np.isfinite(g['vel']))
+
+
+def test_run_history_output_arrays_are_checked():
+    # snapshots can be written in place into caller arrays, e.g. rows of Simulation's
+    # history; anything the library could not write through a plain pointer is refused
+    hist = np.zeros((10, 5, 3))
+    hp, hv = _native._history_arrays(4, 5, False, (hist[1:5], hist[5:9]))
+    assert hp.base is hist and hv.base is hist
+    assert _native._history_arrays(4, 5, False, None) == (None, None)
+    a, b = _native._history_arrays(4, 5, True, None)
+    assert a.shape == b.shape == (4, 5, 3)
+    bad = [np.zeros((4, 5, 3), dtype=np.float32), np.zeros((3, 5, 3)),
+           np.zeros((4, 5, 6))[:, :, :3], np.zeros((4, 5, 3)).view()]
+    bad[3].flags.writeable = False
+    for arr in bad:
+        with pytest.raises(ValueError, match="hist_out"):
+            _native._history_arrays(4, 5, True, (arr, np.zeros((4, 5, 3))))
