@@ -162,6 +162,72 @@ static int one_step(gh_ctx* c, double* x, double* v, const double* m, int64_t np
     return kick_drift(c, acc, np, dt, xh, v, x, hp, hv, s, slot, nullptr, slot_next);
 }
 
+// Copies a run's device history to host memory in slices of steps, on the copy stream,
+// each once an event recorded after its last step has fired.  A copy to pageable host
+// memory holds the calling thread until it is done, so callers issue a slice's copy
+// only after later steps are queued (or after the steps have finished).
+struct HistCopier {
+    gh_ctx* c;
+    hipStream_t s, cs;
+    int64_t n3, nsteps, slice;
+    const double *dp, *dv;  // device history (nullable)
+    double *hp, *hv;        // host destinations (nullable)
+    bool on;
+    int64_t marked = 0;  // steps [0, marked) belong to recorded slices
+    size_t issued = 0;   // slices whose copies have been issued
+    std::vector<std::pair<int64_t, int64_t>> slices;
+    std::vector<hipEvent_t> events;
+
+    HistCopier(gh_ctx* c_, hipStream_t s_, int64_t n3_, int64_t nsteps_, int64_t slice_,
+               const double* dp_, const double* dv_, double* hp_, double* hv_)
+        : c(c_), s(s_), cs(c_->copy_stream), n3(n3_), nsteps(nsteps_), slice(slice_), dp(dp_),
+          dv(dv_), hp(dp_ ? hp_ : nullptr), hv(dv_ ? hv_ : nullptr),
+          on((hp || hv) && c_->copy_stream != nullptr)
+    {
+    }
+    ~HistCopier()
+    {
+        if (on) (void)hipStreamSynchronize(cs);
+        for (hipEvent_t e : events) (void)hipEventDestroy(e);
+    }
+    // steps [marked, done) are queued on s: record them as a slice once it is long
+    // enough or the run is complete
+    int mark(int64_t done)
+    {
+        if (!on || (done - marked < slice && done < nsteps)) return 0;
+        hipEvent_t e = nullptr;
+        GH_HIP(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        events.push_back(e);
+        GH_HIP(c, hipEventRecord(e, s));
+        slices.emplace_back(marked, done);
+        marked = done;
+        return 0;
+    }
+    // issue the copies of the recorded slices [issued, upto)
+    int issue(size_t upto)
+    {
+        for (; issued < upto && issued < slices.size(); issued++) {
+            const int64_t a = slices[issued].first, b = slices[issued].second;
+            const size_t bytes = sizeof(double) * (size_t)(n3 * (b - a));
+            GH_HIP(c, hipStreamWaitEvent(cs, events[issued], 0));
+            if (hp)
+                GH_HIP(c, hipMemcpyAsync(hp + a * n3, dp + a * n3, bytes, hipMemcpyDeviceToHost,
+                                         cs));
+            if (hv)
+                GH_HIP(c, hipMemcpyAsync(hv + a * n3, dv + a * n3, bytes, hipMemcpyDeviceToHost,
+                                         cs));
+        }
+        return 0;
+    }
+    int finish()
+    {
+        GH_TRY(issue(slices.size()));
+        GH_HIP(c, hipStreamSynchronize(cs));
+        return 0;
+    }
+    bool complete() const { return on && marked == nsteps && issued == slices.size(); }
+};
+
 // nsteps DKD steps on device state; d_hist_* (nullable) receive (nsteps, n, 3).
 // Direct-sum runs replay two captured steps as one HIP graph after a first eager step
 // (which sizes every buffer); an odd last step runs eagerly.  In the graph the history
@@ -169,9 +235,11 @@ static int one_step(gh_ctx* c, double* x, double* v, const double* m, int64_t np
 // (step A reads slot[0] and writes slot[1] = slot[0] + 1, step B the reverse), so a step
 // is two launches.  The tree's build reads its node count back to the host, so tree
 // runs stay eager.
-// out_p / out_v (nullable, host memory): when given, a replayed run copies its history
-// there itself, slice by slice on the copy stream while later steps still run, and
-// sets *copied; otherwise the caller copies the device history afterwards.
+// out_p / out_v (nullable, host memory; needs the context's copy stream): the run copies
+// its history there itself while later steps still run, and sets *copied; otherwise the
+// caller copies the device history afterwards.  Replayed runs queue every step first and
+// then copy slices of at least 8 steps (at most about 16); eager runs copy slices of
+// about 16 MiB, each after the next step has been queued.
 static int run_steps(gh_ctx* c, double* x, double* v, const double* m, int64_t np, double dt,
                      double eps, int algorithm, double theta, int precision, int64_t nsteps,
                      double* hist_p, double* hist_v, hipStream_t s, double* out_p = nullptr,
@@ -188,10 +256,20 @@ static int run_steps(gh_ctx* c, double* x, double* v, const double* m, int64_t n
     const bool graph = algorithm == GH_DIRECT && !c->timing && nsteps >= 4 &&
                        c->tune[GH_TUNE_RUN_GRAPH] >= 0;
     if (!graph) {
-        for (int64_t st = 1; st < nsteps; st++)
+        const int64_t step_bytes = (int64_t)sizeof(double) * n3 * 2;
+        HistCopier hc(c, s, n3, nsteps, std::max<int64_t>(1, ((int64_t)16 << 20) / step_bytes),
+                      hist_p, hist_v, out_p, out_v);
+        GH_TRY(hc.mark(1));
+        for (int64_t st = 1; st < nsteps; st++) {
             GH_TRY(one_step(c, x, v, m, np, dt, eps, algorithm, theta, precision,
                             hist_p ? hist_p + st * n3 : nullptr,
                             hist_v ? hist_v + st * n3 : nullptr, nullptr, s));
+            // step st is queued: copy every slice recorded before it
+            GH_TRY(hc.issue(hc.slices.size()));
+            GH_TRY(hc.mark(st + 1));
+        }
+        GH_TRY(hc.finish());
+        if (copied) *copied = hc.complete();
         return 0;
     }
     GH_TRY(ensure(c, c->slot, 256));
@@ -216,62 +294,28 @@ static int run_steps(gh_ctx* c, double* x, double* v, const double* m, int64_t n
     (void)hipGraphDestroy(g);
     if (ie != hipSuccess)
         return fail(c, GH_EHIP, "step graph instantiation failed: %s", hipGetErrorString(ie));
-    // history slices [a, b) of steps, each complete on s once its event has fired
-    const bool out = (out_p && hist_p) || (out_v && hist_v);
-    const int64_t slice = std::max<int64_t>(8, (nsteps + 15) / 16);
-    std::vector<std::pair<int64_t, int64_t>> slices;
-    std::vector<hipEvent_t> events;
-    int64_t marked = 0;
-    auto mark = [&](int64_t done) {
-        if (!out || rc || (done - marked < slice && done < nsteps)) return;
-        hipEvent_t e = nullptr;
-        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
-            rc = fail(c, GH_EHIP, "history event creation failed");
-            return;
+    {
+        HistCopier hc(c, s, n3, nsteps, std::max<int64_t>(8, (nsteps + 15) / 16), hist_p,
+                      hist_v, out_p, out_v);
+        if (!rc) rc = hc.mark(1);
+        int64_t st = 1;
+        for (; st + 1 < nsteps && !rc; st += 2) {
+            if (hipGraphLaunch(ex, s) != hipSuccess) rc = fail(c, GH_EHIP, "graph launch failed");
+            if (!rc) rc = hc.mark(st + 2);
         }
-        events.push_back(e);
-        if (hipEventRecord(e, s) != hipSuccess) {
-            rc = fail(c, GH_EHIP, "history event record failed");
-            return;
+        if (!rc && st < nsteps) {
+            rc = one_step(c, x, v, m, np, dt, eps, algorithm, theta, precision,
+                          hist_p ? hist_p + st * n3 : nullptr,
+                          hist_v ? hist_v + st * n3 : nullptr, nullptr, s);
+            if (!rc) rc = hc.mark(nsteps);
         }
-        slices.emplace_back(marked, done);
-        marked = done;
-    };
-    mark(1);
-    int64_t st = 1;
-    for (; st + 1 < nsteps && !rc; st += 2) {
-        if (hipGraphLaunch(ex, s) != hipSuccess) rc = fail(c, GH_EHIP, "graph launch failed");
-        mark(st + 2);
+        // every step is queued: copy the slices out as they complete
+        if (!rc) rc = hc.finish();
+        if (!rc && copied) *copied = hc.complete();
     }
-    if (!rc && st < nsteps) {
-        rc = one_step(c, x, v, m, np, dt, eps, algorithm, theta, precision,
-                      hist_p ? hist_p + st * n3 : nullptr, hist_v ? hist_v + st * n3 : nullptr,
-                      nullptr, s);
-        mark(nsteps);
-    }
-    // every step is enqueued: copy the slices out as they complete (a copy to pageable
-    // memory holds the host, not the steps still queued on s)
-    hipStream_t cs = c->copy_stream;
-    for (size_t k = 0; k < slices.size() && !rc; k++) {
-        const int64_t a = slices[k].first, b = slices[k].second;
-        const size_t bytes = sizeof(double) * (size_t)(n3 * (b - a));
-        if (hipStreamWaitEvent(cs, events[k], 0) != hipSuccess)
-            rc = fail(c, GH_EHIP, "history copy wait failed");
-        if (!rc && out_p && hist_p &&
-            hipMemcpyAsync(out_p + a * n3, hist_p + a * n3, bytes, hipMemcpyDeviceToHost,
-                           cs) != hipSuccess)
-            rc = fail(c, GH_EHIP, "history copy failed");
-        if (!rc && out_v && hist_v &&
-            hipMemcpyAsync(out_v + a * n3, hist_v + a * n3, bytes, hipMemcpyDeviceToHost,
-                           cs) != hipSuccess)
-            rc = fail(c, GH_EHIP, "history copy failed");
-    }
-    if (out) (void)hipStreamSynchronize(cs);
     // the executable graph may be destroyed once its launches are enqueued
     (void)hipStreamSynchronize(s);
     (void)hipGraphExecDestroy(ex);
-    for (hipEvent_t e : events) (void)hipEventDestroy(e);
-    if (!rc && out && copied) *copied = marked == nsteps;
     return rc;
 }
 

@@ -188,3 +188,21 @@ def test_run_history_slices_equal_eager(ctx):
     assert_bitwise(x, xe, "pos")
     assert_bitwise(v, ve, "vel")
     assert_bitwise(hp[-1], x, "last snapshot")
+
+
+def test_tree_run_history_slices_equal_chained_steps(ctx):
+    """Eager (tree) runs copy their history in slices of about 16 MiB, each once the
+    next step is queued; at N=200000 a slice is one step.  The snapshots must equal
+    those of single-step runs chained from each other."""
+    g = workloads.galaxy(n_halo=100000, n_disk=100000, seed=37)
+    args = (g["mass"], 0.1, 0.05)
+    x, v, hp, hv = ctx.run(g["pos"], g["vel"], *args, 5, algorithm=_native.GH_TREE,
+                           precision=_native.GH_FP32, history=True)
+    xc, vc = g["pos"], g["vel"]
+    for k in range(5):
+        xc, vc = ctx.run(xc, vc, *args, 1, algorithm=_native.GH_TREE,
+                         precision=_native.GH_FP32)
+        assert_bitwise(hp[k], xc, "step %d pos" % k)
+        assert_bitwise(hv[k], vc, "step %d vel" % k)
+    assert_bitwise(x, xc, "final pos")
+    assert_bitwise(v, vc, "final vel")
