@@ -139,7 +139,7 @@ def cpu_tree_baseline(n_targets=131072):
                       "(%.1f s measured)" % (n, n_targets, n, t2 - t0)}
 
 
-def bench_config1(ctx):
+def bench_plummer2000_run(ctx):
     """The reference's CPU-runnable case: Plummer N=2000 (b=1 pc, 1e6 Msun), eps=0.05 pc,
     dt=0.005 Myr, 400 FP64 DKD steps with the direct sum, end to end through gh_run with
     the full history, as Simulation.run does it."""
@@ -155,7 +155,7 @@ def bench_config1(ctx):
             "seconds": el, "ms_per_step": el / 400 * 1e3}
 
 
-def cpu_config1_baseline(n_forces=40):
+def cpu_plummer2000_baseline(n_forces=40):
     """The reference's C direct sum (built from its source) for the N=2000 case, on one
     core: n_forces force evaluations, scaled to the run's 400 (the leapfrog itself is a
     few numpy passes per step, negligible at this N)."""
@@ -407,7 +407,7 @@ def main():
                     help="run only the N=16384 adjoint benchmark (for profiling)")
     ap.add_argument("--only-ensemble", action="store_true",
                     help="run only the 128-sphere ensemble benchmark (for profiling)")
-    ap.add_argument("--only-config1", action="store_true",
+    ap.add_argument("--only-plummer2000", action="store_true",
                     help="run only the N=2000 FP64 run with history (for profiling)")
     ap.add_argument("--tree-lite", action="store_true",
                     help="tree steps without the accuracy and comparison runs")
@@ -444,8 +444,9 @@ def main():
         print(json.dumps({"ensemble": bench_ensemble(ctx, torch, args.ensemble_steps)}),
               flush=True)
         return
-    if args.only_config1:
-        print(json.dumps({"config1": [bench_config1(ctx) for _ in range(3)]}), flush=True)
+    if args.only_plummer2000:
+        print(json.dumps({"plummer2000_fp64_run": [bench_plummer2000_run(ctx)
+                                                   for _ in range(3)]}), flush=True)
         return
     n = args.n
     ic = workloads.plummer(n, b=1e-3, mass=1e6, seed=SEED_DIRECT)
@@ -551,7 +552,7 @@ def main():
         del x64, v64
     if not args.no_extras:
         log("bench: N=2000 FP64 run of 400 steps")
-        extras["config1"] = bench_config1(ctx)
+        extras["plummer2000_fp64_run"] = bench_plummer2000_run(ctx)
     if not args.no_extras and not args.no_tree:
         log("bench: tree step at N=1M")
         extras["tree"] = bench_tree(ctx, torch, args.tree_steps, 1)
@@ -569,12 +570,12 @@ def main():
         result["cpu_baseline"]["host"] = host
         if not args.no_extras:
             log("bench: cpu baseline, reference direct sum at N=2000")
-            c1 = cpu_config1_baseline()
+            c1 = cpu_plummer2000_baseline()
             if c1 is not None:
-                result["cpu_baseline"]["config1"] = c1
-                if "config1" in extras:
-                    extras["config1"]["speedup_over_cpu_reference"] = (
-                        c1["value"] / extras["config1"]["seconds"])
+                result["cpu_baseline"]["plummer2000_fp64_run"] = c1
+                if "plummer2000_fp64_run" in extras:
+                    extras["plummer2000_fp64_run"]["speedup_over_cpu_reference"] = (
+                        c1["value"] / extras["plummer2000_fp64_run"]["seconds"])
         if not args.no_extras and not args.no_tree:
             log("bench: cpu baseline, reference tree at N=1M")
             tb = cpu_tree_baseline()

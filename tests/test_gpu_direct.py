@@ -40,7 +40,7 @@ def test_fp64_direct_bitwise_oracle(ctx, oracle, n):
         assert_bitwise(got, oracle.direct_position(pos, mass, fpos, eps), "pos n=%d" % n)
 
 
-def test_fp64_direct_plummer_config1(ctx, oracle):
+def test_fp64_direct_plummer_n2000(ctx, oracle):
     ic = workloads.plummer(2000, b=1e-3, mass=1e6, seed=20260001)
     got = ctx.direct(ic["pos"], ic["mass"], 5e-5, FP64)
     assert_bitwise(got, oracle.direct(ic["pos"], ic["mass"], 5e-5), "plummer 2000")

@@ -25,7 +25,7 @@ def test_run_fp64_bitwise_vs_oracle_short(ctx, oracle):
     assert_bitwise(hv, hvo, "hist vel")
 
 
-def test_run_config1_400_steps_bitwise(ctx, oracle):
+def test_run_plummer_n2000_400_steps_bitwise(ctx, oracle):
     # README example: Plummer N=2000, b=1 pc, 1e6 Msun, eps=0.05 pc, dt=0.005 Myr, 400 steps
     ic = workloads.plummer(2000, b=1e-3, mass=1e6, seed=20260001)
     x, v = ctx.run(ic["pos"], ic["vel"], ic["mass"], 0.005, 5e-5, 400)
