@@ -196,10 +196,10 @@ __global__ __launch_bounds__(kBlock) void k_direct_f64_spread(const double4* __r
 // TPW targets, whose 3 TPW component sums are held by lanes 0 .. 3 TPW - 1.  Per chunk
 // of 64 sources, lane l computes the terms of source base + l for every target (the
 // same expressions as k_direct_f64, +0.0 for the target itself and for sources past
-// np) into LDS; lane L then adds the 64 terms of its sum in source order.  The next
-// chunk's terms are computed while the current chunk is added (two LDS buffers; a
-// single wavefront reads and writes LDS in program order, so one barrier per chunk
-// orders each buffer's writes after the reads two chunks earlier).
+// np) into LDS; lane L then adds the 64 terms of its sum in source order.  Each pass
+// stores one chunk's terms, computes the next chunk's and adds the stored ones (two
+// LDS buffers; a single wavefront reads and writes LDS in program order, so one
+// barrier per chunk orders each buffer's writes after the reads two chunks earlier).
 template <bool SELF, int TPW>
 __global__ __launch_bounds__(64) void k_direct_f64_chain(const double* __restrict__ pos,
                                                          const double* __restrict__ mass,
