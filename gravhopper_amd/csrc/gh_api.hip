@@ -328,8 +328,10 @@ static int run_steps(gh_ctx* c, double* x, double* v, const double* m, int64_t n
 static int run_ext_steps(gh_ctx* c, double* x, double* v, const double* m, int64_t np,
                          double dt, double eps, int algorithm, double theta, int precision,
                          int64_t nsteps, double* t, gh_external_fn fn, void* user, int pass_v,
-                         double* hist_p, double* hist_v, hipStream_t s)
+                         double* hist_p, double* hist_v, hipStream_t s, double* out_p = nullptr,
+                         double* out_v = nullptr, bool* copied = nullptr)
 {
+    if (copied) *copied = false;
     const int64_t n3 = 3 * np;
     double* xh = c->xh.as<double>();
     double* acc = c->acc64.as<double>();
@@ -340,6 +342,11 @@ static int run_ext_steps(gh_ctx* c, double* x, double* v, const double* m, int64
     hipStream_t cs = c->copy_stream;
     hipEvent_t ready = c->ext_events[0], host = c->ext_events[1], back = c->ext_events[2];
     const size_t bytes = sizeof(double) * (size_t)n3;
+    // history to host memory (out_p / out_v) in slices of about 16 MiB, each copied on the
+    // copy stream after the next step's force is queued, so that it overlaps that force
+    HistCopier hc(c, s, n3, nsteps,
+                  std::max<int64_t>(1, ((int64_t)16 << 20) / (int64_t)(2 * bytes)), hist_p,
+                  hist_v, out_p, out_v);
     GH_TRY(half_drift(c, x, v, np, dt, xh, s));
     for (int64_t st = 0; st < nsteps; st++) {
         GH_HIP(c, hipEventRecord(ready, s));
@@ -355,6 +362,7 @@ static int run_ext_steps(gh_ctx* c, double* x, double* v, const double* m, int64
         } else {
             GH_TRY(zero_dev(c, acc, n3, s));
         }
+        GH_TRY(hc.issue(hc.slices.size()));
         GH_HIP(c, hipEventSynchronize(host));
         const int rc = fn(user, np, h_xh, pass_v ? h_v : nullptr, *t + 0.5 * dt, h_ext);
         if (rc != 0)
@@ -369,7 +377,10 @@ static int run_ext_steps(gh_ctx* c, double* x, double* v, const double* m, int64
                               hist_v ? hist_v + st * n3 : nullptr, s, nullptr, ext));
         }
         *t = *t + dt;
+        GH_TRY(hc.mark(st + 1));
     }
+    GH_TRY(hc.finish());
+    if (copied) *copied = hc.complete();
     return 0;
 }
 
@@ -643,16 +654,20 @@ int gh_run_ext(gh_ctx* c, double* pos, double* vel, const double* mass, int64_t 
     double t = t0;
     for (int64_t s0 = 0; s0 < nsteps; s0 += chunk) {
         const int64_t s1 = (s0 + chunk < nsteps) ? s0 + chunk : nsteps;
+        bool copied = false;
         const int rc = run_ext_steps(c, x, v, m, np, dt, eps, algorithm, theta, precision,
                                      s1 - s0, &t, fn, user, pass_velocity,
                                      hist_pos ? c->hist_p.as<double>() : nullptr,
-                                     hist_vel ? c->hist_v.as<double>() : nullptr, s);
+                                     hist_vel ? c->hist_v.as<double>() : nullptr, s,
+                                     hist_pos ? hist_pos + s0 * n3 : nullptr,
+                                     hist_vel ? hist_vel + s0 * n3 : nullptr, &copied);
         if (rc) {
             // leave both streams idle before returning the error
             (void)hipStreamSynchronize(c->copy_stream);
             (void)hipStreamSynchronize(s);
             return rc;
         }
+        if (copied) continue;
         if (hist_pos)
             GH_HIP(c, hipMemcpyAsync(hist_pos + s0 * n3, c->hist_p.p, bytes * (size_t)(s1 - s0),
                                      hipMemcpyDeviceToHost, s));

@@ -206,3 +206,27 @@ def test_tree_run_history_slices_equal_chained_steps(ctx):
         assert_bitwise(hv[k], vc, "step %d vel" % k)
     assert_bitwise(x, xc, "final pos")
     assert_bitwise(v, vc, "final vel")
+
+
+def test_external_run_history_slices_equal_chained_steps(ctx):
+    """Runs with external forces copy their history in slices of about 16 MiB while the
+    next step's force runs; at N=200000 a slice is one step.  The snapshots must equal
+    those of single-step runs chained from each other."""
+    g = workloads.galaxy(n_halo=100000, n_disk=100000, seed=38)
+    dt = 0.1
+
+    def external(xh, v, t):
+        # a harmonic pull towards a centre that moves with time
+        return -1e-3 * (xh - np.array([0.01 * t, 0.0, 0.0]))
+
+    kw = dict(algorithm=_native.GH_TREE, precision=_native.GH_FP32)
+    x, v, hp, hv = ctx.run_ext(g["pos"], g["vel"], g["mass"], dt, 0.05, 4, external, t0=2.0,
+                               history=True, **kw)
+    xc, vc, t = g["pos"], g["vel"], 2.0
+    for k in range(4):
+        xc, vc = ctx.run_ext(xc, vc, g["mass"], dt, 0.05, 1, external, t0=t, **kw)
+        t = t + dt  # the time advances as in the run: t + dt per step
+        assert_bitwise(hp[k], xc, "step %d pos" % k)
+        assert_bitwise(hv[k], vc, "step %d vel" % k)
+    assert_bitwise(x, xc, "final pos")
+    assert_bitwise(v, vc, "final vel")
