@@ -174,3 +174,17 @@ def test_fp64_direct_targets_per_wave_bitwise(ctx, oracle, tpw):
                            "3 targets eps=%g" % eps)
     finally:
         ctx.set_tuning(_native.TUNE_DIRECT_F64_TPW, 0)
+
+
+@pytest.mark.parametrize("n", [1024, 1025, 4096, 4097])
+def test_fp64_direct_automatic_shapes_bitwise(ctx, oracle, n):
+    # both sides of the automatic shape steps (1, 2 and 4 targets per wavefront of the
+    # per-sum-lane kernel) give the reference's bits
+    rng = np.random.default_rng(400 + n)
+    pos = rng.standard_cauchy(size=(n, 3))
+    mass = rng.uniform(0.5, 2.0, size=n)
+    assert_bitwise(ctx.direct(pos, mass, 1e-3, FP64), oracle.direct(pos, mass, 1e-3),
+                   "self n=%d" % n)
+    fpos = rng.normal(size=(n, 3))
+    assert_bitwise(ctx.direct(pos, mass, 1e-3, FP64, force_pos=fpos),
+                   oracle.direct_position(pos, mass, fpos, 1e-3), "position n=%d" % n)
