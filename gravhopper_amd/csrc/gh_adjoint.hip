@@ -35,9 +35,23 @@ __device__ __forceinline__ float rotn(float v)
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), kRotNext, 0xf, 0xf, false));
 }
 
-// j-side record in LDS: (x, y, z, m) and (mu_x, mu_y, mu_z, 0)
-template <int IPT, bool GUARD>
+// The particles are in the sorted order of gh_frames.hip and the workgroup works in
+// block I's frame, as the force kernel does: src holds local FP32 coordinates and
+// masses, block J's coordinates are shifted by fl(o_J - o_I).  The DS instance takes the
+// items with a flagged block, with both sides as hi + lo words of FP64 differences
+// from o_I (src64), and separations (hi_j - hi_k) + (lo_j - lo_k).
+// j-side records in LDS: (x, y, z, m), (mu_x, mu_y, mu_z, 0) and, DS, (lo x, y, z, 0).
+__device__ __forceinline__ void split_pair(double d, float& hi, float& lo)
+{
+    hi = (float)d;
+    lo = (float)(d - (double)hi);
+}
+
+template <int IPT, bool GUARD, bool DS>
 __global__ __launch_bounds__(kBlock) void k_vjp_sym_f32(const float4* __restrict__ src,
+                                                        const double4* __restrict__ src64,
+                                                        const double4* __restrict__ frame,
+                                                        const int* __restrict__ flag,
                                                         const float4* __restrict__ mu,
                                                         int64_t np_pad,
                                                         const int4* __restrict__ work,
@@ -47,33 +61,67 @@ __global__ __launch_bounds__(kBlock) void k_vjp_sym_f32(const float4* __restrict
     constexpr int NCH = BLK / 64;
     __shared__ float4 tp[NCH * 128];
     __shared__ float4 tm[NCH * 128];
+    __shared__ float4 tl[DS ? NCH * 128 : 1];
     __shared__ float jp[kBlock / 64][8][64];
     const int4 it = work[blockIdx.x];
     const int I = it.x, J = it.y;
+    if ((flag[I] | flag[J]) != (DS ? 1 : 0)) return;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t ibase = (int64_t)I * BLK + wave * 64 * IPT + lane;
     const int64_t jbase = (int64_t)J * BLK;
     eps2 = in_vgpr(eps2);
+    const double4 oI = frame[I];
 
     float xk[IPT], yk[IPT], zk[IPT], mk[IPT], ux[IPT], uy[IPT], uz[IPT];
+    float xl[IPT], yl[IPT], zl[IPT];
     float Ax[IPT], Ay[IPT], Az[IPT], B[IPT], Cx[IPT], Cy[IPT], Cz[IPT], L[IPT];
 #pragma unroll
     for (int k = 0; k < IPT; k++) {
-        const float4 t = src[ibase + k * 64];
+        if (DS) {
+            const double4 t = src64[ibase + k * 64];
+            split_pair(t.x - oI.x, xk[k], xl[k]);
+            split_pair(t.y - oI.y, yk[k], yl[k]);
+            split_pair(t.z - oI.z, zk[k], zl[k]);
+            mk[k] = (float)t.w;
+        } else {
+            const float4 t = src[ibase + k * 64];
+            xk[k] = t.x;
+            yk[k] = t.y;
+            zk[k] = t.z;
+            mk[k] = t.w;
+            xl[k] = yl[k] = zl[k] = 0.f;
+        }
         const float4 q = mu[ibase + k * 64];
-        xk[k] = t.x;
-        yk[k] = t.y;
-        zk[k] = t.z;
-        mk[k] = t.w;
         ux[k] = q.x;
         uy[k] = q.y;
         uz[k] = q.z;
         Ax[k] = Ay[k] = Az[k] = B[k] = Cx[k] = Cy[k] = Cz[k] = L[k] = 0.f;
     }
-    for (int q = threadIdx.x; q < NCH * 128; q += kBlock) {
-        const int64_t g = jbase + (q >> 7) * 64 + (q & 63);
-        tp[q] = src[g];
-        tm[q] = mu[g];
+    {
+        const double4 oJ = frame[J];
+        const float offx = (float)(oJ.x - oI.x), offy = (float)(oJ.y - oI.y),
+                    offz = (float)(oJ.z - oI.z);
+        for (int q = threadIdx.x; q < NCH * 128; q += kBlock) {
+            const int64_t g = jbase + (q >> 7) * 64 + (q & 63);
+            if (DS) {
+                const double4 v = src64[g];
+                float4 h, l;
+                split_pair(v.x - oI.x, h.x, l.x);
+                split_pair(v.y - oI.y, h.y, l.y);
+                split_pair(v.z - oI.z, h.z, l.z);
+                h.w = (float)v.w;
+                l.w = 0.f;
+                tp[q] = h;
+                tl[q] = l;
+            } else {
+                float4 v = src[g];
+                v.x += offx;
+                v.y += offy;
+                v.z += offz;
+                tp[q] = v;
+            }
+            tm[q] = mu[g];
+        }
     }
     __syncthreads();
 
@@ -84,11 +132,18 @@ __global__ __launch_bounds__(kBlock) void k_vjp_sym_f32(const float4* __restrict
             for (int q = 0; q < 64; q++) {
                 const float4 s = tp[c * 128 + q];
                 const float4 m4 = tm[c * 128 + q];
+                float4 sl = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (DS) sl = tl[c * 128 + q];
                 const int jl = c * 64 + q;
 #pragma unroll
                 for (int k = 0; k < IPT; k++) {
                     const int kl = wave * 64 * IPT + k * 64 + lane;
-                    const float dx = s.x - xk[k], dy = s.y - yk[k], dz = s.z - zk[k];
+                    float dx = s.x - xk[k], dy = s.y - yk[k], dz = s.z - zk[k];
+                    if (DS) {
+                        dx = dx + (sl.x - xl[k]);
+                        dy = dy + (sl.y - yl[k]);
+                        dz = dz + (sl.z - zl[k]);
+                    }
                     float r2 = __builtin_fmaf(dx, dx, eps2);
                     r2 = __builtin_fmaf(dy, dy, r2);
                     r2 = __builtin_fmaf(dz, dz, r2);
@@ -116,15 +171,23 @@ __global__ __launch_bounds__(kBlock) void k_vjp_sym_f32(const float4* __restrict
         for (int c = 0; c < NCH; c++) {
             const float4* rp = &tp[c * 128 + lane];
             const float4* rm = &tm[c * 128 + lane];
+            const float4* rl = &tl[DS ? c * 128 + lane : 0];
             float a = 0.f, bx = 0.f, by = 0.f, bz = 0.f, cx = 0.f, cy = 0.f, cz = 0.f, l = 0.f;
             for (int s0 = 0; s0 < 64; s0 += 4) {
 #pragma unroll
                 for (int st = 0; st < 4; st++) {
                     const float4 s = rp[s0 + st];
                     const float4 m4 = rm[s0 + st];
+                    float4 sl = make_float4(0.f, 0.f, 0.f, 0.f);
+                    if (DS) sl = rl[s0 + st];
 #pragma unroll
                     for (int k = 0; k < IPT; k++) {
-                        const float dx = s.x - xk[k], dy = s.y - yk[k], dz = s.z - zk[k];
+                        float dx = s.x - xk[k], dy = s.y - yk[k], dz = s.z - zk[k];
+                        if (DS) {
+                            dx = dx + (sl.x - xl[k]);
+                            dy = dy + (sl.y - yl[k]);
+                            dz = dz + (sl.z - zl[k]);
+                        }
                         float r2 = __builtin_fmaf(dx, dx, eps2);
                         r2 = __builtin_fmaf(dy, dy, r2);
                         r2 = __builtin_fmaf(dz, dz, r2);
@@ -207,45 +270,51 @@ __global__ __launch_bounds__(kBlock) void k_vjp_sym_f32(const float4* __restrict
     }
 }
 
-// mv = lv + c lx; mu = dt A mv (FP32 copy for the pair kernel); keeps mv
+// mv = lv + c lx; mu = dt A mv (FP32 copy for the pair kernel, in sorted order: mu[p]
+// belongs to particle perm[p]); keeps mv
 __global__ __launch_bounds__(kBlock) void k_adj_pre(const double* __restrict__ lx,
                                                     const double* __restrict__ lv, int64_t n,
-                                                    int64_t n_pad, double cdrift, double dtA,
+                                                    int64_t n_pad, const u32* __restrict__ perm,
+                                                    double cdrift, double dtA,
                                                     double* __restrict__ mv,
                                                     float4* __restrict__ mu)
 {
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n_pad) return;
-    if (i >= n) {
-        mu[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (p >= n_pad) return;
+    if (p >= n) {
+        mu[p] = make_float4(0.f, 0.f, 0.f, 0.f);
         return;
     }
+    const int64_t i = perm[p];
     double t[3];
     for (int k = 0; k < 3; k++) {
         t[k] = lv[3 * i + k] + cdrift * lx[3 * i + k];
         mv[3 * i + k] = t[k];
     }
-    mu[i] = make_float4((float)(dtA * t[0]), (float)(dtA * t[1]), (float)(dtA * t[2]), 0.f);
+    mu[p] = make_float4((float)(dtA * t[0]), (float)(dtA * t[1]), (float)(dtA * t[2]), 0.f);
 }
 
-// lxh = lx + g; lx = lxh; lv = mv + c lxh; lm += gm   (g, gm summed over slots in FP64)
+// lxh = lx + g; lx = lxh; lv = mv + c lxh; lm += gm   (g, gm summed over slots in FP64;
+// slot entries are in sorted order, sorted position p being particle perm[p])
 __global__ __launch_bounds__(kBlock) void k_adj_post(const float4* __restrict__ part, int nb,
                                                      int64_t n_pad, int64_t n,
+                                                     const u32* __restrict__ perm,
                                                      const double* __restrict__ mv,
                                                      double cdrift, double* __restrict__ lx,
                                                      double* __restrict__ lv,
                                                      double* __restrict__ lm)
 {
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
+    const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (p >= n) return;
     double g0 = 0.0, g1 = 0.0, g2 = 0.0, gm = 0.0;
     for (int b = 0; b < nb; b++) {
-        const float4 p = part[(int64_t)b * n_pad + i];
-        g0 += (double)p.x;
-        g1 += (double)p.y;
-        g2 += (double)p.z;
-        gm += (double)p.w;
+        const float4 q = part[(int64_t)b * n_pad + p];
+        g0 += (double)q.x;
+        g1 += (double)q.y;
+        g2 += (double)q.z;
+        gm += (double)q.w;
     }
+    const int64_t i = perm[p];
     const double g[3] = {g0, g1, g2};
     for (int k = 0; k < 3; k++) {
         const double lxh = lx[3 * i + k] + g[k];
@@ -253,17 +322,6 @@ __global__ __launch_bounds__(kBlock) void k_adj_post(const float4* __restrict__ 
         lv[3 * i + k] = mv[3 * i + k] + cdrift * lxh;
     }
     lm[i] += gm;
-}
-
-__global__ __launch_bounds__(kBlock) void k_pack_xm(const double* __restrict__ xh,
-                                                    const double* __restrict__ m, int64_t n,
-                                                    int64_t n_pad, float4* __restrict__ out)
-{
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n_pad) return;
-    out[i] = i < n ? make_float4((float)xh[3 * i], (float)xh[3 * i + 1], (float)xh[3 * i + 2],
-                                 (float)m[i])
-                   : make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
 __global__ __launch_bounds__(kBlock) void k_copy_d(const double* __restrict__ a,
@@ -287,17 +345,29 @@ static int vjp_ipt(int64_t n)
     return 1;
 }
 
+template <int IPT, bool GUARD>
+static void launch_vjp_g(unsigned items, hipStream_t s, const OrderBufs& o, const float4* mu,
+                         int64_t np_pad, const int4* work, float eps2, float4* part)
+{
+    const float4* src = o.loc.as<float4>();
+    const double4* s64 = o.s64.as<double4>();
+    const double4* fr = o.frame.as<double4>();
+    const int* fl = o.flag.as<int>();
+    hipLaunchKernelGGL((k_vjp_sym_f32<IPT, GUARD, false>), dim3(items), dim3(kBlock), 0, s, src,
+                       s64, fr, fl, mu, np_pad, work, eps2, part);
+    hipLaunchKernelGGL((k_vjp_sym_f32<IPT, GUARD, true>), dim3(items), dim3(kBlock), 0, s, src,
+                       s64, fr, fl, mu, np_pad, work, eps2, part);
+}
+
 template <int IPT>
-static void launch_vjp_t(bool guard, unsigned items, hipStream_t s, const float4* src,
+static void launch_vjp_t(bool guard, unsigned items, hipStream_t s, const OrderBufs& o,
                          const float4* mu, int64_t np_pad, const int4* work, float eps2,
                          float4* part)
 {
     if (guard)
-        hipLaunchKernelGGL((k_vjp_sym_f32<IPT, true>), dim3(items), dim3(kBlock), 0, s, src, mu,
-                           np_pad, work, eps2, part);
+        launch_vjp_g<IPT, true>(items, s, o, mu, np_pad, work, eps2, part);
     else
-        hipLaunchKernelGGL((k_vjp_sym_f32<IPT, false>), dim3(items), dim3(kBlock), 0, s, src, mu,
-                           np_pad, work, eps2, part);
+        launch_vjp_g<IPT, false>(items, s, o, mu, np_pad, work, eps2, part);
 }
 
 struct AdjWork {
@@ -336,30 +406,29 @@ static int reverse_step(gh_ctx* c, const AdjWork& w, const double* xh, const dou
 {
     const double cdrift = 0.5 * dt * kDriftScale;
     const double dtA = dt * kAccelScale;
-    float4* src = c->adj_src.as<float4>();
     float4* mu = c->adj_mu.as<float4>();
     double* mv = c->adj_mv.as<double>();
     float4* part = c->adj_part.as<float4>();
-    hipLaunchKernelGGL(k_pack_xm, dim3(blocks_for(w.n_pad, kBlock)), dim3(kBlock), 0, s, xh, m, n,
-                       w.n_pad, src);
-    GH_TRY(check_launch(c, "k_pack_xm"));
+    OrderBufs& o = c->ord_p;
+    GH_TRY(spatial_order(c, o, xh, n, s));
+    GH_TRY(gather_frames(c, o, xh, m, o.perm(), n, w.n_pad, kBlock * w.ipt, nullptr, true, s));
     hipLaunchKernelGGL(k_adj_pre, dim3(blocks_for(w.n_pad, kBlock)), dim3(kBlock), 0, s, lx, lv,
-                       n, w.n_pad, cdrift, dtA, mv, mu);
+                       n, w.n_pad, o.perm(), cdrift, dtA, mv, mu);
     GH_TRY(check_launch(c, "k_adj_pre"));
     const float eps2 = (float)(eps * eps);
-    const bool guard = !(eps2 >= 1e-30f);
+    const bool guard = fp32_guard(eps);
     {
         TimedScope ts(c, s, GH_T_VJP);
         switch (w.ipt) {
-            case 1: launch_vjp_t<1>(guard, (unsigned)w.items, s, src, mu, w.n_pad, c->adj_work.as<int4>(), eps2, part); break;
-            case 2: launch_vjp_t<2>(guard, (unsigned)w.items, s, src, mu, w.n_pad, c->adj_work.as<int4>(), eps2, part); break;
-            case 4: launch_vjp_t<4>(guard, (unsigned)w.items, s, src, mu, w.n_pad, c->adj_work.as<int4>(), eps2, part); break;
+            case 1: launch_vjp_t<1>(guard, (unsigned)w.items, s, o, mu, w.n_pad, c->adj_work.as<int4>(), eps2, part); break;
+            case 2: launch_vjp_t<2>(guard, (unsigned)w.items, s, o, mu, w.n_pad, c->adj_work.as<int4>(), eps2, part); break;
+            case 4: launch_vjp_t<4>(guard, (unsigned)w.items, s, o, mu, w.n_pad, c->adj_work.as<int4>(), eps2, part); break;
             default: return fail(c, GH_EARG, "internal: vjp shape");
         }
         GH_TRY(check_launch(c, "k_vjp_sym_f32"));
     }
     hipLaunchKernelGGL(k_adj_post, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, part, w.nb,
-                       w.n_pad, n, mv, cdrift, lx, lv, lm);
+                       w.n_pad, n, o.perm(), mv, cdrift, lx, lv, lm);
     return check_launch(c, "k_adj_post");
 }
 
@@ -426,7 +495,6 @@ int gh_adjoint_backward(gh_ctx* c, const double* d_ckpt, const double* d_m, int6
     if (nsteps == 0) return 0;
     AdjWork w;
     GH_TRY(adj_prepare(c, n, w, s));
-    GH_TRY(ensure(c, c->adj_src, sizeof(float4) * (size_t)w.n_pad));
     GH_TRY(ensure(c, c->adj_mu, sizeof(float4) * (size_t)w.n_pad));
     GH_TRY(ensure(c, c->adj_mv, sizeof(double) * (size_t)n3));
     GH_TRY(ensure(c, c->adj_part, sizeof(float4) * (size_t)w.n_pad * w.nb));

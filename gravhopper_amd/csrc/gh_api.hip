@@ -100,9 +100,12 @@ static void release_spans(gh_ctx* c)
     c->spans.clear();
 }
 
-static void free_buf(DevBuf& b)
+void release(gh_ctx* c, DevBuf& b)
 {
-    if (b.p) (void)hipFree(b.p);
+    if (b.p) {
+        (void)hipFree(b.p);
+        c->arena_bytes -= b.cap;
+    }
     b.p = nullptr;
     b.cap = 0;
 }
@@ -431,8 +434,14 @@ void gh_ctx_destroy(gh_ctx* c)
                       &c->xh,     &c->hist_p, &c->hist_v, &c->scratch, &c->counters,
                       &c->work,   &c->eps_blk, &c->adj_work, &c->adj_src, &c->adj_mu,
                       &c->adj_mv, &c->adj_part, &c->adj_seg, &c->slot};
-    for (DevBuf* b : bufs) free_buf(*b);
-    free_buf(c->ext);
+    for (DevBuf* b : bufs) release(c, *b);
+    release(c, c->ext);
+    for (OrderBufs* o : {&c->ord_p, &c->ord_t}) {
+        DevBuf* ob[] = {&o->keys[0], &o->keys[1], &o->idx[0], &o->idx[1], &o->hist, &o->box,
+                        &o->boxpart, &o->loc,     &o->s64,    &o->frame,  &o->flag,
+                        &o->nflag};
+        for (DevBuf* b : ob) release(c, *b);
+    }
     release_spans(c);
     for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
     for (hipEvent_t e : c->ext_events)

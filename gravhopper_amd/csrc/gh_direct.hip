@@ -13,6 +13,10 @@
 //                 broadcasts, FP32 sums per tile and an FP32 two-sum (hi, lo) across
 //                 tiles; FP64 out.  Sources may be split over grid.y when there are too
 //                 few targets to fill 256 CUs; the splits are added in split order.
+//  k_direct_sym_f32  the self field with each unordered pair computed once.
+//  Both FP32 kernels form coordinates only relative to FP64 origins of groups of
+//  spatially sorted particles (gh_frames.hip), so their accuracy does not depend on
+//  where the system sits; non-compact groups take double-single instances.
 #include <cmath>
 
 #include "gh_internal.h"
@@ -20,41 +24,6 @@
 namespace gh {
 
 int64_t pad_to(int64_t n, int64_t m) { return (n + m - 1) / m * m; }
-
-// ------------------------------------------------------------------------------
-// packing
-// ------------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_pack_src64(const double* __restrict__ pos,
-                                                       const double* __restrict__ mass,
-                                                       int64_t n, double4* __restrict__ out)
-{
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i < n) out[i] = make_double4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], mass[i]);
-}
-
-// float4 (x, y, z, m); entries n..n_pad-1 are massless particles at the origin, whose
-// contribution is exactly zero whenever the pair arithmetic is finite.
-__global__ __launch_bounds__(kBlock) void k_pack_src32(const double* __restrict__ pos,
-                                                       const double* __restrict__ mass,
-                                                       int64_t n, int64_t n_pad,
-                                                       float4* __restrict__ out)
-{
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i < n)
-        out[i] = make_float4((float)pos[3 * i], (float)pos[3 * i + 1], (float)pos[3 * i + 2],
-                             (float)mass[i]);
-    else if (i < n_pad)
-        out[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-}
-
-int pack_src32(gh_ctx* c, const double* d_pos, const double* d_mass, int64_t np,
-               float4* out, int64_t np_pad, hipStream_t s)
-{
-    if (np_pad <= 0) return 0;
-    hipLaunchKernelGGL(k_pack_src32, dim3(blocks_for(np_pad, kBlock)), dim3(kBlock), 0, s,
-                       d_pos, d_mass, np, np_pad, out);
-    return check_launch(c, "k_pack_src32");
-}
 
 // ------------------------------------------------------------------------------
 // FP64, reference operation order
@@ -305,6 +274,12 @@ static int f64_targets_per_wave(const gh_ctx* c, int64_t nt)
 // ------------------------------------------------------------------------------
 // FP32 pairs, compensated accumulation
 // ------------------------------------------------------------------------------
+// Every FP32 kernel works in local frames (gh_frames.hip): the particles are sorted
+// along a Morton curve, each block of consecutive sorted particles has an FP64 origin,
+// and a kernel forms its FP32 coordinates relative to one block's origin from FP64
+// values.  Blocks whose flag is set (not compact) are handled by the double-single
+// (DS) instances, which carry every coordinate as hi + lo FP32 words of an FP64
+// difference and form separations as (hi_j - hi_i) + (lo_j - lo_i).
 __device__ __forceinline__ void two_sum_into(float& hi, float& lo, float t)
 {
     // Knuth's branch-free two-sum: hi + lo + t == s + (lo + e) exactly before the
@@ -316,35 +291,82 @@ __device__ __forceinline__ void two_sum_into(float& hi, float& lo, float t)
     lo = lo + e;
 }
 
-template <int IPT, bool GUARD>
-__global__ __launch_bounds__(kBlock) void k_direct_f32(const float4* __restrict__ src,
+// hi + lo split of an FP64 value: hi = fl32(d), lo = fl32(d - hi) (d - hi is exact).
+__device__ __forceinline__ void split_ds(double d, float& hi, float& lo)
+{
+    hi = (float)d;
+    lo = (float)(d - (double)hi);
+}
+
+// One-sided kernel: IPT targets per lane, 256-source tiles staged in LDS and read as
+// wave-wide broadcasts, FP32 sums per tile and an FP32 two-sum (hi, lo) across tiles;
+// FP64 out.  Workgroup b holds the sorted targets of group b (256 IPT of them), whose
+// frame is tframe[b]; every source tile is re-based into that frame from the FP64
+// sources (src64, any order, massless padding).  A workgroup runs only in the
+// instance matching its group's flag (DS = 1: double-single separations).  Sources may
+// be split over grid.y when there are too few targets to fill 256 CUs; the splits are
+// added in split order.  perm (sorted target -> caller's index) places the results;
+// with splits the per-split fields go to out in sorted order instead.
+template <int IPT, bool GUARD, bool DS>
+__global__ __launch_bounds__(kBlock) void k_direct_f32(const double4* __restrict__ src64,
                                                        int64_t np_pad, int64_t j_per_split,
-                                                       const float4* __restrict__ tgt,
+                                                       const float4* __restrict__ tloc,
+                                                       const double4* __restrict__ t64,
+                                                       const double4* __restrict__ tframe,
+                                                       const int* __restrict__ tflag,
+                                                       const u32* __restrict__ perm,
                                                        int64_t nt, float eps2,
                                                        double* __restrict__ out)
 {
     __shared__ float4 tile[kBlock];
+    __shared__ float4 tile_lo[DS ? kBlock : 1];
+    if ((tflag[blockIdx.x] != 0) != DS) return;
+    const double4 o = tframe[blockIdx.x];
     const int64_t i0 = (int64_t)blockIdx.x * (kBlock * IPT) + threadIdx.x;
     const int64_t j_begin = (int64_t)blockIdx.y * j_per_split;
     const int64_t j_end = min(np_pad, j_begin + j_per_split);
 
     eps2 = in_vgpr(eps2);
-    float xi[IPT], yi[IPT], zi[IPT];
+    float xi[IPT], yi[IPT], zi[IPT], xl[IPT], yl[IPT], zl[IPT];
     float hx[IPT], hy[IPT], hz[IPT], lx[IPT], ly[IPT], lz[IPT];
 #pragma unroll
     for (int k = 0; k < IPT; k++) {
         const int64_t i = i0 + (int64_t)k * kBlock;
-        float4 t = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (i < nt) t = tgt[i];
-        xi[k] = t.x;
-        yi[k] = t.y;
-        zi[k] = t.z;
+        xi[k] = yi[k] = zi[k] = xl[k] = yl[k] = zl[k] = 0.f;
+        if (i < nt) {
+            if (DS) {
+                const double4 q = t64[i];
+                split_ds(q.x - o.x, xi[k], xl[k]);
+                split_ds(q.y - o.y, yi[k], yl[k]);
+                split_ds(q.z - o.z, zi[k], zl[k]);
+            } else {
+                const float4 t = tloc[i];
+                xi[k] = t.x;
+                yi[k] = t.y;
+                zi[k] = t.z;
+            }
+        }
         hx[k] = hy[k] = hz[k] = lx[k] = ly[k] = lz[k] = 0.f;
     }
 
     for (int64_t base = j_begin; base < j_end; base += kBlock) {
         __syncthreads();
-        tile[threadIdx.x] = src[base + threadIdx.x];
+        {
+            const double4 q = src64[base + threadIdx.x];
+            if (DS) {
+                float4 h, l;
+                split_ds(q.x - o.x, h.x, l.x);
+                split_ds(q.y - o.y, h.y, l.y);
+                split_ds(q.z - o.z, h.z, l.z);
+                h.w = (float)q.w;
+                l.w = 0.f;
+                tile[threadIdx.x] = h;
+                tile_lo[threadIdx.x] = l;
+            } else {
+                tile[threadIdx.x] = make_float4((float)(q.x - o.x), (float)(q.y - o.y),
+                                                (float)(q.z - o.z), (float)q.w);
+            }
+        }
         __syncthreads();
         float ax[IPT], ay[IPT], az[IPT];
 #pragma unroll
@@ -352,18 +374,25 @@ __global__ __launch_bounds__(kBlock) void k_direct_f32(const float4* __restrict_
 #pragma unroll 8
         for (int jj = 0; jj < kBlock; jj++) {
             const float4 s = tile[jj];
+            float4 sl = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (DS) sl = tile_lo[jj];
 #pragma unroll
             for (int k = 0; k < IPT; k++) {
-                const float dx = s.x - xi[k];
-                const float dy = s.y - yi[k];
-                const float dz = s.z - zi[k];
+                float dx = s.x - xi[k];
+                float dy = s.y - yi[k];
+                float dz = s.z - zi[k];
+                if (DS) {
+                    dx = dx + (sl.x - xl[k]);
+                    dy = dy + (sl.y - yl[k]);
+                    dz = dz + (sl.z - zl[k]);
+                }
                 float r2 = __builtin_fmaf(dx, dx, eps2);
                 r2 = __builtin_fmaf(dy, dy, r2);
                 r2 = __builtin_fmaf(dz, dz, r2);
                 const float ri = __builtin_amdgcn_rsqf(r2);
                 const float ri2 = ri * ri;
                 float sc = (s.w * ri) * ri2;
-                if (GUARD) sc = (r2 > 0.f) ? sc : 0.f;
+                if (GUARD) sc = clamp_finite(sc);
                 ax[k] = __builtin_fmaf(dx, sc, ax[k]);
                 ay[k] = __builtin_fmaf(dy, sc, ay[k]);
                 az[k] = __builtin_fmaf(dz, sc, az[k]);
@@ -377,14 +406,16 @@ __global__ __launch_bounds__(kBlock) void k_direct_f32(const float4* __restrict_
         }
     }
 
-    double* o = out + (int64_t)blockIdx.y * nt * 3;
+    const bool split = gridDim.y > 1;
+    double* o_split = out + (int64_t)blockIdx.y * nt * 3;
 #pragma unroll
     for (int k = 0; k < IPT; k++) {
         const int64_t i = i0 + (int64_t)k * kBlock;
         if (i < nt) {
-            o[3 * i] = (double)hx[k] + (double)lx[k];
-            o[3 * i + 1] = (double)hy[k] + (double)ly[k];
-            o[3 * i + 2] = (double)hz[k] + (double)lz[k];
+            double* d = split ? o_split + 3 * i : out + 3 * (int64_t)perm[i];
+            d[0] = (double)hx[k] + (double)lx[k];
+            d[1] = (double)hy[k] + (double)ly[k];
+            d[2] = (double)hz[k] + (double)lz[k];
         }
     }
 }
@@ -392,14 +423,18 @@ __global__ __launch_bounds__(kBlock) void k_direct_f32(const float4* __restrict_
 // ------------------------------------------------------------------------------
 // FP32 pairs, each unordered pair computed once (self field only)
 // ------------------------------------------------------------------------------
-// Particles are cut into blocks of BLK = 256*IPT.  A workgroup takes one block pair
-// (I, J), I <= J.  Its 4 wavefronts hold the 256*IPT i-particles of block I in
-// registers (IPT per lane); block J is staged in LDS.  For I < J, block J is walked in
-// chunks of 64: at step s lane l pairs its i-particles with j = (l + s) mod 64 of the
-// chunk and accumulates both the force on its i-particles and the force on that j in
-// a register that moves to lane l-1 after every step (DPP wavefront rotation), so
+// Particles (in sorted order) are cut into blocks of BLK = 256*IPT.  A workgroup takes
+// one block pair (I, J), I <= J.  Its 4 wavefronts hold the 256*IPT i-particles of
+// block I in registers (IPT per lane); block J is staged in LDS.  For I < J, block J is
+// walked in chunks of 64: at step s lane l pairs its i-particles with j = (l + s) mod 64
+// of the chunk and accumulates both the force on its i-particles and the force on that
+// j in a register that moves to lane l-1 after every step (DPP wavefront rotation), so
 // after 64 steps lane l holds the whole wavefront's force on j = l.  The four
 // wavefronts' j-partials are added in LDS.  A diagonal pair I = J is done one-sided.
+// The workgroup works in block I's frame: i-particles are block I's local coordinates,
+// block J's enter as fl(loc_J + fl(o_J - o_I)).  Pairs with a flagged block are left to
+// the DS instance, which stages both sides as hi + lo words of the FP64 differences
+// from o_I.
 //
 // Results are partial fields part[K][p] (FP32, 3 per particle): particle p of block B
 // receives one partial from every block K (K = B for its diagonal); k_reduce_sym adds
@@ -416,10 +451,12 @@ __device__ __forceinline__ float rot_from_next(float v)
 // One block pair.  Work item {I, J, slot_i, slot_j}: global block indices I <= J (of
 // one simulation), the partial slot the i-side field goes to (J's index within its
 // simulation) and the slot of the j-side field (I's index within its simulation).
-template <int IPT, bool GUARD>
-__device__ __forceinline__ void sym_item(const float4* __restrict__ src, int64_t np_pad,
+template <int IPT, bool GUARD, bool DS>
+__device__ __forceinline__ void sym_item(const float4* __restrict__ src,
+                                         const double4* __restrict__ s64,
+                                         const double4* __restrict__ frame, int64_t np_pad,
                                          int4 it, float eps2, float* __restrict__ part,
-                                         float4* tile, float (*jp)[3][64])
+                                         float4* tile, float4* tile_lo, float (*jp)[3][64])
 {
     constexpr int BLK = kBlock * IPT;
     constexpr int NCH = BLK / 64;
@@ -428,19 +465,52 @@ __device__ __forceinline__ void sym_item(const float4* __restrict__ src, int64_t
     const int64_t ibase = (int64_t)I * BLK + wave * 64 * IPT + lane;
     const int64_t jbase = (int64_t)J * BLK;
     eps2 = in_vgpr(eps2);
+    const double4 oI = frame[I];
 
     float xi[IPT], yi[IPT], zi[IPT], mi[IPT], ax[IPT], ay[IPT], az[IPT];
+    float xl[IPT], yl[IPT], zl[IPT];
 #pragma unroll
     for (int k = 0; k < IPT; k++) {
-        const float4 t = src[ibase + k * 64];
-        xi[k] = t.x;
-        yi[k] = t.y;
-        zi[k] = t.z;
-        mi[k] = t.w;
+        if (DS) {
+            const double4 q = s64[ibase + k * 64];
+            split_ds(q.x - oI.x, xi[k], xl[k]);
+            split_ds(q.y - oI.y, yi[k], yl[k]);
+            split_ds(q.z - oI.z, zi[k], zl[k]);
+            mi[k] = (float)q.w;
+        } else {
+            const float4 t = src[ibase + k * 64];
+            xi[k] = t.x;
+            yi[k] = t.y;
+            zi[k] = t.z;
+            mi[k] = t.w;
+            xl[k] = yl[k] = zl[k] = 0.f;
+        }
         ax[k] = ay[k] = az[k] = 0.f;
     }
-    for (int q = threadIdx.x; q < NCH * 128; q += kBlock)
-        tile[q] = src[jbase + (q >> 7) * 64 + (q & 63)];
+    if (DS) {
+        for (int q = threadIdx.x; q < NCH * 128; q += kBlock) {
+            const double4 v = s64[jbase + (q >> 7) * 64 + (q & 63)];
+            float4 h, l;
+            split_ds(v.x - oI.x, h.x, l.x);
+            split_ds(v.y - oI.y, h.y, l.y);
+            split_ds(v.z - oI.z, h.z, l.z);
+            h.w = (float)v.w;
+            l.w = 0.f;
+            tile[q] = h;
+            tile_lo[q] = l;
+        }
+    } else {
+        const double4 oJ = frame[J];
+        const float offx = (float)(oJ.x - oI.x), offy = (float)(oJ.y - oI.y),
+                    offz = (float)(oJ.z - oI.z);
+        for (int q = threadIdx.x; q < NCH * 128; q += kBlock) {
+            float4 v = src[jbase + (q >> 7) * 64 + (q & 63)];
+            v.x += offx;
+            v.y += offy;
+            v.z += offz;
+            tile[q] = v;
+        }
+    }
     __syncthreads();
 
     if (I == J) {
@@ -448,15 +518,22 @@ __device__ __forceinline__ void sym_item(const float4* __restrict__ src, int64_t
 #pragma unroll 8
             for (int q = 0; q < 64; q++) {
                 const float4 s = tile[c * 128 + q];
+                float4 sl = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (DS) sl = tile_lo[c * 128 + q];
 #pragma unroll
                 for (int k = 0; k < IPT; k++) {
-                    const float dx = s.x - xi[k], dy = s.y - yi[k], dz = s.z - zi[k];
+                    float dx = s.x - xi[k], dy = s.y - yi[k], dz = s.z - zi[k];
+                    if (DS) {
+                        dx = dx + (sl.x - xl[k]);
+                        dy = dy + (sl.y - yl[k]);
+                        dz = dz + (sl.z - zl[k]);
+                    }
                     float r2 = __builtin_fmaf(dx, dx, eps2);
                     r2 = __builtin_fmaf(dy, dy, r2);
                     r2 = __builtin_fmaf(dz, dz, r2);
                     const float ri = __builtin_amdgcn_rsqf(r2);
                     float sc = (s.w * ri) * (ri * ri);
-                    if (GUARD) sc = (r2 > 0.f) ? sc : 0.f;
+                    if (GUARD) sc = clamp_finite(sc);
                     ax[k] = __builtin_fmaf(dx, sc, ax[k]);
                     ay[k] = __builtin_fmaf(dy, sc, ay[k]);
                     az[k] = __builtin_fmaf(dz, sc, az[k]);
@@ -466,25 +543,34 @@ __device__ __forceinline__ void sym_item(const float4* __restrict__ src, int64_t
     } else {
         for (int c = 0; c < NCH; c++) {
             const float4* row = &tile[c * 128 + lane];
+            const float4* row_lo = &tile_lo[DS ? c * 128 + lane : 0];
             float fx = 0.f, fy = 0.f, fz = 0.f;
             for (int s0 = 0; s0 < 64; s0 += 8) {
 #pragma unroll
                 for (int s = 0; s < 8; s++) {
                     const float4 sj = row[s0 + s];
+                    float4 sl = make_float4(0.f, 0.f, 0.f, 0.f);
+                    if (DS) sl = row_lo[s0 + s];
 #pragma unroll
                     for (int k = 0; k < IPT; k++) {
-                        const float dx = sj.x - xi[k], dy = sj.y - yi[k], dz = sj.z - zi[k];
+                        float dx = sj.x - xi[k], dy = sj.y - yi[k], dz = sj.z - zi[k];
+                        if (DS) {
+                            dx = dx + (sl.x - xl[k]);
+                            dy = dy + (sl.y - yl[k]);
+                            dz = dz + (sl.z - zl[k]);
+                        }
                         float r2 = __builtin_fmaf(dx, dx, eps2);
                         r2 = __builtin_fmaf(dy, dy, r2);
                         r2 = __builtin_fmaf(dz, dz, r2);
                         const float ri = __builtin_amdgcn_rsqf(r2);
-                        float ri3 = ri * ri * ri;
-                        if (GUARD) ri3 = (r2 > 0.f) ? ri3 : 0.f;
-                        const float si = sj.w * ri3;
+                        const float ri3 = ri * ri * ri;
+                        float si = sj.w * ri3;
+                        if (GUARD) si = clamp_finite(si);
                         ax[k] = __builtin_fmaf(dx, si, ax[k]);
                         ay[k] = __builtin_fmaf(dy, si, ay[k]);
                         az[k] = __builtin_fmaf(dz, si, az[k]);
-                        const float sm = mi[k] * ri3;
+                        float sm = mi[k] * ri3;
+                        if (GUARD) sm = clamp_finite(sm);
                         fx = __builtin_fmaf(dx, sm, fx);
                         fy = __builtin_fmaf(dy, sm, fy);
                         fz = __builtin_fmaf(dz, sm, fz);
@@ -519,11 +605,14 @@ __device__ __forceinline__ void sym_item(const float4* __restrict__ src, int64_t
     }
 }
 
-// One launch over a list of block pairs.  queue == nullptr: workgroup b takes item b.
-// Otherwise the workgroups are persistent and take items from *queue (which must be 0
-// at launch) until the list is exhausted.
+// One launch over a list of block pairs; items with a flagged block are skipped (the DS
+// kernel takes them).  queue == nullptr: workgroup b takes item b.  Otherwise the
+// workgroups are persistent and take items from *queue (which must be 0 at launch)
+// until the list is exhausted.
 template <int IPT, bool GUARD>
 __global__ __launch_bounds__(kBlock) void k_direct_sym_f32(const float4* __restrict__ src,
+                                                           const double4* __restrict__ frame,
+                                                           const int* __restrict__ flag,
                                                            int64_t np_pad,
                                                            const int4* __restrict__ work,
                                                            int64_t items,
@@ -539,7 +628,9 @@ __global__ __launch_bounds__(kBlock) void k_direct_sym_f32(const float4* __restr
     __shared__ int next;
     if (!queue) {
         const int4 it = work[blockIdx.x];
-        sym_item<IPT, GUARD>(src, np_pad, it, eps2_blk[it.x], part, tile, jp);
+        if (flag[it.x] | flag[it.y]) return;
+        sym_item<IPT, GUARD, false>(src, nullptr, frame, np_pad, it, eps2_blk[it.x], part, tile,
+                                    nullptr, jp);
         return;
     }
     for (;;) {
@@ -549,21 +640,51 @@ __global__ __launch_bounds__(kBlock) void k_direct_sym_f32(const float4* __restr
         __syncthreads();
         if (k >= items) break;
         const int4 it = work[k];
-        sym_item<IPT, GUARD>(src, np_pad, it, eps2_blk[it.x], part, tile, jp);
+        if (flag[it.x] | flag[it.y]) continue;
+        sym_item<IPT, GUARD, false>(src, nullptr, frame, np_pad, it, eps2_blk[it.x], part, tile,
+                                    nullptr, jp);
         __syncthreads();
     }
 }
 
-// acc[p] = sum over the partial slots of p's simulation, in slot order, in FP64.
-// nb_blk[b] (nullable): number of slots of the simulation that block b belongs to;
-// otherwise every particle has nb slots.
-// Four consecutive components per thread, read as one 16-byte load per partial slab
-// (each slab, 3 np_pad floats with np_pad a multiple of the block size, starts on a
-// 16-byte boundary, and a group of four never straddles two blocks, as 3 x 256 is a
-// multiple of 4); every component is still summed in block order.
+// The items the fast kernel skips, in double-single form: a persistent grid strides
+// over the whole list and takes the items with a flagged block.
+template <int IPT, bool GUARD>
+__global__ __launch_bounds__(kBlock) void k_direct_sym_ds(const double4* __restrict__ s64,
+                                                          const double4* __restrict__ frame,
+                                                          const int* __restrict__ flag,
+                                                          const int* __restrict__ nflag,
+                                                          int64_t np_pad,
+                                                          const int4* __restrict__ work,
+                                                          int64_t items,
+                                                          const float* __restrict__ eps2_blk,
+                                                          float* __restrict__ part)
+{
+    constexpr int NCH = kBlock * IPT / 64;
+    __shared__ float4 tile[NCH * 128];
+    __shared__ float4 tile_lo[NCH * 128];
+    __shared__ float jp[kBlock / 64][3][64];
+    if (*nflag == 0) return;  // no flagged block: nothing to do
+    for (int64_t k = blockIdx.x; k < items; k += gridDim.x) {
+        const int4 it = work[k];
+        if (!(flag[it.x] | flag[it.y])) continue;
+        sym_item<IPT, GUARD, true>(nullptr, s64, frame, np_pad, it, eps2_blk[it.x], part, tile,
+                                   tile_lo, jp);
+        __syncthreads();
+    }
+}
+
+// acc[perm[p]] = sum over the partial slots of p's simulation, in slot order, in FP64
+// (perm == nullptr: acc[p]).  nb_blk[b] (nullable): number of slots of the simulation
+// that block b belongs to; otherwise every particle has nb slots.  Four consecutive
+// components per thread, read as one 16-byte load per partial slab (each slab, 3 np_pad
+// floats with np_pad a multiple of the block size, starts on a 16-byte boundary, and a
+// group of four never straddles two blocks, as 3 x 256 is a multiple of 4); every
+// component is still summed in block order.
 __global__ __launch_bounds__(kBlock) void k_reduce_sym(const float* __restrict__ part, int nb,
                                                        const int* __restrict__ nb_blk,
                                                        int blk, int64_t np_pad, int64_t n3,
+                                                       const u32* __restrict__ perm,
                                                        double* __restrict__ acc)
 {
     const int64_t k0 = ((int64_t)blockIdx.x * kBlock + threadIdx.x) * 4;
@@ -571,39 +692,50 @@ __global__ __launch_bounds__(kBlock) void k_reduce_sym(const float* __restrict__
     const int slots = nb_blk ? nb_blk[(k0 / 3) / blk] : nb;
     const float4* p = (const float4*)(part + k0);
     const int64_t stride = np_pad * 3 / 4;  // float4 per slab
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
     for (int b = 0; b < slots; b++) {
         const float4 v = p[(int64_t)b * stride];
-        s0 += (double)v.x;
-        s1 += (double)v.y;
-        s2 += (double)v.z;
-        s3 += (double)v.w;
+        s[0] += (double)v.x;
+        s[1] += (double)v.y;
+        s[2] += (double)v.z;
+        s[3] += (double)v.w;
     }
-    acc[k0] = s0;
-    if (k0 + 1 < n3) acc[k0 + 1] = s1;
-    if (k0 + 2 < n3) acc[k0 + 2] = s2;
-    if (k0 + 3 < n3) acc[k0 + 3] = s3;
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        const int64_t k = k0 + t;
+        if (k < n3) {
+            const int64_t q = k / 3;
+            acc[perm ? 3 * (int64_t)perm[q] + (k - 3 * q) : k] = s[t];
+        }
+    }
 }
 
-// Sum of per-split partial fields, in split order.
+// Sum of per-split partial fields (sorted target order), in split order, written to
+// acc[perm[i]].
 __global__ __launch_bounds__(kBlock) void k_reduce_splits(const double* __restrict__ part,
                                                           int splits, int64_t n3,
+                                                          const u32* __restrict__ perm,
                                                           double* __restrict__ acc)
 {
     const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (k >= n3) return;
     double s = part[k];
     for (int p = 1; p < splits; p++) s += part[(int64_t)p * n3 + k];
-    acc[k] = s;
+    const int64_t q = k / 3;
+    acc[3 * (int64_t)perm[q] + (k - 3 * q)] = s;
 }
 
-__global__ __launch_bounds__(kBlock) void k_pack_tgt32(const double* __restrict__ pos,
-                                                       int64_t n, float4* __restrict__ out)
+// (x, y, z, m) in input order, entries n..n_pad-1 massless at the origin
+__global__ __launch_bounds__(kBlock) void k_pack_src64(const double* __restrict__ pos,
+                                                       const double* __restrict__ mass,
+                                                       int64_t n, int64_t n_pad,
+                                                       double4* __restrict__ out)
 {
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i < n)
-        out[i] = make_float4((float)pos[3 * i], (float)pos[3 * i + 1], (float)pos[3 * i + 2],
-                             0.f);
+        out[i] = make_double4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], mass[i]);
+    else if (i < n_pad)
+        out[i] = make_double4(0.0, 0.0, 0.0, 0.0);
 }
 
 // ------------------------------------------------------------------------------
@@ -636,19 +768,34 @@ F32Shape pick_shape(gh_ctx* c, int64_t nt, int64_t np_pad)
     return sh;
 }
 
-template <int IPT>
-void launch_f32(bool guard, dim3 grid, hipStream_t s, const float4* src, int64_t np_pad,
-                int64_t jps, const float4* tgt, int64_t nt, float eps2, double* out)
+template <int IPT, bool DS>
+void launch_f32(bool guard, dim3 grid, hipStream_t s, const double4* src64, int64_t np_pad,
+                int64_t jps, const OrderBufs& t, int64_t nt, float eps2, double* out)
 {
+    const float4* tloc = t.loc.as<float4>();
+    const double4* t64 = t.s64.as<double4>();
+    const double4* fr = t.frame.as<double4>();
+    const int* fl = t.flag.as<int>();
+    const u32* perm = t.perm();
     if (guard)
-        hipLaunchKernelGGL((k_direct_f32<IPT, true>), grid, dim3(kBlock), 0, s, src, np_pad,
-                           jps, tgt, nt, eps2, out);
+        hipLaunchKernelGGL((k_direct_f32<IPT, true, DS>), grid, dim3(kBlock), 0, s, src64,
+                           np_pad, jps, tloc, t64, fr, fl, perm, nt, eps2, out);
     else
-        hipLaunchKernelGGL((k_direct_f32<IPT, false>), grid, dim3(kBlock), 0, s, src, np_pad,
-                           jps, tgt, nt, eps2, out);
+        hipLaunchKernelGGL((k_direct_f32<IPT, false, DS>), grid, dim3(kBlock), 0, s, src64,
+                           np_pad, jps, tloc, t64, fr, fl, perm, nt, eps2, out);
 }
 
 }  // namespace
+
+// FP32 pair arithmetic needs the zero-separation guard whenever a coincident pair (or
+// the self pair of the one-sided kernels) could produce 0 * inf: m / eps^3 overflows FP32
+// for eps^2 below about 1e-15 once masses reach 1e15 Msun, and the guard is also what
+// the reference does for targets at zero distance (:327-330).
+bool fp32_guard(double eps)
+{
+    const float eps2 = (float)(eps * eps);
+    return !(eps2 >= 1e-15f);
+}
 
 // Symmetric (pair-once) kernel for the self field: particles per lane, or 0 for the
 // one-sided kernel.  Block pairs must number a few thousand to fill 256 CUs.
@@ -680,41 +827,49 @@ int sym_ipt(gh_ctx* c, int64_t np)
 
 template <int IPT>
 static void launch_sym_t(bool guard, unsigned grid, hipStream_t s, const float4* src,
-                         int64_t np_pad, const int4* work, int64_t items, const float* eps2_blk,
-                         float* part, int* queue)
+                         const double4* s64, const double4* frame, const int* flag,
+                         const int* nflag, int64_t np_pad, const int4* work, int64_t items,
+                         const float* eps2_blk, float* part, int* queue, unsigned ds_grid)
 {
-    if (guard)
+    if (guard) {
         hipLaunchKernelGGL((k_direct_sym_f32<IPT, true>), dim3(grid), dim3(kBlock), 0, s, src,
-                           np_pad, work, items, eps2_blk, part, queue);
-    else
+                           frame, flag, np_pad, work, items, eps2_blk, part, queue);
+        hipLaunchKernelGGL((k_direct_sym_ds<IPT, true>), dim3(ds_grid), dim3(kBlock), 0, s, s64,
+                           frame, flag, nflag, np_pad, work, items, eps2_blk, part);
+    } else {
         hipLaunchKernelGGL((k_direct_sym_f32<IPT, false>), dim3(grid), dim3(kBlock), 0, s, src,
-                           np_pad, work, items, eps2_blk, part, queue);
+                           frame, flag, np_pad, work, items, eps2_blk, part, queue);
+        hipLaunchKernelGGL((k_direct_sym_ds<IPT, false>), dim3(ds_grid), dim3(kBlock), 0, s,
+                           s64, frame, flag, nflag, np_pad, work, items, eps2_blk, part);
+    }
 }
 
-int launch_sym(gh_ctx* c, int ipt, bool guard, const float4* src, int64_t np_pad,
+int launch_sym(gh_ctx* c, int ipt, bool guard, const float4* src, const double4* s64,
+               const double4* frame, const int* flag, const int* nflag, int64_t np_pad,
                const int4* work, int64_t items, const float* eps2_blk, float* part, int* queue,
                unsigned persistent_grid, hipStream_t s)
 {
     if (items > 0x7fffffff) return fail(c, GH_EARG, "too many block pairs for one launch");
     const unsigned grid = queue ? persistent_grid : (unsigned)items;
     if (grid == 0) return 0;
+    const unsigned ds_grid = (unsigned)std::min<int64_t>(items, 2 * (int64_t)c->n_cu);
     switch (ipt) {
-        case 1: launch_sym_t<1>(guard, grid, s, src, np_pad, work, items, eps2_blk, part, queue); break;
-        case 2: launch_sym_t<2>(guard, grid, s, src, np_pad, work, items, eps2_blk, part, queue); break;
-        case 4: launch_sym_t<4>(guard, grid, s, src, np_pad, work, items, eps2_blk, part, queue); break;
+        case 1: launch_sym_t<1>(guard, grid, s, src, s64, frame, flag, nflag, np_pad, work, items, eps2_blk, part, queue, ds_grid); break;
+        case 2: launch_sym_t<2>(guard, grid, s, src, s64, frame, flag, nflag, np_pad, work, items, eps2_blk, part, queue, ds_grid); break;
+        case 4: launch_sym_t<4>(guard, grid, s, src, s64, frame, flag, nflag, np_pad, work, items, eps2_blk, part, queue, ds_grid); break;
         default: return fail(c, GH_EARG, "unsupported particles-per-lane %d", ipt);
     }
     return check_launch(c, "k_direct_sym_f32");
 }
 
 int reduce_sym(gh_ctx* c, const float* part, int nb, const int* nb_blk, int blk,
-               int64_t np_pad, int64_t n, double* acc, hipStream_t s)
+               int64_t np_pad, int64_t n, const u32* perm, double* acc, hipStream_t s)
 {
     if (n <= 0) return 0;
     if ((np_pad * 3) % 4 != 0 || (3 * blk) % 4 != 0)
         return fail(c, GH_EARG, "partial slabs are not 16-byte aligned");
     hipLaunchKernelGGL(k_reduce_sym, dim3(blocks_for((3 * n + 3) / 4, kBlock)), dim3(kBlock), 0,
-                       s, part, nb, nb_blk, blk, np_pad, 3 * n, acc);
+                       s, part, nb, nb_blk, blk, np_pad, 3 * n, perm, acc);
     return check_launch(c, "k_reduce_sym");
 }
 
@@ -732,7 +887,6 @@ int direct_sym(gh_ctx* c, const double* d_pos, const double* d_mass, int64_t np,
     const int nb = (int)(np_pad / blk);
     const int64_t items = (int64_t)nb * (nb + 1) / 2;
     if (items > 0x7fffffff) return fail(c, GH_EARG, "too many particles for one launch");
-    GH_TRY(ensure(c, c->src32a, sizeof(float4) * (size_t)np_pad));
     GH_TRY(ensure(c, c->scratch, sizeof(float) * 3 * (size_t)np_pad * nb));
     GH_TRY(ensure(c, c->eps_blk, sizeof(float) * (size_t)nb));
     const float eps2 = (float)(eps * eps);
@@ -748,35 +902,39 @@ int direct_sym(gh_ctx* c, const double* d_pos, const double* d_mass, int64_t np,
         GH_HIP(c, hipStreamSynchronize(s));
         c->sym_nb = nb;
         c->sym_ipt = ipt;
+        c->sym_eps2 = -1.f;
     }
-    hipLaunchKernelGGL(k_fill_f32, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, s,
-                       c->eps_blk.as<float>(), (int64_t)nb, eps2);
-    GH_TRY(check_launch(c, "k_fill_f32"));
-    float4* src = c->src32a.as<float4>();
-    GH_TRY(pack_src32(c, d_pos, d_mass, np, src, np_pad, s));
-    const bool guard = !(eps2 >= 1e-30f);
+    if (c->sym_eps2 != eps2) {  // the block list is rebuilt above whenever nb changes
+        hipLaunchKernelGGL(k_fill_f32, dim3(blocks_for(nb, kBlock)), dim3(kBlock), 0, s,
+                           c->eps_blk.as<float>(), (int64_t)nb, eps2);
+        GH_TRY(check_launch(c, "k_fill_f32"));
+        c->sym_eps2 = eps2;
+    }
+    OrderBufs& o = c->ord_p;
+    GH_TRY(spatial_order(c, o, d_pos, np, s));
+    GH_TRY(gather_frames(c, o, d_pos, d_mass, o.perm(), np, np_pad, (int)blk, nullptr, true, s));
     float* part = c->scratch.as<float>();
     {
         TimedScope ts(c, s, GH_T_DIRECT_F32);
-        GH_TRY(launch_sym(c, ipt, guard, src, np_pad, c->work.as<int4>(), items,
-                          c->eps_blk.as<float>(), part, nullptr, 0, s));
+        GH_TRY(launch_sym(c, ipt, fp32_guard(eps), o.loc.as<float4>(), o.s64.as<double4>(),
+                          o.frame.as<double4>(), o.flag.as<int>(), o.nflag.as<int>(), np_pad,
+                          c->work.as<int4>(),
+                          items, c->eps_blk.as<float>(), part, nullptr, 0, s));
     }
-    return reduce_sym(c, part, nb, nullptr, (int)blk, np_pad, np, d_acc, s);
+    return reduce_sym(c, part, nb, nullptr, (int)blk, np_pad, np, o.perm(), d_acc, s);
 }
 
-int direct_f32_packed(gh_ctx* c, const float4* src, int64_t np_pad, const float4* tgt,
-                      int64_t nt, bool self, double eps, double* d_acc, hipStream_t s)
+// One-sided FP32 field at nt targets.  src64: the sources as (x, y, z, m), np_pad of
+// them (massless padding); t: the targets' spatial order and frames in groups of
+// 256 x the chosen targets per lane.
+static int direct_f32_onesided(gh_ctx* c, const double4* src64, int64_t np_pad, OrderBufs& t,
+                               int64_t nt, const F32Shape& sh, double eps, double* d_acc,
+                               hipStream_t s)
 {
-    (void)self;  // the self pair contributes exactly 0 (dx = dy = dz = 0)
     if (nt <= 0) return 0;
     if (np_pad % kBlock != 0) return fail(c, GH_EARG, "internal: source count not padded");
     const float eps2 = (float)(eps * eps);
-    // Without softening (or with one that underflows in FP32) a zero separation would
-    // produce 0 * inf; guard it to 0 as the reference does for targets (:327-330).
-    const bool guard = !(eps2 >= 1e-30f);
-    F32Shape sh = pick_shape(c, nt, np_pad);
-    const int forced = c->tune[GH_TUNE_DIRECT_IPT];
-    if (forced == 1 || forced == 2 || forced == 4) sh.ipt = forced;
+    const bool guard = fp32_guard(eps);
     int64_t jps = pad_to((np_pad + sh.splits - 1) / sh.splits, kBlock);
     const int splits = (int)((np_pad + jps - 1) / jps);
     double* out = d_acc;
@@ -785,17 +943,28 @@ int direct_f32_packed(gh_ctx* c, const float4* src, int64_t np_pad, const float4
         out = c->scratch.as<double>();
     }
     dim3 grid(blocks_for(nt, kBlock * sh.ipt), splits);
-    TimedScope ts(c, s, GH_T_DIRECT_F32);
-    switch (sh.ipt) {
-        case 1: launch_f32<1>(guard, grid, s, src, np_pad, jps, tgt, nt, eps2, out); break;
-        case 2: launch_f32<2>(guard, grid, s, src, np_pad, jps, tgt, nt, eps2, out); break;
-        case 4: launch_f32<4>(guard, grid, s, src, np_pad, jps, tgt, nt, eps2, out); break;
-        default: return fail(c, GH_EARG, "unsupported targets-per-lane %d", sh.ipt);
+    {
+        TimedScope ts(c, s, GH_T_DIRECT_F32);
+        switch (sh.ipt) {
+            case 1:
+                launch_f32<1, false>(guard, grid, s, src64, np_pad, jps, t, nt, eps2, out);
+                launch_f32<1, true>(guard, grid, s, src64, np_pad, jps, t, nt, eps2, out);
+                break;
+            case 2:
+                launch_f32<2, false>(guard, grid, s, src64, np_pad, jps, t, nt, eps2, out);
+                launch_f32<2, true>(guard, grid, s, src64, np_pad, jps, t, nt, eps2, out);
+                break;
+            case 4:
+                launch_f32<4, false>(guard, grid, s, src64, np_pad, jps, t, nt, eps2, out);
+                launch_f32<4, true>(guard, grid, s, src64, np_pad, jps, t, nt, eps2, out);
+                break;
+            default: return fail(c, GH_EARG, "unsupported targets-per-lane %d", sh.ipt);
+        }
+        GH_TRY(check_launch(c, "k_direct_f32"));
     }
-    GH_TRY(check_launch(c, "k_direct_f32"));
     if (splits > 1) {
         hipLaunchKernelGGL(k_reduce_splits, dim3(blocks_for(3 * nt, kBlock)), dim3(kBlock), 0,
-                           s, out, splits, 3 * nt, d_acc);
+                           s, out, splits, 3 * nt, t.perm(), d_acc);
         GH_TRY(check_launch(c, "k_reduce_splits"));
     }
     return 0;
@@ -842,7 +1011,7 @@ int direct_dev(gh_ctx* c, const double* d_pos, const double* d_mass, int64_t np,
         GH_TRY(ensure(c, c->src64, sizeof(double4) * (size_t)np));
         double4* src = c->src64.as<double4>();
         hipLaunchKernelGGL(k_pack_src64, dim3(blocks_for(np, kBlock)), dim3(kBlock), 0, s,
-                           d_pos, d_mass, np, src);
+                           d_pos, d_mass, np, np, src);
         GH_TRY(check_launch(c, "k_pack_src64"));
         TimedScope ts(c, s, GH_T_DIRECT_F64);
         if (tpw == 64) {
@@ -878,16 +1047,28 @@ int direct_dev(gh_ctx* c, const double* d_pos, const double* d_mass, int64_t np,
     if (precision != GH_FP32) return fail(c, GH_EARG, "unknown precision %d", precision);
     const int sym = self ? sym_ipt(c, np) : 0;
     if (sym > 0) return direct_sym(c, d_pos, d_mass, np, sym, eps, d_acc, s);
+    F32Shape sh = pick_shape(c, nt, pad_to(np, kBlock));
+    const int forced = c->tune[GH_TUNE_DIRECT_IPT];
+    if (forced == 1 || forced == 2 || forced == 4) sh.ipt = forced;
+    const int group = kBlock * sh.ipt;
+    const int64_t nt_pad = pad_to(nt, group);
+    if (self) {
+        // targets and sources are the particles: one order, the sorted FP64 particles
+        // (massless padding at their group's origin) are the sources
+        OrderBufs& o = c->ord_p;
+        GH_TRY(spatial_order(c, o, d_pos, np, s));
+        GH_TRY(gather_frames(c, o, d_pos, d_mass, o.perm(), np, nt_pad, group, nullptr, true, s));
+        return direct_f32_onesided(c, o.s64.as<double4>(), nt_pad, o, nt, sh, eps, d_acc, s);
+    }
     const int64_t np_pad = pad_to(np, kBlock);
-    GH_TRY(ensure(c, c->src32a, sizeof(float4) * (size_t)np_pad));
-    GH_TRY(ensure(c, c->tgt32, sizeof(float4) * (size_t)nt));
-    float4* src = c->src32a.as<float4>();
-    float4* t32 = c->tgt32.as<float4>();
-    GH_TRY(pack_src32(c, d_pos, d_mass, np, src, np_pad, s));
-    hipLaunchKernelGGL(k_pack_tgt32, dim3(blocks_for(nt, kBlock)), dim3(kBlock), 0, s, tgt,
-                       nt, t32);
-    GH_TRY(check_launch(c, "k_pack_tgt32"));
-    return direct_f32_packed(c, src, np_pad, t32, nt, self, eps, d_acc, s);
+    GH_TRY(ensure(c, c->src64, sizeof(double4) * (size_t)np_pad));
+    hipLaunchKernelGGL(k_pack_src64, dim3(blocks_for(np_pad, kBlock)), dim3(kBlock), 0, s, d_pos,
+                       d_mass, np, np_pad, c->src64.as<double4>());
+    GH_TRY(check_launch(c, "k_pack_src64"));
+    OrderBufs& t = c->ord_t;
+    GH_TRY(spatial_order(c, t, tgt, nt, s));
+    GH_TRY(gather_frames(c, t, tgt, nullptr, t.perm(), nt, nt_pad, group, nullptr, true, s));
+    return direct_f32_onesided(c, c->src64.as<double4>(), np_pad, t, nt, sh, eps, d_acc, s);
 }
 
 }  // namespace gh

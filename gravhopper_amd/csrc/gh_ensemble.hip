@@ -2,8 +2,10 @@
 // together, one force launch per step for all of them.
 //
 // Layout: the simulations' particles are concatenated, each simulation starting at a
-// multiple of the block size BLK = 256*IPT (padding particles are massless, at the
-// origin, and never updated).  The work list holds every block pair (I, J), I <= J,
+// multiple of the block size BLK = 256*IPT (padding particles are massless and never
+// updated).  Every step sorts each simulation along a Morton curve in its own box
+// (spatial_order_seg) and gathers the FP32 local frames of the blocks, exactly as a
+// single run does.  The work list holds every block pair (I, J), I <= J,
 // of every simulation, so one launch of the pair-once kernel covers the whole ensemble;
 // all items are BLK x BLK pairs, so the load is balanced whatever the mix of N.  With
 // `persistent` the launch is a fixed grid of workgroups that take items from an atomic
@@ -11,8 +13,9 @@
 // a simulation's result does not depend on the others: it is bit-identical to running
 // that simulation alone through the pair-once kernel with the same IPT.
 //
-// A step is four kernels -- zero the queue counter, forces, slot reduction, fused
-// kick/drift/pack -- which can be captured once in a HIP graph and replayed.
+// A step -- order, frames, zero the queue counter, forces (and their double-single
+// instance), slot reduction, fused kick/drift -- can be captured once in a HIP graph
+// and replayed.
 #include <algorithm>
 #include <cstring>
 
@@ -26,7 +29,9 @@ struct gh_ensemble {
     int nb_max = 0;
     unsigned grid = 0;
     std::vector<int64_t> start, count;
-    gh::DevBuf x, v, xh, m, acc, src, part, work, eps_blk, nb_blk, valid_blk, dt_blk, queue;
+    gh::DevBuf x, v, xh, m, acc, part, work, eps_blk, nb_blk, valid_blk, dt_blk, queue;
+    gh::DevBuf seg_blk, seg_start, seg_count;  // simulation of each block, ranges
+    gh::OrderBufs ord;                           // per-step order and frames
     bool guard = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -35,36 +40,28 @@ struct gh_ensemble {
 namespace gh {
 
 __global__ __launch_bounds__(kBlock) void k_ens_half_drift(
-    const double* __restrict__ x, const double* __restrict__ v, const double* __restrict__ m,
-    int64_t n_pad, int blk, const int* __restrict__ valid_blk, const double* __restrict__ dt_blk,
-    double* __restrict__ xh, float4* __restrict__ src)
+    const double* __restrict__ x, const double* __restrict__ v, int64_t n_pad, int blk,
+    const int* __restrict__ valid_blk, const double* __restrict__ dt_blk, double* __restrict__ xh)
 {
     const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (p >= n_pad) return;
     const int64_t b = p / blk;
-    if ((int)(p - b * blk) >= valid_blk[b]) return;  // padding: stays massless at 0
+    if ((int)(p - b * blk) >= valid_blk[b]) return;  // padding: never updated
     const double dt = dt_blk[b];
-    double h[3];
-    for (int k = 0; k < 3; k++) {
-        h[k] = x[3 * p + k] + 0.5 * v[3 * p + k] * dt * kDriftScale;
-        xh[3 * p + k] = h[k];
-    }
-    src[p] = make_float4((float)h[0], (float)h[1], (float)h[2], (float)m[p]);
+    for (int k = 0; k < 3; k++) xh[3 * p + k] = x[3 * p + k] + 0.5 * v[3 * p + k] * dt * kDriftScale;
 }
 
-// v' = v + (a A + 0) dt; x' = xh + ((0.5 v') dt) D; xh' = x' + ((0.5 v') dt) D; the
-// next step's FP32 sources are packed from xh'.
+// v' = v + (a A + 0) dt; x' = xh + ((0.5 v') dt) D; xh' = x' + ((0.5 v') dt) D
 __global__ __launch_bounds__(kBlock) void k_ens_kick_drift(
-    const double* __restrict__ acc, const double* __restrict__ m, int64_t n_pad, int blk,
-    const int* __restrict__ valid_blk, const double* __restrict__ dt_blk, double* __restrict__ xh,
-    double* __restrict__ v, double* __restrict__ x, float4* __restrict__ src)
+    const double* __restrict__ acc, int64_t n_pad, int blk, const int* __restrict__ valid_blk,
+    const double* __restrict__ dt_blk, double* __restrict__ xh, double* __restrict__ v,
+    double* __restrict__ x)
 {
     const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (p >= n_pad) return;
     const int64_t b = p / blk;
     if ((int)(p - b * blk) >= valid_blk[b]) return;
     const double dt = dt_blk[b];
-    double h[3];
     for (int k = 0; k < 3; k++) {
         const double a = acc[3 * p + k] * kAccelScale + 0.0;
         const double vn = v[3 * p + k] + a * dt;
@@ -72,10 +69,8 @@ __global__ __launch_bounds__(kBlock) void k_ens_kick_drift(
         const double xn = xh[3 * p + k] + inc;
         v[3 * p + k] = vn;
         x[3 * p + k] = xn;
-        h[k] = xn + inc;
-        xh[3 * p + k] = h[k];
+        xh[3 * p + k] = xn + inc;
     }
-    src[p] = make_float4((float)h[0], (float)h[1], (float)h[2], (float)m[p]);
 }
 
 __global__ void k_zero_int(int* __restrict__ p)
@@ -86,23 +81,31 @@ __global__ void k_zero_int(int* __restrict__ p)
 static int ens_step(gh_ensemble* e, hipStream_t s)
 {
     gh_ctx* c = e->c;
+    OrderBufs& o = e->ord;
+    GH_TRY(spatial_order_seg(c, o, e->xh.as<double>(), e->n_pad, e->blk, e->nsims,
+                             e->seg_blk.as<int>(), e->seg_start.as<int64_t>(),
+                             e->seg_count.as<int64_t>(), s));
+    GH_TRY(gather_frames(c, o, e->xh.as<double>(), e->m.as<double>(), o.perm(), e->n_pad,
+                         e->n_pad, e->blk, e->valid_blk.as<int>(), true, s));
     if (e->persistent) {
         hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, e->queue.as<int>());
         GH_TRY(check_launch(c, "k_zero_int"));
     }
     {
         TimedScope ts(c, s, GH_T_DIRECT_F32);
-        GH_TRY(launch_sym(c, e->ipt, e->guard, e->src.as<float4>(), e->n_pad, e->work.as<int4>(),
+        GH_TRY(launch_sym(c, e->ipt, e->guard, o.loc.as<float4>(), o.s64.as<double4>(),
+                          o.frame.as<double4>(), o.flag.as<int>(), o.nflag.as<int>(), e->n_pad,
+                          e->work.as<int4>(),
                           e->items, e->eps_blk.as<float>(), e->part.as<float>(),
                           e->persistent ? e->queue.as<int>() : nullptr, e->grid, s));
     }
     GH_TRY(reduce_sym(c, e->part.as<float>(), 0, e->nb_blk.as<int>(), e->blk, e->n_pad,
-                      e->n_pad, e->acc.as<double>(), s));
+                      e->n_pad, o.perm(), e->acc.as<double>(), s));
     TimedScope tu(c, s, GH_T_UPDATE);
     hipLaunchKernelGGL(k_ens_kick_drift, dim3(blocks_for(e->n_pad, kBlock)), dim3(kBlock), 0, s,
-                       e->acc.as<double>(), e->m.as<double>(), e->n_pad, e->blk,
-                       e->valid_blk.as<int>(), e->dt_blk.as<double>(), e->xh.as<double>(),
-                       e->v.as<double>(), e->x.as<double>(), e->src.as<float4>());
+                       e->acc.as<double>(), e->n_pad, e->blk, e->valid_blk.as<int>(),
+                       e->dt_blk.as<double>(), e->xh.as<double>(), e->v.as<double>(),
+                       e->x.as<double>());
     return check_launch(c, "k_ens_kick_drift");
 }
 
@@ -110,11 +113,15 @@ static void ens_free(gh_ensemble* e)
 {
     if (e->exec) (void)hipGraphExecDestroy(e->exec);
     if (e->graph) (void)hipGraphDestroy(e->graph);
-    DevBuf* bufs[] = {&e->x,    &e->v,       &e->xh,     &e->m,         &e->acc,
-                      &e->src,  &e->part,    &e->work,   &e->eps_blk,   &e->nb_blk,
-                      &e->valid_blk, &e->dt_blk, &e->queue};
-    for (DevBuf* b : bufs)
-        if (b->p) (void)hipFree(b->p);
+    gh_ctx* c = e->c;
+    OrderBufs& o = e->ord;
+    DevBuf* bufs[] = {&e->x,         &e->v,         &e->xh,       &e->m,         &e->acc,
+                      &e->part,      &e->work,      &e->eps_blk,  &e->nb_blk,    &e->valid_blk,
+                      &e->dt_blk,    &e->queue,     &e->seg_blk,  &e->seg_start, &e->seg_count,
+                      &o.keys[0],    &o.keys[1],    &o.idx[0],    &o.idx[1],     &o.hist,
+                      &o.box,        &o.boxpart,    &o.loc,       &o.s64,        &o.frame,
+                      &o.flag,       &o.nflag};
+    for (DevBuf* b : bufs) release(c, *b);
 }
 
 }  // namespace gh
@@ -141,7 +148,7 @@ int gh_ensemble_create(gh_ctx* c, int n_sims, const int64_t* counts, const doubl
     e->blk = kBlock * ipt;
     e->persistent = persistent != 0;
     int64_t total = 0, pad = 0;
-    std::vector<int> nb_blk, valid_blk;
+    std::vector<int> nb_blk, valid_blk, seg_blk;
     std::vector<float> eps_blk;
     std::vector<double> dt_blk;
     std::vector<int4> work;
@@ -157,13 +164,14 @@ int gh_ensemble_create(gh_ctx* c, int n_sims, const int64_t* counts, const doubl
         const int64_t b0 = pad / e->blk;
         for (int64_t I = 0; I < nb; I++) {
             nb_blk.push_back((int)nb);
+            seg_blk.push_back(s);
             valid_blk.push_back((int)std::min<int64_t>(e->blk, n - I * e->blk));
             eps_blk.push_back((float)(eps[s] * eps[s]));
             dt_blk.push_back(dt[s]);
             for (int64_t J = I; J < nb; J++)
                 work.push_back(make_int4((int)(b0 + I), (int)(b0 + J), (int)J, (int)I));
         }
-        if (!((float)(eps[s] * eps[s]) >= 1e-30f)) e->guard = true;
+        if (fp32_guard(eps[s])) e->guard = true;
         e->nb_max = std::max<int>(e->nb_max, (int)nb);
         pad += nb * e->blk;
         total += n;
@@ -182,7 +190,9 @@ int gh_ensemble_create(gh_ctx* c, int n_sims, const int64_t* counts, const doubl
     al(e->xh, 24 * np);
     al(e->m, 8 * np);
     al(e->acc, 24 * np);
-    al(e->src, 16 * np);
+    al(e->seg_blk, sizeof(int) * seg_blk.size());
+    al(e->seg_start, sizeof(int64_t) * (size_t)n_sims);
+    al(e->seg_count, sizeof(int64_t) * (size_t)n_sims);
     al(e->part, 12 * np * (size_t)e->nb_max);
     al(e->work, sizeof(int4) * work.size());
     al(e->eps_blk, sizeof(float) * eps_blk.size());
@@ -221,13 +231,18 @@ int gh_ensemble_create(gh_ctx* c, int n_sims, const int64_t* counts, const doubl
     up(e->nb_blk, nb_blk.data(), sizeof(int) * nb_blk.size());
     up(e->valid_blk, valid_blk.data(), sizeof(int) * valid_blk.size());
     up(e->dt_blk, dt_blk.data(), sizeof(double) * dt_blk.size());
+    up(e->seg_blk, seg_blk.data(), sizeof(int) * seg_blk.size());
+    up(e->seg_start, e->start.data(), sizeof(int64_t) * (size_t)n_sims);
+    up(e->seg_count, e->count.data(), sizeof(int64_t) * (size_t)n_sims);
+    // size the order and frame buffers now, so that a captured step allocates nothing
+    if (!rc)
+        rc = spatial_order_seg(c, e->ord, e->xh.as<double>(), e->n_pad, e->blk, e->nsims,
+                               e->seg_blk.as<int>(), e->seg_start.as<int64_t>(),
+                               e->seg_count.as<int64_t>(), s);
+    if (!rc)
+        rc = gather_frames(c, e->ord, e->xh.as<double>(), e->m.as<double>(), e->ord.perm(),
+                           e->n_pad, e->n_pad, e->blk, e->valid_blk.as<int>(), true, s);
     if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = fail(c, GH_EHIP, "ensemble upload");
-    if (!rc) {
-        // padding sources: massless at the origin
-        std::vector<float4> z(np, make_float4(0.f, 0.f, 0.f, 0.f));
-        up(e->src, z.data(), 16 * np);
-        if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = fail(c, GH_EHIP, "upload");
-    }
     if (rc) {
         ens_free(e);
         delete e;
@@ -246,9 +261,8 @@ int gh_ensemble_run(gh_ensemble* e, int64_t nsteps, int use_graph)
     GH_HIP(c, hipSetDevice(c->device));
     hipStream_t s = c->stream;
     hipLaunchKernelGGL(k_ens_half_drift, dim3(blocks_for(e->n_pad, kBlock)), dim3(kBlock), 0, s,
-                       e->x.as<double>(), e->v.as<double>(), e->m.as<double>(), e->n_pad, e->blk,
-                       e->valid_blk.as<int>(), e->dt_blk.as<double>(), e->xh.as<double>(),
-                       e->src.as<float4>());
+                       e->x.as<double>(), e->v.as<double>(), e->n_pad, e->blk,
+                       e->valid_blk.as<int>(), e->dt_blk.as<double>(), e->xh.as<double>());
     GH_TRY(check_launch(c, "k_ens_half_drift"));
     if (!use_graph || c->timing) {
         for (int64_t k = 0; k < nsteps; k++) GH_TRY(ens_step(e, s));

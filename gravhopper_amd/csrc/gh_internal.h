@@ -31,6 +31,25 @@ struct DevBuf {
 
 struct TreeBufs;  // gh_tree.hip
 
+typedef unsigned long long u64;
+typedef unsigned int u32;
+
+// Spatial order and local frames of the FP32 pair kernels (gh_frames.hip).  Points are
+// sorted along a Morton curve and cut into groups of consecutive sorted points; each
+// group has an FP64 origin (one of its points), and FP32 coordinates are only ever
+// formed relative to such an origin, so their rounding error scales with the group's
+// extent rather than with the distance from the coordinate origin.
+struct OrderBufs {
+    DevBuf keys[2], idx[2], hist, box, boxpart;  // the sort
+    DevBuf loc;    // float4 (x - o, y - o, z - o, m) per sorted point, o its group's origin
+    DevBuf s64;    // double4 (x, y, z, m) per sorted point
+    DevBuf frame;  // double4 per group: origin (x, y, z), largest |x - o| (max norm)
+    DevBuf flag;   // int per group: 1 if the group is not compact (see k_frames)
+    DevBuf nflag;  // int: number of flagged groups (cleared by the key kernels)
+    int sel = 0;   // which of idx[0] / idx[1] holds the order
+    const u32* perm() const { return idx[sel].as<u32>(); }
+};
+
 }  // namespace gh
 
 struct gh_ctx {
@@ -47,10 +66,12 @@ struct gh_ctx {
     gh::DevBuf work;                // block-pair list of the pair-once direct kernel
     gh::DevBuf eps_blk;             // its per-block squared softening
     int sym_nb = -1, sym_ipt = -1;
+    float sym_eps2 = -1.f;          // the softening eps_blk holds
     // adjoint workspaces (gh_adjoint.hip)
     gh::DevBuf adj_work, adj_src, adj_mu, adj_mv, adj_part, adj_seg;
     int adj_nb = -1, adj_ipt = -1;
     gh::DevBuf slot;  // history slot counter of graph-replayed runs
+    gh::OrderBufs ord_p, ord_t;  // spatial order of sources and of targets (FP32 paths)
     // gh_run_ext: copy stream, pinned host staging (x_half, v, external acceleration)
     // and the device copy of the external acceleration
     hipStream_t copy_stream = nullptr;
@@ -91,6 +112,8 @@ int fail(gh_ctx* c, int code, const char* fmt, ...);
     } while (0)
 
 int ensure(gh_ctx* c, DevBuf& b, size_t bytes);
+// Frees an arena buffer and takes its capacity off the context's arena total.
+void release(gh_ctx* c, DevBuf& b);
 
 // Brackets the kernels launched between construction and destruction with HIP events
 // of class `cls` on stream s when timing is enabled on the context.
@@ -120,6 +143,12 @@ __device__ __forceinline__ float in_vgpr(float x)
     return r;
 }
 
+// The zero-separation guard of the FP32 kernels: a pair's scale factor m r^-3 made
+// finite (+inf and NaN become FLT_MAX), so that a zero separation (the self pair, a
+// coincident pair, a massless padding particle) contributes exactly 0 instead of
+// 0 * inf.  Applied when fp32_guard(eps) holds.
+__device__ __forceinline__ float clamp_finite(float x) { return fminf(x, 3.402823466e38f); }
+
 inline unsigned blocks_for(int64_t n, int per_block) {
     return static_cast<unsigned>((n + per_block - 1) / per_block);
 }
@@ -128,19 +157,22 @@ inline unsigned blocks_for(int64_t n, int per_block) {
 int direct_dev(gh_ctx* c, const double* d_pos, const double* d_mass, int64_t np,
                const double* d_fpos, int64_t nf, double eps, int precision, double* d_acc,
                hipStream_t s);
-int pack_src32(gh_ctx* c, const double* d_pos, const double* d_mass, int64_t np,
-               float4* out, int64_t np_pad, hipStream_t s);
 int64_t pad_to(int64_t n, int64_t m);
 int sym_ipt(gh_ctx* c, int64_t np);
+// true when the FP32 pair kernels need the zero-separation guard for this softening
+bool fp32_guard(double eps);
 int direct_sym(gh_ctx* c, const double* d_pos, const double* d_mass, int64_t np, int ipt,
                double eps, double* d_acc, hipStream_t s);
-int launch_sym(gh_ctx* c, int ipt, bool guard, const float4* src, int64_t np_pad,
+// The pair-once kernel over a block-pair list (src: local FP32 coordinates per sorted
+// particle, s64: the same particles in FP64, frame/flag: per block), followed by its
+// double-single instance for the items with a flagged block.
+int launch_sym(gh_ctx* c, int ipt, bool guard, const float4* src, const double4* s64,
+               const double4* frame, const int* flag, const int* nflag, int64_t np_pad,
                const int4* work, int64_t items, const float* eps2_blk, float* part, int* queue,
                unsigned persistent_grid, hipStream_t s);
+// acc[perm[p]] (perm nullable) = the FP64 sum of particle p's partial slots
 int reduce_sym(gh_ctx* c, const float* part, int nb, const int* nb_blk, int blk,
-               int64_t np_pad, int64_t n, double* acc, hipStream_t s);
-int direct_f32_packed(gh_ctx* c, const float4* src, int64_t np_pad, const float4* tgt,
-                      int64_t nt, bool self, double eps, double* d_acc, hipStream_t s);
+               int64_t np_pad, int64_t n, const u32* perm, double* acc, hipStream_t s);
 
 // leapfrog update (gh_step.hip).  kick_drift: ext (nullable) is an extra acceleration
 // in km/s/Myr added to the N-body one, as a = acc*A + ext; slot (nullable) is the
@@ -152,6 +184,33 @@ int kick_drift(gh_ctx* c, const double* acc, int64_t n, double dt, double* xh, d
                double* x, double* hp, double* hv, hipStream_t s,
                const long long* slot = nullptr, const double* ext = nullptr,
                long long* slot_next = nullptr);
+
+// spatial order and frames (gh_frames.hip)
+constexpr int kFrameLog2 = 10;  // a group is flagged when R > 2^kFrameLog2 x its mean gap
+// Sorts n points (pos (n,3)) along a Morton curve of 16 levels in their root box; the
+// order is o.perm().
+int spatial_order(gh_ctx* c, OrderBufs& o, const double* d_pos, int64_t n, hipStream_t s);
+// The same order for a padded layout of nseg segments (segment g: positions
+// [start[g], start[g] + count[g]) of pos (n_pad,3), blocks of blk positions, seg[b] the
+// segment of block b, all device arrays): each segment sorted in its own box exactly as
+// spatial_order sorts it alone, segments in place, padding last in its segment.
+int spatial_order_seg(gh_ctx* c, OrderBufs& o, const double* d_pos, int64_t n_pad, int blk,
+                      int nseg, const int* seg, const int64_t* start, const int64_t* count,
+                      hipStream_t s);
+// Fills o.loc, o.s64 (nullable output: skipped if want_s64 is false), o.frame and
+// o.flag for n_pad / group points in groups of `group` (256, 512 or 1024) consecutive
+// sorted points; perm == nullptr means the points are already in order.  valid
+// (nullable) gives each group's count of real points; otherwise groups are full up to
+// n.  Padding entries are massless at their group's origin.  mass may be null (w = 0).
+int gather_frames(gh_ctx* c, OrderBufs& o, const double* d_pos, const double* d_mass,
+                  const u32* perm, int64_t n, int64_t n_pad, int group, const int* valid,
+                  bool want_s64, hipStream_t s);
+
+// tree helpers shared with the spatial order (gh_tree.hip)
+int radix_sort(gh_ctx* c, DevBuf* keys, DevBuf* idx, DevBuf& hist, int64_t n, int words,
+               int lo_shift, hipStream_t s, int* sel_out, int hi_shift = 64);
+int bbox_dev(gh_ctx* c, const double* d_pos, int64_t n, double eps, DevBuf& part, double* box,
+             hipStream_t s);
 
 // tree (gh_tree.hip)
 int tree_dev(gh_ctx* c, const double* d_pos, const double* d_mass, int64_t np,

@@ -887,7 +887,7 @@ __device__ __forceinline__ void walk_unit_f32(const char* __restrict__ nodes,
                 r2 = __builtin_fmaf(dz, dz, r2);
                 const float ri = __builtin_amdgcn_rsqf(r2);
                 float sc = (nd[w][7] * ri) * (ri * ri);
-                if (GUARD) sc = (r2 > 0.f) ? sc : 0.f;
+                if (GUARD) sc = clamp_finite(sc);
                 fx[w] = __builtin_fmaf(dx, sc, fx[w]);
                 fy[w] = __builtin_fmaf(dy, sc, fy[w]);
                 fz[w] = __builtin_fmaf(dz, sc, fz[w]);
@@ -1072,16 +1072,17 @@ __global__ __launch_bounds__(kBlock) void k_walk_f64(const NodeD* __restrict__ n
 // ------------------------------------------------------------------------------
 // host orchestration
 // ------------------------------------------------------------------------------
-// Stable LSD sort of (keys[0], idx[0]) by the key bits [lo_shift, 64) of every word
-// (word-major, `words` words); returns which of the two buffer pairs holds the result.
-static int radix_sort(gh_ctx* c, DevBuf* keys, DevBuf* idx, DevBuf& hist, int64_t n, int words,
-                      int lo_shift, hipStream_t s, int* sel_out)
+// Stable LSD sort of (keys[0], idx[0]) by the key bits [lo_shift, hi_shift) of every
+// word (word-major, `words` words); returns which of the two buffer pairs holds the
+// result.
+int radix_sort(gh_ctx* c, DevBuf* keys, DevBuf* idx, DevBuf& hist, int64_t n, int words,
+               int lo_shift, hipStream_t s, int* sel_out, int hi_shift)
 {
     const int nblocks = (int)((n + kRsChunk - 1) / kRsChunk);
     GH_TRY(ensure(c, hist, sizeof(u32) * 256 * (size_t)(nblocks + 1)));
     int sel = 0;
     for (int w = words - 1; w >= 0; w--) {
-        for (int shift = lo_shift; shift < 64; shift += 8) {
+        for (int shift = lo_shift; shift < hi_shift; shift += 8) {
             const u64* kin = keys[sel].as<u64>();
             hipLaunchKernelGGL(k_rs_hist, dim3(nblocks), dim3(kBlock), 0, s, kin + (int64_t)w * n,
                                n, shift, hist.as<u32>(), nblocks);
@@ -1107,6 +1108,21 @@ static int radix_sort(gh_ctx* c, DevBuf* keys, DevBuf* idx, DevBuf& hist, int64_
     return 0;
 }
 
+// Root box of the reference (centre box[0..2], size box[3], gravhopper/_jbgrav.c:747-772)
+// of n points, written to the device array box; part holds the partial extents.
+int bbox_dev(gh_ctx* c, const double* d_pos, int64_t n, double eps, DevBuf& part, double* box,
+             hipStream_t s)
+{
+    const int nparts = (int)std::min<int64_t>(1024, (n + kBlock - 1) / kBlock);
+    GH_TRY(ensure(c, part, sizeof(double) * 6 * (size_t)nparts));
+    hipLaunchKernelGGL(k_bbox_partial, dim3(nparts), dim3(kBlock), 0, s, d_pos, n,
+                       part.as<double>());
+    GH_TRY(check_launch(c, "k_bbox_partial"));
+    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(kBlock), 0, s, part.as<double>(), nparts,
+                       eps, box);
+    return check_launch(c, "k_bbox_final");
+}
+
 // Builds the tree of (d_pos, d_mass) into c->tree.  One host synchronisation reads the
 // node count and the key checks.
 static int build_tree(gh_ctx* c, const double* d_pos, const double* d_mass, int64_t n,
@@ -1117,15 +1133,8 @@ static int build_tree(gh_ctx* c, const double* d_pos, const double* d_mass, int6
     TimedScope ts(c, s, GH_T_TREE_BUILD);
 
     // 1. box
-    const int nparts = (int)std::min<int64_t>(1024, (n + kBlock - 1) / kBlock);
-    GH_TRY(ensure(c, tb->bbox_part, sizeof(double) * 6 * (size_t)nparts));
     GH_TRY(ensure(c, tb->box, sizeof(double) * 8));
-    hipLaunchKernelGGL(k_bbox_partial, dim3(nparts), dim3(kBlock), 0, s, d_pos, n,
-                       tb->bbox_part.as<double>());
-    GH_TRY(check_launch(c, "k_bbox_partial"));
-    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(kBlock), 0, s, tb->bbox_part.as<double>(), nparts,
-                       eps, tb->box.as<double>());
-    GH_TRY(check_launch(c, "k_bbox_final"));
+    GH_TRY(bbox_dev(c, d_pos, n, eps, tb->bbox_part, tb->box.as<double>(), s));
 
     GH_TRY(ensure(c, tb->idx[0], sizeof(u32) * (size_t)n));
     GH_TRY(ensure(c, tb->idx[1], sizeof(u32) * (size_t)n));
@@ -1300,7 +1309,7 @@ static int walk(gh_ctx* c, const double* tgt, bool self, int64_t nt, double eps,
         return check_launch(c, "k_walk_f64");
     }
     const float eps2 = (float)(eps * eps);
-    const bool guard = !(eps2 >= 1e-30f);
+    const bool guard = fp32_guard(eps);
     // node offsets are 32-bit byte offsets inside the kernel (records below 4 GiB)
     const int64_t max_nodes = ((int64_t)1 << 32) / (int64_t)sizeof(NodeF);
     if (tb->n_nodes >= max_nodes)
@@ -1375,8 +1384,7 @@ void tree_free(gh_ctx* c)
                       &tb->nodes64, &tb->nodes32,
                       &tb->skip,    &tb->level,   &tb->branch,    &tb->leaf,    &tb->tkeys[0],
                       &tb->tkeys[1], &tb->tidx[0], &tb->tidx[1],  &tb->walk_queue};
-    for (DevBuf* b : bufs)
-        if (b->p) (void)hipFree(b->p);
+    for (DevBuf* b : bufs) release(c, *b);
     if (tb->readback) (void)hipHostFree(tb->readback);
     delete tb;
     c->tree = nullptr;

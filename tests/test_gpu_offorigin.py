@@ -1,0 +1,195 @@
+"""FP32 paths away from the coordinate origin.
+
+The reference forms every pair separation in FP64 (gravhopper/_jbgrav.c:155-168), so
+its forces do not depend on where a system sits.  The FP32 kernels must keep that
+property: they form coordinates only relative to FP64 origins of groups of spatially
+sorted particles.  Cases: the config-2 Plummer sphere (b = 1 pc, eps = 0.01 pc) moved to
+(8, 0, 0) kpc and to (100, 0, 0) kpc, and two such spheres 100 kpc apart (whose sorted
+blocks straddle the two spheres, so the double-single instances run too), in input
+order and shuffled.  Tolerance against the FP64 oracle: median relative force error
+<= 1e-6 and p99 <= 1e-4, as for centred inputs.
+"""
+import numpy as np
+import pytest
+
+from conftest import assert_bitwise, rel_err
+from gravhopper_amd import _native, workloads
+
+pytestmark = pytest.mark.gpu
+
+FP32, FP64 = _native.GH_FP32, _native.GH_FP64
+EPS = 1e-5
+MED, P99 = 1e-6, 1e-4
+
+
+def sphere(n, seed, shift=(0.0, 0.0, 0.0)):
+    ic = workloads.plummer(n, b=1e-3, mass=1e6, seed=seed)
+    return ic["pos"] + np.asarray(shift, dtype=np.float64), ic["mass"]
+
+
+def case(name, n, seed=11):
+    if name == "shift8":
+        return sphere(n, seed, (8.0, 0.0, 0.0))
+    if name == "shift100":
+        return sphere(n, seed, (100.0, 0.0, 0.0))
+    pa, ma = sphere(n // 2, seed, (-50.0, 0.0, 0.0))
+    pb, mb = sphere(n - n // 2, seed + 1, (50.0, 0.0, 0.0))
+    pos, mass = np.vstack([pa, pb]), np.concatenate([ma, mb])
+    if name == "two_spheres_shuffled":
+        idx = np.random.default_rng(seed).permutation(n)
+        pos, mass = pos[idx], mass[idx]
+    return pos, mass
+
+
+CASES = ["shift8", "shift100", "two_spheres", "two_spheres_shuffled"]
+
+
+def check(got, want, what):
+    err = rel_err(got, want)
+    med, p99 = np.median(err), np.percentile(err, 99)
+    print("%s: median %.2e p99 %.2e" % (what, med, p99), flush=True)
+    assert med <= MED and p99 <= P99, what
+
+
+@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("ipt", [1, 2, 4])
+def test_pair_once_off_origin(ctx, oracle, name, ipt):
+    pos, mass = case(name, 20000)
+    want = oracle.direct(pos, mass, EPS)
+    ctx.set_tuning(_native.TUNE_DIRECT_SELF, ipt)
+    try:
+        got = ctx.direct(pos, mass, EPS, FP32)
+    finally:
+        ctx.set_tuning(_native.TUNE_DIRECT_SELF, 0)
+    check(got, want, "pair-once ipt %d %s" % (ipt, name))
+
+
+@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("ipt", [1, 2, 4])
+def test_one_sided_self_off_origin(ctx, oracle, name, ipt):
+    pos, mass = case(name, 5000)
+    want = oracle.direct(pos, mass, EPS)
+    ctx.set_tuning(_native.TUNE_DIRECT_SELF, -1)
+    ctx.set_tuning(_native.TUNE_DIRECT_IPT, ipt)
+    try:
+        got = ctx.direct(pos, mass, EPS, FP32)
+    finally:
+        ctx.set_tuning(_native.TUNE_DIRECT_SELF, 0)
+        ctx.set_tuning(_native.TUNE_DIRECT_IPT, 0)
+    check(got, want, "one-sided ipt %d %s" % (ipt, name))
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_position_targets_off_origin(ctx, oracle, name):
+    pos, mass = case(name, 6000)
+    # targets: an independent draw of the same system (typical separations from the
+    # sources), a tenth of the particles themselves, and a few far points
+    tgt, _ = case(name, 3000, seed=12)
+    rng = np.random.default_rng(4)
+    tgt = np.vstack([tgt, pos[rng.choice(len(mass), 300, replace=False)],
+                     pos.mean(axis=0) + rng.normal(scale=0.2, size=(40, 3))])
+    want = oracle.direct_position(pos, mass, tgt, EPS)
+    got = ctx.direct(pos, mass, EPS, FP32, force_pos=tgt)
+    check(got, want, "position targets %s" % name)
+
+
+def test_position_targets_near_sources_limit(ctx, oracle):
+    # Targets 0.3 eps from a source: that one pair dominates the field, and FP32
+    # coordinates in a frame of extent R carry 2^-24 R of rounding, so the pair's
+    # relative error is about 2^-24 R / 0.3 eps wherever the system sits.  Measured:
+    # median 7e-6, p99 1.5e-4 for the config-2 sphere, centred or moved (a stated limit).
+    for shift in ((0.0, 0.0, 0.0), (100.0, 0.0, 0.0)):
+        pos, mass = sphere(6000, 11, shift)
+        rng = np.random.default_rng(4)
+        tgt = pos[rng.choice(6000, 3000, replace=False)] + rng.normal(scale=0.3 * EPS,
+                                                                      size=(3000, 3))
+        err = rel_err(ctx.direct(pos, mass, EPS, FP32, force_pos=tgt),
+                      oracle.direct_position(pos, mass, tgt, EPS))
+        print("near sources, shift %s: median %.2e p99 %.2e"
+              % (shift, np.median(err), np.percentile(err, 99)), flush=True)
+        assert np.median(err) < 2e-5 and np.percentile(err, 99) < 5e-4, shift
+
+
+def test_translation_changes_fp32_forces_by_rounding_only(ctx):
+    pos, mass = sphere(20000, 21)
+    a0 = ctx.direct(pos, mass, EPS, FP32)
+    for shift in ((8.0, 0.0, 0.0), (0.0, -100.0, 30.0)):
+        a1 = ctx.direct(pos + np.asarray(shift), mass, EPS, FP32)
+        err = rel_err(a1, a0)
+        assert np.median(err) < 3e-7 and np.percentile(err, 99) < 3e-5, shift
+
+
+def test_coincident_guard_small_softening(ctx, oracle):
+    # eps = 1e-14 kpc: m / eps^3 overflows FP32, so the self pair and coincident pairs
+    # need the zero-separation guard; the result must stay finite like the reference's
+    pos, mass = sphere(3000, 31)
+    pos[1] = pos[0]
+    for tune in (-1, 1):
+        ctx.set_tuning(_native.TUNE_DIRECT_SELF, tune)
+        try:
+            got = ctx.direct(pos, mass, 1e-14, FP32)
+        finally:
+            ctx.set_tuning(_native.TUNE_DIRECT_SELF, 0)
+        assert np.all(np.isfinite(got)), tune
+        want = oracle.direct_position(pos, mass, pos, 1e-14)
+        err = rel_err(got, want)
+        assert np.median(err) < 1e-5, tune
+
+
+def shifted_ensemble(seed=9, n_sims=5):
+    rng = np.random.default_rng(seed)
+    sims = []
+    for k in range(n_sims):
+        n = int(rng.integers(500, 4000))
+        b = rng.uniform(0.5e-3, 2e-3)
+        m = 10 ** rng.uniform(5, 7)
+        ic = workloads.plummer(n, b=b, mass=m, seed=seed * 100 + k)
+        ic["pos"] = ic["pos"] + np.array([8.0 + 40.0 * k, -20.0 * k, 100.0 * (k % 2)])
+        ic["eps"] = 0.02 * b
+        ic["dt"] = 0.005 * workloads.plummer_crossing_time(b, m)
+        sims.append(ic)
+    return sims
+
+
+def test_ensemble_off_origin(ctx, oracle):
+    sims = shifted_ensemble()
+    ens = _native.Ensemble(ctx, sims, ipt=1)
+    ens.run(3, graph=True)
+    pos, _ = ens.state()
+    ens.close()
+    ctx.set_tuning(_native.TUNE_DIRECT_SELF, 1)
+    try:
+        for k, s in enumerate(sims):
+            xo, _ = oracle.dkd_direct(s["pos"], s["vel"], s["mass"], s["dt"], s["eps"], 3)
+            disp = np.linalg.norm(xo - s["pos"], axis=1)
+            err = np.linalg.norm(pos[k] - xo, axis=1)
+            assert np.median(err / disp) < 1e-5, k
+            x, _ = ctx.run(s["pos"], s["vel"], s["mass"], s["dt"], s["eps"], 3, precision=FP32)
+            assert_bitwise(pos[k], x, "sim %d" % k)
+    finally:
+        ctx.set_tuning(_native.TUNE_DIRECT_SELF, 0)
+
+
+def test_adjoint_translation_invariant(ctx):
+    torch = pytest.importorskip("torch")
+    from gravhopper_amd.autograd import leapfrog
+    n, steps, every, dt, eps = 2048, 10, 5, 0.005, 5e-5
+    ic = workloads.plummer(n, b=1e-3, mass=1e6, seed=41)
+    target, _ = ctx.run(ic["pos"], ic["vel"], ic["mass"] * 1.01, dt, eps, steps, precision=FP64)
+    grads = []
+    for shift in ((0.0, 0.0, 0.0), (8.0, 0.0, 0.0), (100.0, -30.0, 0.0)):
+        s = np.asarray(shift)
+        x0 = torch.tensor(ic["pos"] + s, device="cuda", requires_grad=True)
+        v0 = torch.tensor(ic["vel"], device="cuda", requires_grad=True)
+        m = torch.tensor(ic["mass"], device="cuda", requires_grad=True)
+        tgt = torch.tensor(target + s, device="cuda")
+        x, _ = leapfrog(x0, v0, m, dt, eps, steps, checkpoint_every=every, precision="fp32")
+        ((x - tgt) ** 2).sum(-1).mean().backward()
+        grads.append([g.grad.cpu().numpy() for g in (x0, v0, m)])
+    for k in (1, 2):
+        for a, b, what in zip(grads[k], grads[0], ("x0", "v0", "m")):
+            a2 = a.reshape(n, -1)
+            b2 = b.reshape(n, -1)
+            err = np.linalg.norm(a2 - b2, axis=1) / np.maximum(np.linalg.norm(b2, axis=1), 1e-300)
+            print("adjoint shift %d %s: median %.2e" % (k, what, np.median(err)), flush=True)
+            assert np.median(err) < 1e-4, (k, what)
