@@ -49,6 +49,7 @@ SIGNATURES = {
     "gh_tree_force_dev": ([_P, _P, _P, _I64, _P, _I64, _D, _D, _I, _P, _P], _I),
     "gh_tree_dump": ([_P, _P, _P, _I64, _D, _P, _I64, ctypes.POINTER(_I64)], _I),
     "gh_tree_interactions": ([_P, _P, _P, _I64, _P, _I64, _D, _D, _P, _P], _I),
+    "gh_tree_interactions_fp32": ([_P, _P, _P, _I64, _P, _I64, _D, _D, _P, _P], _I),
     "gh_run": ([_P, _P, _P, _P, _I64, _D, _D, _I, _D, _I, _I64, _P, _P], _I),
     "gh_run_dev": ([_P, _P, _P, _P, _I64, _D, _D, _I, _D, _I, _I64, _P, _P, _P], _I),
     "gh_run_ext": ([_P, _P, _P, _P, _I64, _D, _D, _I, _D, _I, _I64, _D, EXTERNAL_FN, _P, _I,
@@ -68,8 +69,8 @@ SIGNATURES = {
 }
 
 # gh_set_tuning keys
-TUNE_DIRECT_SELF, TUNE_DIRECT_IPT, TUNE_WALKS_PER_WAVE, TUNE_RUN_GRAPH = 0, 1, 2, 3
-TUNE_WALK_SCHEDULE, TUNE_ALLOC_LIMIT_MB, TUNE_DIRECT_F64_TPW = 4, 5, 6
+TUNE_DIRECT_SELF, TUNE_DIRECT_IPT, TUNE_WALK_GROUP, TUNE_RUN_GRAPH = 0, 1, 2, 3
+TUNE_WALK_F64EVAL, TUNE_ALLOC_LIMIT_MB, TUNE_DIRECT_F64_TPW = 4, 5, 6
 
 # kernel classes of gh_timing_read
 T_DIRECT_F32, T_DIRECT_F64, T_TREE_BUILD, T_TREE_WALK, T_UPDATE, T_VJP = range(6)
@@ -212,15 +213,18 @@ class Context:
                                          float(eps), _ptr(out), n.value, ctypes.byref(n)))
         return out
 
-    def tree_interactions(self, pos, mass, eps, theta=0.7, force_pos=None):
-        """FP64 tree force and accepted-node count per target."""
+    def tree_interactions(self, pos, mass, eps, theta=0.7, force_pos=None, precision=GH_FP64):
+        """Tree force and interaction count per target: FP64, the reference's accepted
+        nodes; FP32, the group walk's list lengths."""
         pos = np.ascontiguousarray(pos, dtype=np.float64)
         mass = np.ascontiguousarray(mass, dtype=np.float64)
         fpos = None if force_pos is None else np.ascontiguousarray(force_pos, dtype=np.float64)
         nt = len(mass) if fpos is None else len(fpos)
         out = np.empty((nt, 3))
         cnt = np.empty(nt, dtype=np.int64)
-        self.check(self.lib.gh_tree_interactions(
+        fn = (self.lib.gh_tree_interactions if precision == GH_FP64
+              else self.lib.gh_tree_interactions_fp32)
+        self.check(fn(
             self.handle, _ptr(pos), _ptr(mass), len(mass), _ptr(fpos), nt if fpos is not None
             else 0, float(eps), float(theta), _ptr(out), _ptr(cnt)))
         return out, cnt

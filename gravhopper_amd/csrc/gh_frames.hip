@@ -5,7 +5,7 @@
 // star cluster 8 kpc from the coordinate origin is a relative error of order 1e-4 on
 // its internal forces (the reference forms every difference in FP64,
 // gravhopper/_jbgrav.c:155-168).  The FP32 kernels therefore never cast an absolute
-// position.  Points are sorted along a Morton curve (16 levels in their root box) and
+// position.  Points are sorted along a Morton curve (about 11 levels in their root box) and
 // cut into groups of consecutive sorted points (the kernels' blocks); each group g has
 // an FP64 origin o_g, one of its own points, and the kernels work in the frame of one
 // group: a point of group h enters as fl(x - o_h) + fl(o_h - o_g), both formed from FP64
@@ -37,6 +37,9 @@ __device__ __forceinline__ u64 spread3(u64 v)
     v = (v | (v << 2)) & 0x1249249249249249ull;
     return v;
 }
+
+// The sort uses key bits [kKeyShift, 63) of the 63-bit Morton key.
+constexpr int kKeyShift = 31;
 
 __device__ __forceinline__ u32 quantize(double x, double lo, double scale)
 {
@@ -79,8 +82,11 @@ int spatial_order(gh_ctx* c, OrderBufs& o, const double* d_pos, int64_t n, hipSt
                        o.box.as<double>(), o.keys[0].as<u64>(), o.idx[0].as<u32>(),
                        o.nflag.as<int>());
     GH_TRY(check_launch(c, "k_morton"));
-    // bits 15..62: the top 16 levels
-    return radix_sort(c, o.keys, o.idx, o.hist, n, 1, 15, s, &o.sel, 63);
+    // bits 31..62 (the top 10 2/3 levels): four 8-bit passes.  Deeper keys change the
+    // blocks' compactness, and so the forces' rounding, by little (Plummer sphere,
+    // 65536 particles, 1024-particle blocks: median relative error 3.0e-8 with 10 levels,
+    // 3.4e-8 with 16) and cost two more passes.
+    return radix_sort(c, o.keys, o.idx, o.hist, n, 1, kKeyShift, s, &o.sel, 63);
 }
 
 // Root box (as k_bbox_final with eps = 0) of each segment's real points, one workgroup
@@ -124,9 +130,10 @@ __global__ __launch_bounds__(kBlock) void k_seg_box(const double* __restrict__ p
 }
 
 // Keys of the padded, segmented layout: position p of block b belongs to segment
-// seg[b]; its key is (segment << 48) | the top 16 Morton levels in the segment's box
-// (the key spatial_order sorts by), and padding positions take their segment's largest
-// key, so that a stable sort keeps every segment in place with its padding last.
+// seg[b]; its key is (segment << 32) | the key bits spatial_order sorts by (in the
+// segment's box), and padding positions take their segment's largest key, so that a
+// stable sort keeps every segment in place, in the order a single sort of it gives,
+// with its padding last.
 __global__ __launch_bounds__(kBlock) void k_morton_seg(const double* __restrict__ pos,
                                                        int64_t n_pad, int blk,
                                                        const int* __restrict__ seg,
@@ -141,7 +148,7 @@ __global__ __launch_bounds__(kBlock) void k_morton_seg(const double* __restrict_
     if (p == 0) *nflag = 0;
     if (p >= n_pad) return;
     const int g = seg[p / blk];
-    u64 k = ((u64)1 << 48) - 1;
+    u64 k = ((u64)1 << (63 - kKeyShift)) - 1;
     if (p - start[g] < count[g]) {
         const double* bx = box + 4 * g;
         const double size = bx[3];
@@ -149,9 +156,9 @@ __global__ __launch_bounds__(kBlock) void k_morton_seg(const double* __restrict_
         u64 m = 0;
         for (int a = 0; a < 3; a++)
             m |= spread3(quantize(pos[3 * p + a], bx[a] - 0.5 * size, scale)) << a;
-        k = m >> 15;
+        k = m >> kKeyShift;
     }
-    keys[p] = ((u64)g << 48) | k;
+    keys[p] = ((u64)g << (63 - kKeyShift)) | k;
     idx[p] = (u32)p;
 }
 
@@ -173,8 +180,8 @@ int spatial_order_seg(gh_ctx* c, OrderBufs& o, const double* d_pos, int64_t n_pa
                        n_pad, blk, seg, start, count, o.box.as<double>(), o.keys[0].as<u64>(),
                        o.idx[0].as<u32>(), o.nflag.as<int>());
     GH_TRY(check_launch(c, "k_morton_seg"));
-    int bits = 48;
-    while (bits < 64 && ((int64_t)1 << (bits - 48)) < nseg) bits++;
+    int bits = 63 - kKeyShift;
+    while (bits < 64 && ((int64_t)1 << (bits - (63 - kKeyShift))) < nseg) bits++;
     return radix_sort(c, o.keys, o.idx, o.hist, n_pad, 1, 0, s, &o.sel, bits);
 }
 

@@ -187,7 +187,7 @@ int kick_drift(gh_ctx* c, const double* acc, int64_t n, double dt, double* xh, d
 
 // spatial order and frames (gh_frames.hip)
 constexpr int kFrameLog2 = 10;  // a group is flagged when R > 2^kFrameLog2 x its mean gap
-// Sorts n points (pos (n,3)) along a Morton curve of 16 levels in their root box; the
+// Sorts n points (pos (n,3)) along a Morton curve (about 11 levels) in their root box; the
 // order is o.perm().
 int spatial_order(gh_ctx* c, OrderBufs& o, const double* d_pos, int64_t n, hipStream_t s);
 // The same order for a padded layout of nseg segments (segment g: positions

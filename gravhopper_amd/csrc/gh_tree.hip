@@ -59,13 +59,6 @@ struct NodeD {
     double cx, cy, cz, size;
     double mx, my, mz, mass;  // centre of mass, mass
 };
-// FP32 walk record: geo = (centre, (size/theta)^2 or -1 for a leaf), mon = (com, mass)
-struct NodeF {
-    float4 geo;
-    float4 mon;
-};
-typedef float f32x8 __attribute__((ext_vector_type(8)));  // a NodeF as one load
-static_assert(sizeof(NodeF) == sizeof(f32x8), "NodeF is 32 bytes");
 typedef double f64x8 __attribute__((ext_vector_type(8)));  // a NodeD as one load
 static_assert(sizeof(NodeD) == sizeof(f64x8), "NodeD is 64 bytes");
 
@@ -78,9 +71,8 @@ struct TreeBufs {
     DevBuf idx[2];
     DevBuf hist, bbox_part, box, flags, delta, off, prefix, block_sums, block_counts, packed;
     DevBuf spos;  // (x, y, z, m) of the particles in sorted order
-    DevBuf nodes64, nodes32, skip, level, branch, leaf;
+    DevBuf nodes64, kids, skip, level, branch, leaf;
     DevBuf tkeys[2], tidx[2];  // Morton order of position targets (tree_force_position)
-    DevBuf walk_queue;  // 8 unit counters of the dynamically scheduled FP32 walk
     long long* readback = nullptr;  // pinned host words: flags[2], node count
     int64_t n_nodes = 0;
     int words = 0;
@@ -664,9 +656,7 @@ struct FillArgs {
     const double4* spos;  // sorted (x, y, z, m)
     const double* box;
     ScanView<OpDD4> pref;  // moment prefix sums, n+1 entries
-    double theta;
     NodeD* nodes64;
-    NodeF* nodes32;
     int* skip;
     short* level;
     signed char* branch;
@@ -674,8 +664,8 @@ struct FillArgs {
     int what;   // kFill* bits: which records this build needs
 };
 
-constexpr int kFillF64 = 1;   // NodeD + leaf flags (FP64 walk)
-constexpr int kFillF32 = 2;   // NodeF (FP32 walk)
+constexpr int kFillF64 = 1;   // NodeD + leaf flags (both walks)
+constexpr int kFillKids = 2;  // children lists (FP32 group walk)
 constexpr int kFillDump = 4;  // levels and branches (canonical dump)
 
 __device__ void emit_node(const FillArgs& a, int64_t n, int64_t id, int lv, int br, double cx,
@@ -710,20 +700,6 @@ __device__ void emit_node(const FillArgs& a, int64_t n, int64_t id, int lv, int 
         a.level[id] = (short)lv;
         a.branch[id] = (signed char)br;
     }
-    if (!(a.what & kFillF32)) return;
-    NodeF f;
-    float open2;
-    if (is_leaf) {
-        open2 = -1.0f;
-    } else if (a.theta > 0.0) {
-        const double r = size / a.theta;
-        open2 = (float)(r * r);
-    } else {
-        open2 = __int_as_float(0x7f800000);  // +inf: never accepted (size/dist < theta <= 0)
-    }
-    f.geo = make_float4((float)cx, (float)cy, (float)cz, open2);
-    f.mon = make_float4((float)d.mx, (float)d.my, (float)d.mz, (float)d.mass);
-    a.nodes32[id] = f;
 }
 
 // One thread per sorted particle i: the internal nodes starting at i (levels lprev+1 ..
@@ -768,190 +744,359 @@ __global__ __launch_bounds__(kFillBlock) void k_fill_nodes(FillArgs a, int64_t n
 // ------------------------------------------------------------------------------
 // 7. walks
 // ------------------------------------------------------------------------------
-// FP32 walk.  Targets: tgt (AoS f64) in the order given by order[] (nullable), results
-// written to acc[order[k]].
-//
-// A "unit" is lanes * WPW consecutive targets (lanes <= 64 targets per walk), walked by
-// one wavefront as WPW independent walks whose node loads are issued together, so one
-// walk's load latency overlaps the others'.  Lanes from `lanes` on stay idle.  The node
-// index of a walk is wave-uniform, so node records and skip pointers come through
-// scalar loads.
-//
-// Per visited node: a leaf (geo.w = -1, tested on the scalar unit) is accepted by every
-// lane waiting for it; an internal node gets the opening test in each waiting lane, and
-// the wave moves to the first child if any lane opened it and past its subtree
-// otherwise.  Accepted monopoles are summed in FP32 (fused multiply-add) and the FP32 partial sums
-// are added into FP64 accumulators every 64 visits and at the end.
-//
-// Scheduling.  Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD
-// b mod 8), and the units are split into 8 contiguous (Morton-ordered) ranges, so the
-// L2 of XCD x holds the nodes of one region of the tree.
-//  - static (DYN = false): xcd_block() renumbers the workgroups so that XCD x takes
-//    range x, one unit per wavefront; the grid has 8 * per_xcd workgroups and numbers
-//    past the last unit are idle;
-//  - dynamic (DYN = true): a wavefront takes units of its XCD's range from the atomic
-//    counter queue[x] until the range is exhausted, then from the other ranges in turn.
-//    Every wavefront ends after at most (units + 8) counter increments in total, so
-//    the grid drains.  queue[0..7] must be zero at launch.
+// Scheduling (both walks).  Workgroups are dealt to the 8 XCDs round-robin (workgroup
+// b runs on XCD b mod 8); xcd_block() renumbers them so that XCD x takes the x-th of 8
+// contiguous, Morton-ordered ranges of walk units, and the L2 of XCD x holds the nodes
+// of one region of the tree.  One wavefront per workgroup, so that the hardware refills
+// a SIMD slot as soon as its walk ends rather than when the slowest of several does.
 constexpr int kXcds = 8;
-constexpr int kFlushVisits = 64;
+constexpr int64_t kWalkUnits = 4096;  // walks in flight for small target counts
 constexpr int64_t kSortTargetsMin = 4096;  // position targets sorted from this many on
-constexpr int64_t kWalkWaves = 4096;       // walks in flight for small target counts
-// Threads per walk workgroup: one wavefront, so that the hardware refills a SIMD's slot
-// as soon as its walk ends rather than when the slowest of four walks does (1.5 %
-// faster than 256-thread workgroups at N=2^20, FP32 and FP64).
 constexpr int kWalkBlock = 64;
 __device__ __forceinline__ int64_t xcd_block(int64_t per_xcd)
 {
     return (int64_t)(blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
 }
 
-template <int WPW, bool GUARD>
-__device__ __forceinline__ void walk_unit_f32(const char* __restrict__ nodes,
-                                              const int* __restrict__ skip, int n_nodes,
-                                              const double* __restrict__ tgt,
-                                              const double4* __restrict__ stgt,
-                                              const u32* __restrict__ order, int64_t nt,
-                                              float eps2, double* __restrict__ acc,
-                                              int64_t unit, int lane, int lanes)
+// FP32 walk: group opening test, shared interaction lists.
+//
+// A wavefront walks G consecutive targets in walk (Morton) order, G a power of two up to
+// 16.  The group's frame is the FP64 position o of one of its targets; every coordinate
+// below is relative to o, so FP32 rounding scales with the group's extent and the
+// distances involved, not with where the system sits.  With the box B of the group's
+// targets, an internal node is accepted for the whole group when
+//     d_min(B, centre)^2 > (size / theta)^2,
+// d_min the distance from the node's geometric centre to B, or else when every target x
+// of the group has |centre - x|^2 > (size / theta)^2.  Either way every target passes the
+// reference's test size/|centre - x| < theta (gravhopper/_jbgrav.c:496-502) there too:
+// each target's interactions refine the ones the reference accepts for it (a node the
+// reference accepts is either accepted here or opened and replaced by its children), and
+// leaves are accepted as in the reference.
+//
+// Traversal: an LDS stack of node ids; each step the 64 lanes take up to 64 ids from
+// its top and test them in parallel (centre and size from the FP64 record, re-based to
+// the frame).  An accepted node's centre of mass, re-based from FP64, goes to the
+// group's interaction list; an opened node's children (kids[8 n ..]) are pushed.  Each
+// popped id pushes at most 8, so a step pops at most (kStackCap - top) / 7 ids; with
+// less room left than one id needs, the wave walks the top id's subtree one node at a
+// time with the skip pointers instead, pushing nothing.
+//
+// Evaluation: whenever the list holds 64 entries they are evaluated by the whole wave,
+// lane l taking target l mod G and the entries of slice l / G (64 / G slices), in FP32
+// (fused multiply-adds), each flush's FP32 sums added into FP64 accumulators; the
+// slices are added in FP64 at the end.  F64 evaluates the same lists (the traversal and
+// its tests are identical) in FP64, to check the FP32 arithmetic.  counts (nullable)
+// receives each target's list length.
+constexpr int kStackCap = 1024;
+constexpr int kListCap = 128;
+
+template <bool F64>
+struct GroupEntry {
+    typedef float4 T;
+};
+template <>
+struct GroupEntry<true> {
+    typedef double4 T;
+};
+
+__device__ __forceinline__ int wave_incl_sum(int v, int lane)
 {
-    const int64_t first = unit * lanes * WPW;  // the unit's first target
-    const int64_t base = first + lane;
-    eps2 = in_vgpr(eps2);
-    int64_t t[WPW];
-    float x[WPW], y[WPW], z[WPW], fx[WPW], fy[WPW], fz[WPW];
-    double ax[WPW], ay[WPW], az[WPW];
-    int my_skip[WPW], node[WPW];
-#pragma unroll
-    for (int w = 0; w < WPW; w++) {
-        const int64_t k = base + w * lanes;
-        const bool live = lane < lanes && k < nt;
-        t[w] = 0;
-        x[w] = y[w] = z[w] = 0.f;
-        if (live) {
-            t[w] = order ? (int64_t)order[k] : k;
-            if (stgt) {  // self walk: the particles in walk order, read contiguously
-                const double4 q = stgt[k];
-                x[w] = (float)q.x;
-                y[w] = (float)q.y;
-                z[w] = (float)q.z;
-            } else {
-                x[w] = (float)tgt[3 * t[w]];
-                y[w] = (float)tgt[3 * t[w] + 1];
-                z[w] = (float)tgt[3 * t[w] + 2];
-            }
-        }
-        fx[w] = fy[w] = fz[w] = 0.f;
-        ax[w] = ay[w] = az[w] = 0.0;
-        my_skip[w] = live ? 0 : 0x7fffffff;
-        // a walk with no live lane never starts
-        node[w] = __builtin_amdgcn_readfirstlane((first + w * lanes < nt) ? 0 : n_nodes);
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(v, o);
+        if (lane >= o) v += t;
     }
-    for (int it = 1;; it++) {
-        bool more = false;
-#pragma unroll
-        for (int w = 0; w < WPW; w++) more = more || node[w] < n_nodes;
-        if (!more) break;
-        // loads of every walk issued together (a finished walk reloads the last node)
-        // (one 32-byte vector load per record, so that the compiler issues it whole)
-        f32x8 nd[WPW];
-        int sk[WPW];
-#pragma unroll
-        for (int w = 0; w < WPW; w++) {
-            const u32 nw = (u32)(node[w] < n_nodes ? node[w] : n_nodes - 1);
-            nd[w] = *(const f32x8*)(nodes + nw * (u32)sizeof(NodeF));
-            sk[w] = *(const int*)((const char*)skip + nw * (u32)sizeof(int));
-        }
-#pragma unroll
-        for (int w = 0; w < WPW; w++) {
-            if (node[w] >= n_nodes) continue;  // this walk is done
-            // Opening test in every lane waiting for this node: node >= my_skip, the end
-            // of the last subtree the lane accepted (nodes are visited in increasing
-            // order, so a lane that opened this node's ancestors is waiting, one that
-            // accepted an ancestor is not, and opening needs no update).  A leaf has
-            // geo.w = -1, so it is always far, i.e. accepted.  Lane masks come straight
-            // from the compares (predicates 37: unsigned <=, 2: ordered greater).
-            const float cdx = nd[w][0] - x[w], cdy = nd[w][1] - y[w], cdz = nd[w][2] - z[w];
-            const float d2 = __builtin_fmaf(cdx, cdx, __builtin_fmaf(cdy, cdy, cdz * cdz));
-            const u64 waiting = __builtin_amdgcn_uicmp((u32)my_skip[w], (u32)node[w], 37);
-            const u64 far = __builtin_amdgcn_fcmpf(d2, nd[w][3], 2);
-            const u64 opened = waiting & ~far;  // the wave descends if any lane opens
-            const bool take = ((u32)my_skip[w] <= (u32)node[w]) && (d2 > nd[w][3]);
-            int next = opened ? node[w] + 1 : sk[w];
-            // uniform by construction; readfirstlane lets the compiler keep it scalar
-            next = __builtin_amdgcn_readfirstlane(next);
-            if (take) {
-                const float dx = nd[w][4] - x[w], dy = nd[w][5] - y[w], dz = nd[w][6] - z[w];
-                float r2 = __builtin_fmaf(dx, dx, eps2);
-                r2 = __builtin_fmaf(dy, dy, r2);
-                r2 = __builtin_fmaf(dz, dz, r2);
-                const float ri = __builtin_amdgcn_rsqf(r2);
-                float sc = (nd[w][7] * ri) * (ri * ri);
-                if (GUARD) sc = clamp_finite(sc);
-                fx[w] = __builtin_fmaf(dx, sc, fx[w]);
-                fy[w] = __builtin_fmaf(dy, sc, fy[w]);
-                fz[w] = __builtin_fmaf(dz, sc, fz[w]);
-                my_skip[w] = sk[w];
-            }
-            node[w] = next;
-        }
-        if (it % kFlushVisits == 0) {
-#pragma unroll
-            for (int w = 0; w < WPW; w++) {
-                ax[w] += (double)fx[w];
-                ay[w] += (double)fy[w];
-                az[w] += (double)fz[w];
-                fx[w] = fy[w] = fz[w] = 0.f;
-            }
-        }
-    }
-#pragma unroll
-    for (int w = 0; w < WPW; w++) {
-        if (lane < lanes && base + w * lanes < nt) {
-            acc[3 * t[w]] = ax[w] + (double)fx[w];
-            acc[3 * t[w] + 1] = ay[w] + (double)fy[w];
-            acc[3 * t[w] + 2] = az[w] + (double)fz[w];
-        }
-    }
+    return v;
 }
 
-template <int WPW, bool GUARD, bool DYN>
-__global__ __launch_bounds__(kBlock) void k_walk_f32(const NodeF* __restrict__ nodes,
-                                                     const int* __restrict__ skip, int n_nodes,
-                                                     const double* __restrict__ tgt,
-                                                     const double4* __restrict__ stgt,
-                                                     const u32* __restrict__ order, int64_t nt,
-                                                     float eps2, double* __restrict__ acc,
-                                                     int64_t n_units, int64_t per_xcd,
-                                                     u32* __restrict__ queue, int lanes)
+// One walk with the opening box B (centre bc, half-sizes bh) in the frame o: traversal
+// and evaluation as described above.  This lane's target is (dx0, dy0, dz0) = x - o
+// (FP64; t its FP32 rounding), its slice of each flush `slice` of `nslices`.  Adds the
+// lane's partial field to (ax, ay, az); listed = the list length.
+template <bool F64, bool GUARD>
+__device__ __forceinline__ void walk_lists(
+    const NodeD* __restrict__ nodes, const int* __restrict__ kids, const int* __restrict__ skip,
+    int n_nodes, double ox, double oy, double oz, float bcx, float bcy, float bcz, float bhx,
+    float bhy, float bhz, double dx0, double dy0, double dz0, int slice, int nslices,
+    float inv_theta, double eps2, int* stack, typename GroupEntry<F64>::T* list,
+    const float4* tg, int cnt, double& ax, double& ay, double& az, int& listed)
 {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const char* nb = (const char*)nodes;
-    if (!DYN) {
-        const int64_t unit = xcd_block(per_xcd) * (kWalkBlock / 64) + wave;
-        if (unit < n_units)
-            walk_unit_f32<WPW, GUARD>(nb, skip, n_nodes, tgt, stgt, order, nt, eps2, acc, unit,
-                                      lane, lanes);
+    typedef typename GroupEntry<F64>::T Entry;
+    const int lane = threadIdx.x;
+    const float tx = (float)dx0, ty = (float)dy0, tz = (float)dz0;
+    int head = 0, held = 0;  // list ring: first entry, entries held
+    const float eps2f = in_vgpr((float)eps2);
+
+    // evaluates `take` entries (<= 64) from the list head
+    auto flush = [&](int take) {
+        if (F64) {
+            double fx = 0.0, fy = 0.0, fz = 0.0;
+            for (int j = slice; j < take; j += nslices) {
+                const Entry e = list[(head + j) & (kListCap - 1)];
+                const double ex = (double)e.x - dx0, ey = (double)e.y - dy0,
+                             ez = (double)e.z - dz0;
+                const double d = ex * ex + ey * ey + ez * ez + eps2;
+                double inv = 0.0;
+                if (d != 0.0) {
+                    const double r = __builtin_amdgcn_rsq(d);
+                    const double rn = r * __builtin_fma(-0.5 * d, r * r, 1.5);
+                    inv = rn * rn * rn;
+                }
+                const double mi = (double)e.w * inv;
+                fx += ex * mi; fy += ey * mi; fz += ez * mi;
+            }
+            ax += fx; ay += fy; az += fz;
+        } else {
+            float fx = 0.f, fy = 0.f, fz = 0.f;
+            for (int j = slice; j < take; j += nslices) {
+                const Entry e = list[(head + j) & (kListCap - 1)];
+                const float ex = (float)e.x - tx, ey = (float)e.y - ty, ez = (float)e.z - tz;
+                float r2 = __builtin_fmaf(ex, ex, eps2f);
+                r2 = __builtin_fmaf(ey, ey, r2);
+                r2 = __builtin_fmaf(ez, ez, r2);
+                const float ri = __builtin_amdgcn_rsqf(r2);
+                float sc = ((float)e.w * ri) * (ri * ri);
+                if (GUARD) sc = clamp_finite(sc);
+                fx = __builtin_fmaf(ex, sc, fx);
+                fy = __builtin_fmaf(ey, sc, fy);
+                fz = __builtin_fmaf(ez, sc, fz);
+            }
+            ax += (double)fx; ay += (double)fy; az += (double)fz;
+        }
+        head = (head + take) & (kListCap - 1);
+        held -= take;
+    };
+    // the group opening test (identical in both instances); rec = (centre, size with the
+    // leaf's sign bit).  The box decides first; a node it rejects is still accepted when
+    // every target passes the reference's test against it (tg: the targets in the frame),
+    // which keeps a group whose box spans a region none of its targets is near (two halo
+    // particles on either side of a disk) from opening that region.
+    auto accepts = [&](const double4 rec) -> bool {
+        if (__double_as_longlong(rec.w) < 0) return true;  // leaf
+        const float cx = (float)(rec.x - ox), cy = (float)(rec.y - oy), cz = (float)(rec.z - oz);
+        const float ddx = fmaxf(0.f, fabsf(cx - bcx) - bhx);
+        const float ddy = fmaxf(0.f, fabsf(cy - bcy) - bhy);
+        const float ddz = fmaxf(0.f, fabsf(cz - bcz) - bhz);
+        const float d2 = __builtin_fmaf(ddx, ddx, __builtin_fmaf(ddy, ddy, ddz * ddz));
+        const float R = (float)rec.w * inv_theta;  // +inf for theta = 0
+        const float R2 = R * R;
+        if (d2 > R2) return true;
+        for (int i = 0; i < cnt; i++) {
+            const float4 t = tg[i];
+            const float ex = cx - t.x, ey = cy - t.y, ez = cz - t.z;
+            if (!(__builtin_fmaf(ex, ex, __builtin_fmaf(ey, ey, ez * ez)) > R2)) return false;
+        }
+        return true;
+    };
+    // the list entry of node n: centre of mass and mass, re-based to the frame
+    auto entry_of = [&](int n) -> Entry {
+        const double4 m = *(const double4*)((const double*)(nodes + n) + 4);
+        Entry e;
+        e.x = m.x - ox; e.y = m.y - oy; e.z = m.z - oz; e.w = m.w;
+        return e;
+    };
+
+    __syncthreads();  // the previous walk of this wavefront is done with stack and list
+    if (lane == 0) stack[0] = 0;
+    int top = n_nodes > 0 ? 1 : 0;
+    __syncthreads();
+    while (top > 0) {
+        int k = top < 64 ? top : 64;
+        const int room = (kStackCap - top) / 7;
+        if (room < 1) {
+            // no room for a parallel step: walk the top id's subtree sequentially
+            const int n0 = stack[top - 1];
+            top -= 1;
+            int p = n0;
+            const int end = skip[n0];
+            while (p < end) {
+                const double4 rec = *(const double4*)(nodes + p);
+                if (accepts(rec)) {
+                    if (lane == 0) list[(head + held) & (kListCap - 1)] = entry_of(p);
+                    held += 1;
+                    listed += 1;
+                    p = skip[p];
+                } else {
+                    p = p + 1;
+                }
+                __syncthreads();
+                if (held >= 64) {
+                    flush(64);
+                    __syncthreads();
+                }
+            }
+            continue;
+        }
+        if (k > room) k = room;
+        const bool active = lane < k;
+        const int n = active ? stack[top - 1 - lane] : 0;
+        bool acc_n = false;
+        int4 ka = make_int4(-1, -1, -1, -1), kb = make_int4(-1, -1, -1, -1);
+        int nk = 0;
+        if (active) {
+            const double4 rec = *(const double4*)(nodes + n);
+            acc_n = accepts(rec);
+            if (!acc_n) {
+                ka = *(const int4*)(kids + 8 * (int64_t)n);
+                kb = *(const int4*)(kids + 8 * (int64_t)n + 4);
+                nk = (ka.x >= 0) + (ka.y >= 0) + (ka.z >= 0) + (ka.w >= 0) + (kb.x >= 0) +
+                     (kb.y >= 0) + (kb.z >= 0) + (kb.w >= 0);
+            }
+        }
+        // appends to the list, in lane order
+        const unsigned long long am = __ballot(acc_n);
+        Entry e;
+        if (acc_n) e = entry_of(n);
+        const int before = (int)__popcll(am & ((1ull << lane) - 1ull));
+        const int added = (int)__popcll(am);
+        // pushes: the popped ids' slots are reused, children in lane order
+        const int incl = wave_incl_sum(nk, lane);
+        const int total = __shfl(incl, 63);
+        __syncthreads();  // every lane has read its popped id
+        if (acc_n) list[(head + held + before) & (kListCap - 1)] = e;
+        if (nk > 0) {
+            int w = top - k + incl - nk;
+            const int kk[8] = {ka.x, ka.y, ka.z, ka.w, kb.x, kb.y, kb.z, kb.w};
+#pragma unroll
+            for (int q = 0; q < 8; q++)
+                if (kk[q] >= 0) stack[w++] = kk[q];
+        }
+        held += added;
+        listed += added;
+        top = top - k + total;
+        __syncthreads();
+        while (held >= 64) flush(64);
+        __syncthreads();
+    }
+    if (held > 0) flush(held);
+}
+
+constexpr int kLooseLog2 = 6;
+
+__device__ __forceinline__ int exponent_f(float x)
+{
+    return (int)((__float_as_int(x) >> 23) & 0xff) - 127;  // 0 gives -127
+}
+
+template <bool F64, bool GUARD>
+__global__ __launch_bounds__(kWalkBlock) void k_walk_group(
+    const NodeD* __restrict__ nodes, const int* __restrict__ kids, const int* __restrict__ skip,
+    int n_nodes, const double* __restrict__ tgt, const double4* __restrict__ stgt,
+    const u32* __restrict__ order, int64_t nt, int G, float inv_theta, double eps2,
+    double* __restrict__ acc, int* __restrict__ counts, int64_t per_xcd)
+{
+    typedef typename GroupEntry<F64>::T Entry;
+    __shared__ int stack[kStackCap];
+    __shared__ Entry list[kListCap];
+    __shared__ float4 tg[16];
+    const int lane = threadIdx.x;
+    const int64_t first = xcd_block(per_xcd) * G;
+    if (first >= nt) return;
+    const int cnt = (int)(nt - first < G ? nt - first : G);
+    const int ti = lane & (G - 1);  // this lane's target within the group
+    auto target = [&](int64_t k) -> double4 {
+        if (stgt) return stgt[k];
+        const int64_t t = order ? (int64_t)order[k] : k;
+        return make_double4(tgt[3 * t], tgt[3 * t + 1], tgt[3 * t + 2], 0.0);
+    };
+    auto store = [&](int64_t k, double fx, double fy, double fz, int listed) {
+        const int64_t t = order ? (int64_t)order[k] : k;
+        acc[3 * t] = fx;
+        acc[3 * t + 1] = fy;
+        acc[3 * t + 2] = fz;
+        if (counts) counts[t] = listed;
+    };
+    // the group's frame: one of its targets, in FP64
+    const double4 o = target(first + (G / 2 < cnt - 1 ? G / 2 : cnt - 1));
+    // this lane's target (lanes past the group's count repeat its last target)
+    const double4 q = target(first + (ti < cnt ? ti : cnt - 1));
+    const double dx0 = q.x - o.x, dy0 = q.y - o.y, dz0 = q.z - o.z;
+    const float tx = (float)dx0, ty = (float)dy0, tz = (float)dz0;
+    // the targets' box, in FP32 in the frame (every lane holds one of the targets)
+    float lox = tx, loy = ty, loz = tz, hix = tx, hiy = ty, hiz = tz;
+    for (int s = 32; s > 0; s >>= 1) {
+        lox = fminf(lox, __shfl_xor(lox, s)); hix = fmaxf(hix, __shfl_xor(hix, s));
+        loy = fminf(loy, __shfl_xor(loy, s)); hiy = fmaxf(hiy, __shfl_xor(hiy, s));
+        loz = fminf(loz, __shfl_xor(loz, s)); hiz = fmaxf(hiz, __shfl_xor(hiz, s));
+    }
+    // Looseness: the box extent against the geometric mean of the gaps between
+    // consecutive targets (max norm), through binary exponents.  A group whose sorted
+    // targets jump between distant systems has targets far from its frame, whose FP32
+    // coordinates would then carry 2^-24 times that distance; such a group is walked
+    // target by target instead.  (With 16 targets one long jump among 15 gaps moves the
+    // geometric mean little; groups of 2 to 8 are not checked.)
+    const float nx = __shfl(tx, (lane + 1) & 63), ny = __shfl(ty, (lane + 1) & 63),
+                nz = __shfl(tz, (lane + 1) & 63);
+    int eg = lane < cnt - 1 && lane + 1 < G
+                 ? exponent_f(fmaxf(fabsf(nx - tx), fmaxf(fabsf(ny - ty), fabsf(nz - tz))))
+                 : 0;
+    for (int s = 32; s > 0; s >>= 1) eg += __shfl_xor(eg, s);
+    const int ext = exponent_f(fmaxf(hix - lox, fmaxf(hiy - loy, hiz - loz)));
+    const bool loose = G == 16 && cnt > 1 && ext * (cnt - 1) - eg > kLooseLog2 * (cnt - 1);
+
+    if (!loose) {
+        // the group's targets in the frame, for the per-target test
+        if (lane < cnt) tg[lane] = make_float4(tx, ty, tz, 0.f);
+        // box centre and half-sizes, the half-sizes widened by a few ulp so that rounding
+        // never shrinks the box
+        const float bcx = 0.5f * (lox + hix), bcy = 0.5f * (loy + hiy),
+                    bcz = 0.5f * (loz + hiz);
+        const float bhx = fmaxf(hix - bcx, bcx - lox) * 1.000001f;
+        const float bhy = fmaxf(hiy - bcy, bcy - loy) * 1.000001f;
+        const float bhz = fmaxf(hiz - bcz, bcz - loz) * 1.000001f;
+        double ax = 0.0, ay = 0.0, az = 0.0;
+        int listed = 0;
+        walk_lists<F64, GUARD>(nodes, kids, skip, n_nodes, o.x, o.y, o.z, bcx, bcy, bcz, bhx,
+                               bhy, bhz, dx0, dy0, dz0, lane / G, 64 / G, inv_theta, eps2,
+                               stack, list, tg, cnt, ax, ay, az, listed);
+        // add the slices of each target (lanes ti, ti + G, ...), in FP64
+        for (int s = G; s < 64; s <<= 1) {
+            ax += __shfl_xor(ax, s);
+            ay += __shfl_xor(ay, s);
+            az += __shfl_xor(az, s);
+        }
+        if (lane < cnt) store(first + lane, ax, ay, az, listed);
         return;
     }
-    const int home = (int)(blockIdx.x % kXcds);
-    for (int i = 0; i < kXcds; i++) {
-        const int q = (home + i) % kXcds;
-        const int64_t lo = q * n_units / kXcds, hi = (q + 1) * n_units / kXcds;
-        for (;;) {
-            u32 k = 0;
-            if (lane == 0) k = atomicAdd(&queue[q], 1u);
-            k = __builtin_amdgcn_readfirstlane(k);
-            if (lo + (int64_t)k >= hi) break;
-            walk_unit_f32<WPW, GUARD>(nb, skip, n_nodes, tgt, stgt, order, nt, eps2, acc,
-                                      lo + (int64_t)k, lane, lanes);
+    // target by target, each in its own frame with a point as box (the reference's own
+    // opening test), 64 slices
+    if (lane == 0) tg[0] = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int k = 0; k < cnt; k++) {
+        const double4 p = target(first + k);
+        double ax = 0.0, ay = 0.0, az = 0.0;
+        int listed = 0;
+        walk_lists<F64, GUARD>(nodes, kids, skip, n_nodes, p.x, p.y, p.z, 0.f, 0.f, 0.f, 0.f,
+                               0.f, 0.f, 0.0, 0.0, 0.0, lane, 64, inv_theta, eps2, stack, list,
+                               tg, 1, ax, ay, az, listed);
+        for (int s = 1; s < 64; s <<= 1) {
+            ax += __shfl_xor(ax, s);
+            ay += __shfl_xor(ay, s);
+            az += __shfl_xor(az, s);
         }
+        if (lane == 0) store(first + k, ax, ay, az, listed);
     }
 }
 
-__global__ void k_zero_queue(u32* __restrict__ q)
+// Children of every internal node, in pre-order: kids[8 n + k], -1 after the last.  The
+// children of n are n + 1 and then, while below skip[n], the skip pointer of the previous
+// child.
+__global__ __launch_bounds__(kBlock) void k_kids(const NodeD* __restrict__ nodes,
+                                                 const int* __restrict__ skip, int n_nodes,
+                                                 int* __restrict__ kids)
 {
-    if (threadIdx.x < kXcds) q[threadIdx.x] = 0u;
+    const int n = (int)(blockIdx.x * kBlock + threadIdx.x);
+    if (n >= n_nodes) return;
+    int k[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
+    if (__double_as_longlong(nodes[n].size) >= 0) {  // internal
+        const int end = skip[n];
+        int c = n + 1, q = 0;
+        while (c < end && q < 8) {
+            k[q++] = c;
+            c = skip[c];
+        }
+    }
+    int4* o = (int4*)(kids + 8 * (int64_t)n);
+    o[0] = make_int4(k[0], k[1], k[2], k[3]);
+    o[1] = make_int4(k[4], k[5], k[6], k[7]);
 }
 
 // FP64 walk: the reference's opening test decided exactly, the monopole to a few ulp.
@@ -1223,7 +1368,7 @@ static int build_tree(gh_ctx* c, const double* d_pos, const double* d_mass, int6
         GH_TRY(ensure(c, tb->nodes64, sizeof(NodeD) * T));
         GH_TRY(ensure(c, tb->leaf, sizeof(int) * T));
     }
-    if (what & kFillF32) GH_TRY(ensure(c, tb->nodes32, sizeof(NodeF) * T));
+    if (what & kFillKids) GH_TRY(ensure(c, tb->kids, 8 * sizeof(int) * T));
     if (what & kFillDump) {
         GH_TRY(ensure(c, tb->level, sizeof(short) * T));
         GH_TRY(ensure(c, tb->branch, T));
@@ -1237,9 +1382,7 @@ static int build_tree(gh_ctx* c, const double* d_pos, const double* d_mass, int6
     fa.spos = tb->spos.as<double4>();
     fa.box = tb->box.as<double>();
     fa.pref = pref_view;
-    fa.theta = theta;
     fa.nodes64 = tb->nodes64.as<NodeD>();
-    fa.nodes32 = tb->nodes32.as<NodeF>();
     fa.skip = tb->skip.as<int>();
     fa.level = tb->level.as<short>();
     fa.branch = tb->branch.as<signed char>();
@@ -1248,6 +1391,12 @@ static int build_tree(gh_ctx* c, const double* d_pos, const double* d_mass, int6
     hipLaunchKernelGGL(k_fill_nodes, dim3(blocks_for(n, kFillBlock)), dim3(kFillBlock), 0, s, fa,
                        n);
     GH_TRY(check_launch(c, "k_fill_nodes"));
+    if (what & kFillKids) {
+        hipLaunchKernelGGL(k_kids, dim3(blocks_for(total, kBlock)), dim3(kBlock), 0, s,
+                           tb->nodes64.as<NodeD>(), tb->skip.as<int>(), (int)total,
+                           tb->kids.as<int>());
+        GH_TRY(check_launch(c, "k_kids"));
+    }
     c->last_tree_nodes = total;
     return 0;
 }
@@ -1284,11 +1433,11 @@ static int walk(gh_ctx* c, const double* tgt, bool self, int64_t nt, double eps,
         order = tb->tidx[sel].as<u32>();
     }
     // Targets per walk: 64 (a full wavefront) when there are enough targets to fill the
-    // GPU; fewer targets give each walk fewer lanes, so that about kWalkWaves walks run
+    // GPU; fewer targets give each walk fewer lanes, so that about kWalkUnits walks run
     // side by side (a walk's length grows with the spread of its targets, and a few
     // long walks would leave most of the GPU idle).
-    const int lanes = (int)std::min<int64_t>(64, std::max<int64_t>(1, (nt + kWalkWaves - 1) /
-                                                                          kWalkWaves));
+    const int lanes = (int)std::min<int64_t>(64, std::max<int64_t>(1, (nt + kWalkUnits - 1) /
+                                                                          kWalkUnits));
     const int waves_per_block = kWalkBlock / 64;
     if (precision == GH_FP64) {
         const int64_t n_walks = (nt + lanes - 1) / lanes;
@@ -1308,54 +1457,34 @@ static int walk(gh_ctx* c, const double* tgt, bool self, int64_t nt, double eps,
                                c_lo, c_hi, d_acc, counts, per, lanes);
         return check_launch(c, "k_walk_f64");
     }
-    const float eps2 = (float)(eps * eps);
+    // FP32: the group walk, 16 targets per wavefront, or one target per wavefront when
+    // there are too few targets for kWalkUnits groups of 16 (then every target walks in
+    // its own frame with the reference's opening test)
+    int G = c->tune[GH_TUNE_WALK_GROUP];
+    if (!(G == 1 || G == 2 || G == 4 || G == 8 || G == 16))
+        G = (nt >= 16 * kWalkUnits) ? 16 : 1;
+    const bool f64eval = c->tune[GH_TUNE_WALK_F64EVAL] > 0;
     const bool guard = fp32_guard(eps);
-    // node offsets are 32-bit byte offsets inside the kernel (records below 4 GiB)
-    const int64_t max_nodes = ((int64_t)1 << 32) / (int64_t)sizeof(NodeF);
-    if (tb->n_nodes >= max_nodes)
-        return fail(c, GH_EARG, "FP32 tree walk supports at most %lld nodes (tree has %lld)",
-                    (long long)(max_nodes - 1), (long long)tb->n_nodes);
-    // measured at N = 2^20 (halo + disk): one walk per wavefront and the static schedule
-    // are fastest (1.30 ms; two walks 1.34 ms; dynamic 1.67-1.69 ms, as consecutive
-    // units then land on different CUs and share less of the scalar cache)
-    const int tw = c->tune[GH_TUNE_WALKS_PER_WAVE];
-    const int wpw = (tw == 1 || tw == 2) ? tw : 1;
-    const bool dyn = c->tune[GH_TUNE_WALK_SCHEDULE] > 0;
-    const NodeF* nd = tb->nodes32.as<NodeF>();
+    const float inv_theta = theta > 0.0 ? (float)(1.0 / theta) : __builtin_huge_valf();
+    const int64_t n_groups = (nt + G - 1) / G;
+    const int64_t per = (n_groups + kXcds - 1) / kXcds;
+    const dim3 grid((unsigned)(per * kXcds));
+    const NodeD* nd = tb->nodes64.as<NodeD>();
+    const int* kd = tb->kids.as<int>();
     const int* sk = tb->skip.as<int>();
     const int T = (int)tb->n_nodes;
-    const int64_t n_units = (nt + (int64_t)lanes * wpw - 1) / ((int64_t)lanes * wpw);
-    const int64_t nblk = (n_units + waves_per_block - 1) / waves_per_block;
-    const int64_t per = (nblk + kXcds - 1) / kXcds;
-    u32* q = nullptr;
-    dim3 grid((unsigned)(per * kXcds));
-    if (dyn) {
-        GH_TRY(ensure(c, tb->walk_queue, sizeof(u32) * kXcds));
-        q = tb->walk_queue.as<u32>();
-        hipLaunchKernelGGL(k_zero_queue, dim3(1), dim3(64), 0, s, q);
-        GH_TRY(check_launch(c, "k_zero_queue"));
-        // one resident wave per SIMD slot: 8 wavefronts on each of the 4 SIMDs of a CU
-        const int64_t resident = (int64_t)c->n_cu * 32 / waves_per_block;
-        grid = dim3((unsigned)(nblk < resident ? nblk : resident));
-    }
-#define GH_WALK32(W, G, D)                                                                 \
-    hipLaunchKernelGGL((k_walk_f32<W, G, D>), grid, dim3(kWalkBlock), 0, s, nd, sk, T, tgt, \
-                       stgt, order, nt, eps2, d_acc, n_units, per, q, lanes)
-    if (wpw == 2) {
-        if (dyn) {
-            if (guard) GH_WALK32(2, true, true); else GH_WALK32(2, false, true);
-        } else {
-            if (guard) GH_WALK32(2, true, false); else GH_WALK32(2, false, false);
-        }
-    } else {
-        if (dyn) {
-            if (guard) GH_WALK32(1, true, true); else GH_WALK32(1, false, true);
-        } else {
-            if (guard) GH_WALK32(1, true, false); else GH_WALK32(1, false, false);
-        }
-    }
-#undef GH_WALK32
-    return check_launch(c, "k_walk_f32");
+    const double eps2 = eps * eps;
+#define GH_GWALK(F, GU)                                                                    \
+    hipLaunchKernelGGL((k_walk_group<F, GU>), grid, dim3(kWalkBlock), 0, s, nd, kd, sk, T, tgt, \
+                       stgt, order, nt, G, inv_theta, eps2, d_acc, counts, per)
+    if (f64eval)
+        GH_GWALK(true, false);
+    else if (guard)
+        GH_GWALK(false, true);
+    else
+        GH_GWALK(false, false);
+#undef GH_GWALK
+    return check_launch(c, "k_walk_group");
 }
 
 int tree_dev(gh_ctx* c, const double* d_pos, const double* d_mass, int64_t np,
@@ -1369,7 +1498,7 @@ int tree_dev(gh_ctx* c, const double* d_pos, const double* d_mass, int64_t np,
     if (precision != GH_FP64 && precision != GH_FP32)
         return fail(c, GH_EARG, "unknown precision %d", precision);
     GH_TRY(build_tree(c, d_pos, d_mass, np, eps, theta,
-                      precision == GH_FP64 ? kFillF64 : kFillF32, s));
+                      precision == GH_FP64 ? kFillF64 : (kFillF64 | kFillKids), s));
     return walk(c, self ? d_pos : d_fpos, self, nt, eps, theta, precision, d_acc, nullptr, s);
 }
 
@@ -1381,9 +1510,9 @@ void tree_free(gh_ctx* c)
                       &tb->bbox_part, &tb->box,   &tb->flags,     &tb->delta,   &tb->off,
                       &tb->prefix,  &tb->spos,      &tb->block_sums, &tb->block_counts,
                       &tb->packed,
-                      &tb->nodes64, &tb->nodes32,
+                      &tb->nodes64, &tb->kids,
                       &tb->skip,    &tb->level,   &tb->branch,    &tb->leaf,    &tb->tkeys[0],
-                      &tb->tkeys[1], &tb->tidx[0], &tb->tidx[1],  &tb->walk_queue};
+                      &tb->tkeys[1], &tb->tidx[0], &tb->tidx[1]};
     for (DevBuf* b : bufs) release(c, *b);
     if (tb->readback) (void)hipHostFree(tb->readback);
     delete tb;
@@ -1440,10 +1569,11 @@ extern "C" int gh_tree_dump(gh_ctx* c, const double* pos, const double* mass, in
     return 0;
 }
 
-// Accepted-node counts of the FP64 walk, for comparing the walk with the reference's.
-extern "C" int gh_tree_interactions(gh_ctx* c, const double* pos, const double* mass,
-                                    int64_t np, const double* force_pos, int64_t nf, double eps,
-                                    double theta, double* acc, int64_t* counts)
+// Interaction counts per target: the FP64 walk's accepted nodes (comparable with the
+// reference's walk) or the FP32 group walk's list lengths.
+static int tree_counts(gh_ctx* c, const double* pos, const double* mass, int64_t np,
+                       const double* force_pos, int64_t nf, double eps, double theta,
+                       int precision, double* acc, int64_t* counts)
 {
     if (!c) return GH_EARG;
     if (np < 1) return fail(c, GH_EARG, "the tree needs at least one particle");
@@ -1467,8 +1597,8 @@ extern "C" int gh_tree_interactions(gh_ctx* c, const double* pos, const double* 
         tgt = c->fpos64.as<double>();
     }
     GH_TRY(build_tree(c, c->pos64.as<double>(), c->mass64.as<double>(), np, eps, theta,
-                      kFillF64, s));
-    GH_TRY(walk(c, tgt, self, nt, eps, theta, GH_FP64, c->acc64.as<double>(),
+                      precision == GH_FP64 ? kFillF64 : (kFillF64 | kFillKids), s));
+    GH_TRY(walk(c, tgt, self, nt, eps, theta, precision, c->acc64.as<double>(),
                 c->scratch.as<int>(), s));
     std::vector<int> cnt((size_t)(nt > 0 ? nt : 1));
     GH_HIP(c, hipMemcpyAsync(acc, c->acc64.p, sizeof(double) * 3 * (size_t)nt,
@@ -1478,4 +1608,18 @@ extern "C" int gh_tree_interactions(gh_ctx* c, const double* pos, const double* 
     GH_HIP(c, hipStreamSynchronize(s));
     for (int64_t k = 0; k < nt; k++) counts[k] = cnt[k];
     return 0;
+}
+
+extern "C" int gh_tree_interactions(gh_ctx* c, const double* pos, const double* mass,
+                                    int64_t np, const double* force_pos, int64_t nf, double eps,
+                                    double theta, double* acc, int64_t* counts)
+{
+    return tree_counts(c, pos, mass, np, force_pos, nf, eps, theta, GH_FP64, acc, counts);
+}
+
+extern "C" int gh_tree_interactions_fp32(gh_ctx* c, const double* pos, const double* mass,
+                                         int64_t np, const double* force_pos, int64_t nf,
+                                         double eps, double theta, double* acc, int64_t* counts)
+{
+    return tree_counts(c, pos, mass, np, force_pos, nf, eps, theta, GH_FP32, acc, counts);
 }

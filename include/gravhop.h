@@ -117,6 +117,13 @@ int gh_tree_interactions(gh_ctx *ctx, const double *pos, const double *mass, int
                          const double *force_pos, int64_t nf, double eps, double theta,
                          double *acc, int64_t *counts);
 
+/* FP32 tree force (the group walk) together with each target's interaction-list length:
+ * the nodes the group opening test accepted for the target's group, at least as many as
+ * the reference's walk accepts for the target.  No reference counterpart. */
+int gh_tree_interactions_fp32(gh_ctx *ctx, const double *pos, const double *mass,
+                              int64_t np, const double *force_pos, int64_t nf, double eps,
+                              double theta, double *acc, int64_t *counts);
+
 /* Device-resident drift-kick-drift leapfrog: nsteps of Simulation.perform_timestep
  * (gravhopper/gravhopper.py:405-416) with the force of calculate_acceleration
  * (:419-459) and no external forces, in code units (kpc, km/s, Msun, Myr):
@@ -208,13 +215,12 @@ int gh_timing_read(gh_ctx *ctx, int kernel_class, double *total_ms, int64_t *lau
  *   GH_TUNE_DIRECT_SELF  FP32 self field: 0 automatic, -1 one-sided kernel,
  *                        1/2/4 pair-once kernel with that many particles per lane
  *   GH_TUNE_DIRECT_IPT   FP32 one-sided kernel: 0 automatic, 1/2/4 targets per lane
- *   GH_TUNE_WALKS_PER_WAVE  FP32 tree walk: 0 automatic, 1 or 2 independent 64-target
- *                        walks per wavefront
+ *   GH_TUNE_WALK_GROUP   FP32 tree walk: 0 automatic, 1/2/4/8/16 targets per group
+ *                        (the targets that share one opening test and interaction list)
  *   GH_TUNE_RUN_GRAPH    direct-sum gh_run / gh_run_dev: 0 replay two captured steps as
  *                        one HIP graph (from 4 steps on), -1 launch every kernel eagerly
- *   GH_TUNE_WALK_SCHEDULE  FP32 tree walk: 0 automatic (static), -1 static (one unit of
- *                        targets per wavefront), 1 dynamic (wavefronts take units from
- *                        per-XCD atomic queues)
+ *   GH_TUNE_WALK_F64EVAL FP32 tree walk: 1 evaluates the same interaction lists in FP64
+ *                        (a check of the FP32 arithmetic), 0 FP32
  *   GH_TUNE_DIRECT_F64_TPW  FP64 direct sum: 0 automatic, 1/2/4/8/16/32/64 targets per
  *                        wavefront (64: one target per lane; fewer: the terms of a
  *                        target are computed by 64/value lanes and then added in order),
@@ -224,14 +230,16 @@ int gh_timing_read(gh_ctx *ctx, int kernel_class, double *total_ms, int64_t *lau
  *   GH_TUNE_ALLOC_LIMIT_MB  cap, in MiB, on the device memory the context's arenas may
  *                        hold (0: no cap); a call that would exceed it fails with
  *                        GH_ENOMEM and leaves the context usable
- * The FP32 direct-sum keys change results only by FP32 rounding; GH_TUNE_DIRECT_F64_TPW,
- * the walk keys and GH_TUNE_RUN_GRAPH do not change results; GH_TUNE_ALLOC_LIMIT_MB
- * only decides whether a call may allocate. */
+ * The FP32 direct-sum keys and GH_TUNE_WALK_F64EVAL change results only by rounding;
+ * GH_TUNE_WALK_GROUP changes which nodes the FP32 walk accepts (never fewer than the
+ * reference's opening test accepts for any target); GH_TUNE_DIRECT_F64_TPW and
+ * GH_TUNE_RUN_GRAPH do not change results; GH_TUNE_ALLOC_LIMIT_MB only decides whether a
+ * call may allocate. */
 #define GH_TUNE_DIRECT_SELF 0
 #define GH_TUNE_DIRECT_IPT 1
-#define GH_TUNE_WALKS_PER_WAVE 2
+#define GH_TUNE_WALK_GROUP 2
 #define GH_TUNE_RUN_GRAPH 3
-#define GH_TUNE_WALK_SCHEDULE 4
+#define GH_TUNE_WALK_F64EVAL 4
 #define GH_TUNE_ALLOC_LIMIT_MB 5
 #define GH_TUNE_DIRECT_F64_TPW 6
 #define GH_TUNE_COUNT 7

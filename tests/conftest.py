@@ -8,6 +8,14 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
+# torch ships its own HIP runtime; loaded after libgravhop.so has initialised the system
+# one, it finds no GPU ("No HIP GPUs are available").  Tests that hand torch tensors to
+# the library therefore need torch loaded first, whatever order the tests run in.
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 GOLDEN = os.path.join(ROOT, "tests", "golden", "reference_vectors.npz")
 
 

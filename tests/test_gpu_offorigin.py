@@ -193,3 +193,29 @@ def test_adjoint_translation_invariant(ctx):
             err = np.linalg.norm(a2 - b2, axis=1) / np.maximum(np.linalg.norm(b2, axis=1), 1e-300)
             print("adjoint shift %d %s: median %.2e" % (k, what, np.median(err)), flush=True)
             assert np.median(err) < 1e-4, (k, what)
+
+
+@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("group", [0, 16])
+def test_tree_walk_off_origin(ctx, oracle, name, group):
+    # the FP32 group walk re-bases every list entry and the opening tests to its group's
+    # frame from FP64: FP32 against the same lists in FP64 at the direct-sum tolerance,
+    # and no less accurate than the reference tree against the FP64 direct sum (to FP32
+    # rounding: with one target per group the lists are the reference's own)
+    pos, mass = case(name, 20000)
+    ctx.set_tuning(_native.TUNE_WALK_GROUP, group)
+    try:
+        got = ctx.tree(pos, mass, EPS, 0.7, FP32)
+        ctx.set_tuning(_native.TUNE_WALK_F64EVAL, 1)
+        lists64 = ctx.tree(pos, mass, EPS, 0.7, FP32)
+    finally:
+        ctx.set_tuning(_native.TUNE_WALK_F64EVAL, 0)
+        ctx.set_tuning(_native.TUNE_WALK_GROUP, 0)
+    check(got, lists64, "fp32 walk %s" % name)
+    direct = oracle.direct(pos, mass, EPS)
+    eg = rel_err(got, direct)
+    er = rel_err(oracle.tree(pos, mass, EPS, 0.7), direct)
+    print("%s vs direct: walk median %.2e p99 %.2e, reference tree %.2e / %.2e"
+          % (name, np.median(eg), np.percentile(eg, 99), np.median(er), np.percentile(er, 99)))
+    assert np.median(eg) <= np.median(er) * 1.001
+    assert np.percentile(eg, 99) <= np.percentile(er, 99) * 1.001

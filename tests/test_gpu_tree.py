@@ -4,7 +4,15 @@ Topology (node levels, branches, centres, sizes and per-leaf particle) must be
 bit-identical to the reference's insertion-built octree (oracle restatement, itself
 bit-identical to gravhopper/_jbgrav.c's tree forces).  The FP64 walk must accept the
 same nodes as the reference walk (exact interaction counts per target) and give the
-same forces up to summation order; the FP32 walk is checked with a stated tolerance.
+same forces up to summation order.
+
+The FP32 walk opens nodes by a group test that accepts a node only where the
+reference's test accepts it for every target of the group, so each target's
+interactions refine the reference's.  It is checked three ways: (1) its FP32 arithmetic
+against the same interaction lists evaluated in FP64 (GH_TUNE_WALK_F64EVAL), at the
+direct sum's FP32 tolerance; (2) its accuracy against the FP64 direct sum, which must be
+no worse than the reference tree's; (3) at theta = 0 every list is every particle, so
+the walk must equal the direct sum.
 """
 import numpy as np
 import pytest
@@ -16,10 +24,42 @@ pytestmark = pytest.mark.gpu
 
 FP64, FP32 = _native.GH_FP64, _native.GH_FP32
 
-# FP32 walk tolerance against the FP64 reference tree: FP32 monopoles and opening test
-# (a node on the opening boundary may go either way), FP32 partial sums added into FP64
-# every 64 node visits.
-FP32_TREE_MEDIAN = 1e-5
+# FP32 walk arithmetic against the same lists in FP64: FP32 monopoles, FP32 sums of at
+# most 64 / (64 / G) list entries per lane added into FP64 (the direct sum's tolerance)
+FP32_LIST_MEDIAN, FP32_LIST_P99 = 1e-6, 1e-4
+
+
+def fp32_walk(ctx, pos, mass, eps, theta, fpos=None):
+    """FP32 group walk, and the same interaction lists evaluated in FP64."""
+    got = ctx.tree(pos, mass, eps, theta, FP32, force_pos=fpos)
+    ctx.set_tuning(_native.TUNE_WALK_F64EVAL, 1)
+    try:
+        ref = ctx.tree(pos, mass, eps, theta, FP32, force_pos=fpos)
+    finally:
+        ctx.set_tuning(_native.TUNE_WALK_F64EVAL, 0)
+    return got, ref
+
+
+def check_fp32_walk(ctx, oracle, pos, mass, eps, theta, fpos=None, what=""):
+    got, lists64 = fp32_walk(ctx, pos, mass, eps, theta, fpos)
+    e = rel_err(got, lists64)
+    print("%s: fp32 vs same lists in fp64: median %.2e p99 %.2e"
+          % (what, np.median(e), np.percentile(e, 99)), flush=True)
+    assert np.median(e) <= FP32_LIST_MEDIAN and np.percentile(e, 99) <= FP32_LIST_P99, what
+    # accuracy against the FP64 direct sum: no worse than the reference tree's
+    if fpos is None:
+        direct = oracle.direct(pos, mass, eps)
+        reftree = oracle.tree(pos, mass, eps, theta)
+    else:
+        direct = oracle.direct_position(pos, mass, fpos, eps)
+        reftree = oracle.tree(pos, mass, eps, theta, force_pos=fpos)
+    eg, er = rel_err(lists64, direct), rel_err(reftree, direct)
+    print("%s: vs direct: group walk median %.2e p99 %.2e, reference tree %.2e / %.2e"
+          % (what, np.median(eg), np.percentile(eg, 99), np.median(er),
+             np.percentile(er, 99)), flush=True)
+    assert np.median(eg) <= np.median(er) * 1.001 + 1e-15, what
+    assert np.percentile(eg, 99) <= np.percentile(er, 99) * 1.05 + 1e-15, what
+    return got
 
 
 def compare_topology(got, want, name):
@@ -118,23 +158,22 @@ def test_position_walk_counts(ctx, oracle, n_targets):
     np.testing.assert_allclose(got, want, rtol=1e-10, atol=1e-12 * np.abs(want).max())
     got = ctx.tree(pos, mass, 0.01, 0.7, FP64, force_pos=fpos)
     np.testing.assert_allclose(got, want, rtol=1e-10, atol=1e-12 * np.abs(want).max())
-    got32 = ctx.tree(pos, mass, 0.01, 0.7, FP32, force_pos=fpos)
-    assert np.median(rel_err(got32, want)) < FP32_TREE_MEDIAN
+    check_fp32_walk(ctx, oracle, pos, mass, 0.01, 0.7, fpos=fpos, what="%d targets" % n_targets)
 
 
-def test_fp32_walk_accuracy(ctx, oracle):
+@pytest.mark.parametrize("theta", [0.3, 0.7, 1.0])
+def test_fp32_walk_accuracy(ctx, oracle, theta):
     g = workloads.galaxy(n_halo=16384, n_disk=16384, seed=79)
-    want = oracle.tree(g["pos"], g["mass"], 0.05, 0.7)
-    got = ctx.tree(g["pos"], g["mass"], 0.05, 0.7, FP32)
-    e = rel_err(got, want)
-    print("fp32 tree vs fp64 tree: median %.3g p99 %.3g" % (np.median(e), np.percentile(e, 99)))
-    assert np.median(e) < FP32_TREE_MEDIAN
-    # and the approximation error of the reference algorithm itself, for the record
-    direct = oracle.direct(g["pos"], g["mass"], 0.05)
-    e2 = rel_err(got, direct)
-    print("tree (theta=0.7) vs direct: median %.3g p99 %.3g" % (np.median(e2),
-                                                               np.percentile(e2, 99)))
-    assert np.median(e2) < 1e-2
+    check_fp32_walk(ctx, oracle, g["pos"], g["mass"], 0.05, theta,
+                    what="galaxy 32768 theta %g" % theta)
+
+
+def test_fp32_walk_theta0_is_direct_sum(ctx, oracle):
+    # theta = 0 opens every internal node: every list is every particle exactly once
+    g = workloads.galaxy(n_halo=10000, n_disk=10000, seed=92)
+    got = ctx.tree(g["pos"], g["mass"], 0.05, 0.0, FP32)
+    e = rel_err(got, oracle.direct(g["pos"], g["mass"], 0.05))
+    assert np.median(e) < 1e-6 and np.percentile(e, 99) < 1e-4
 
 
 def test_tree_edge_cases(ctx):
@@ -166,25 +205,19 @@ def test_simulation_tree_device_vs_host(ctx):
     assert_bitwise(a.velocities, b.velocities)
 
 
-def test_walks_per_wave_bitwise(ctx):
-    # one or two independent walks per wavefront, static or queue-driven scheduling:
-    # every target accepts the same nodes in the same order, so the forces are
-    # bit-identical
-    g = workloads.galaxy(n_halo=20000, n_disk=20000, seed=82)
-    out = []
+@pytest.mark.parametrize("group", [1, 2, 4, 8, 16])
+def test_fp32_walk_group_sizes(ctx, oracle, group):
+    # any group size: FP32 arithmetic against the same lists in FP64, accuracy no worse
+    # than the reference tree, and repeated calls bit-identical
+    g = workloads.galaxy(n_halo=12000, n_disk=12000, seed=82)
+    ctx.set_tuning(_native.TUNE_WALK_GROUP, group)
     try:
-        for sched in (-1, 1):
-            for wpw in (1, 2):
-                ctx.set_tuning(_native.TUNE_WALK_SCHEDULE, sched)
-                ctx.set_tuning(_native.TUNE_WALKS_PER_WAVE, wpw)
-                out.append(ctx.tree(g["pos"], g["mass"], 0.05, 0.7, FP32))
-                # a second call reuses the queue counters, which must start from zero
-                out.append(ctx.tree(g["pos"], g["mass"], 0.05, 0.7, FP32))
+        a = check_fp32_walk(ctx, oracle, g["pos"], g["mass"], 0.05, 0.7,
+                            what="group %d" % group)
+        b = ctx.tree(g["pos"], g["mass"], 0.05, 0.7, FP32)
     finally:
-        ctx.set_tuning(_native.TUNE_WALKS_PER_WAVE, 0)
-        ctx.set_tuning(_native.TUNE_WALK_SCHEDULE, 0)
-    for o in out[1:]:
-        assert_bitwise(out[0], o)
+        ctx.set_tuning(_native.TUNE_WALK_GROUP, 0)
+    assert_bitwise(a, b)
 
 
 def test_fp64_walk_independent_of_target_grouping(ctx):
@@ -211,9 +244,7 @@ def test_fp32_walk_few_targets(ctx, oracle):
     mass = rng.uniform(size=20000)
     for nt in (1, 7, 300):
         fpos = rng.normal(size=(nt, 3)) * 2
-        want = oracle.tree(pos, mass, 0.01, 0.7, force_pos=fpos)
-        got = ctx.tree(pos, mass, 0.01, 0.7, FP32, force_pos=fpos)
-        assert np.median(rel_err(got, want)) < FP32_TREE_MEDIAN, nt
+        check_fp32_walk(ctx, oracle, pos, mass, 0.01, 0.7, fpos=fpos, what="%d targets" % nt)
 
 
 def test_fp32_walk_zero_softening(ctx, oracle):
@@ -222,12 +253,27 @@ def test_fp32_walk_zero_softening(ctx, oracle):
     rng = np.random.default_rng(89)
     pos = rng.normal(size=(6000, 3))
     mass = rng.uniform(1, 2, 6000)
-    want = oracle.tree(pos, mass, 0.0, 0.7)
-    got = ctx.tree(pos, mass, 0.0, 0.7, FP32)
+    got = check_fp32_walk(ctx, oracle, pos, mass, 0.0, 0.7, what="eps 0")
     assert np.all(np.isfinite(got))
-    assert np.median(rel_err(got, want)) < FP32_TREE_MEDIAN
     fpos = np.concatenate([pos[:50], rng.normal(size=(250, 3))])
-    want = oracle.tree(pos, mass, 0.0, 0.7, force_pos=fpos)
-    got = ctx.tree(pos, mass, 0.0, 0.7, FP32, force_pos=fpos)
+    got = check_fp32_walk(ctx, oracle, pos, mass, 0.0, 0.7, fpos=fpos, what="eps 0 targets")
     assert np.all(np.isfinite(got))
-    assert np.median(rel_err(got, want)) < FP32_TREE_MEDIAN
+
+
+@pytest.mark.parametrize("group", [0, 2, 16])
+def test_fp32_walk_list_lengths(ctx, group):
+    # every target's list refines the reference's accepted nodes, so it is never shorter;
+    # and the group test must not open far more than the reference (a loose group, whose
+    # sorted targets jump across the system, is walked target by target)
+    g = workloads.galaxy(n_halo=32768, n_disk=32768, seed=3)
+    _, cref = ctx.tree_interactions(g["pos"], g["mass"], 0.05, 0.7)
+    ctx.set_tuning(_native.TUNE_WALK_GROUP, group)
+    try:
+        _, c32 = ctx.tree_interactions(g["pos"], g["mass"], 0.05, 0.7, precision=FP32)
+    finally:
+        ctx.set_tuning(_native.TUNE_WALK_GROUP, 0)
+    print("group %d: list mean %.0f max %d; reference mean %.0f max %d"
+          % (group, c32.mean(), c32.max(), cref.mean(), cref.max()), flush=True)
+    assert np.all(c32 >= cref)
+    assert c32.mean() < 2.5 * cref.mean()
+    assert c32.max() < 8 * cref.max()
