@@ -470,6 +470,7 @@ int gh_adjoint_forward(gh_ctx* c, double* d_x, double* d_v, const double* d_m, i
         return fail(c, GH_EARG, "unknown precision %d", precision);
     GH_HIP(c, hipSetDevice(c->device));
     hipStream_t s = pick_stream(c, stream);
+    StreamOrder order_(c, s);
     const int64_t n3 = 3 * n;
     for (int64_t s0 = 0, seg = 0; s0 < nsteps; s0 += every, seg++) {
         GH_TRY(copy_d(c, d_x, d_ckpt + (2 * seg) * n3, n3, s));
@@ -490,6 +491,7 @@ int gh_adjoint_backward(gh_ctx* c, const double* d_ckpt, const double* d_m, int6
         return fail(c, GH_EARG, "unknown precision %d", precision);
     GH_HIP(c, hipSetDevice(c->device));
     hipStream_t s = pick_stream(c, stream);
+    StreamOrder order_(c, s);
     const int64_t n3 = 3 * n;
     GH_TRY(zero_dev(c, d_lm, n, s));
     if (nsteps == 0) return 0;

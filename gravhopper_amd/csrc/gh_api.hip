@@ -91,6 +91,25 @@ TimedScope::~TimedScope()
     c->spans.push_back({b, e, cls});
 }
 
+StreamOrder::StreamOrder(gh_ctx* c_, hipStream_t s_) : c(c_), s(s_)
+{
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &st) != hipSuccess || st != hipStreamCaptureStatusNone) return;
+    live = true;
+    if (c->last_event && c->last_stream != s) (void)hipStreamWaitEvent(s, c->last_event, 0);
+}
+
+StreamOrder::~StreamOrder()
+{
+    if (!live) return;
+    if (!c->last_event &&
+        hipEventCreateWithFlags(&c->last_event, hipEventDisableTiming) != hipSuccess) {
+        c->last_event = nullptr;
+        return;
+    }
+    if (hipEventRecord(c->last_event, s) == hipSuccess) c->last_stream = s;
+}
+
 static void release_spans(gh_ctx* c)
 {
     for (auto& sp : c->spans) {
@@ -428,6 +447,7 @@ void gh_ctx_destroy(gh_ctx* c)
     if (!c) return;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    if (c->last_event) (void)hipEventSynchronize(c->last_event);  // a call on a caller's stream
     tree_free(c);
     DevBuf* bufs[] = {&c->pos64, &c->mass64, &c->fpos64, &c->acc64, &c->src64,
                       &c->src32a, &c->src32b, &c->tgt32,  &c->xs,    &c->vs,
@@ -451,6 +471,7 @@ void gh_ctx_destroy(gh_ctx* c)
         (void)hipStreamDestroy(c->copy_stream);
     }
     if (c->pinned) (void)hipHostFree(c->pinned);
+    if (c->last_event) (void)hipEventDestroy(c->last_event);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -462,6 +483,7 @@ int gh_sync(gh_ctx* c)
     if (!c) return GH_EARG;
     GH_HIP(c, hipSetDevice(c->device));
     GH_HIP(c, hipStreamSynchronize(c->stream));
+    if (c->last_event) GH_HIP(c, hipEventSynchronize(c->last_event));
     return 0;
 }
 
@@ -471,6 +493,7 @@ int gh_direct_summation_dev(gh_ctx* c, const double* d_pos, const double* d_mass
 {
     GH_TRY(check_common(c, np, nf, eps, precision));
     GH_HIP(c, hipSetDevice(c->device));
+    StreamOrder order_(c, pick_stream(c, stream));
     return direct_dev(c, d_pos, d_mass, np, d_fpos, nf, eps, precision, d_acc,
                       pick_stream(c, stream));
 }
@@ -480,6 +503,7 @@ int gh_direct_summation(gh_ctx* c, const double* pos, const double* mass, int64_
 {
     GH_TRY(check_common(c, np, 0, eps, precision));
     GH_HIP(c, hipSetDevice(c->device));
+    StreamOrder order_(c, c->stream);
     GH_TRY(upload(c, c->pos64, pos, 3 * np));
     GH_TRY(upload(c, c->mass64, mass, np));
     GH_TRY(ensure(c, c->acc64, sizeof(double) * 3 * (size_t)(np > 0 ? np : 1)));
@@ -494,6 +518,7 @@ int gh_direct_summation_position(gh_ctx* c, const double* pos, const double* mas
 {
     GH_TRY(check_common(c, np, nf, eps, precision));
     GH_HIP(c, hipSetDevice(c->device));
+    StreamOrder order_(c, c->stream);
     GH_TRY(upload(c, c->pos64, pos, 3 * np));
     GH_TRY(upload(c, c->mass64, mass, np));
     GH_TRY(upload(c, c->fpos64, fpos, 3 * nf));
@@ -510,6 +535,7 @@ int gh_tree_force_dev(gh_ctx* c, const double* d_pos, const double* d_mass, int6
 {
     GH_TRY(check_common(c, np, nf, eps, precision));
     GH_HIP(c, hipSetDevice(c->device));
+    StreamOrder order_(c, pick_stream(c, stream));
     return tree_dev(c, d_pos, d_mass, np, d_fpos, d_fpos ? nf : np, eps, theta, precision,
                     d_acc, pick_stream(c, stream));
 }
@@ -519,6 +545,7 @@ int gh_tree_force(gh_ctx* c, const double* pos, const double* mass, int64_t np, 
 {
     GH_TRY(check_common(c, np, 0, eps, precision));
     GH_HIP(c, hipSetDevice(c->device));
+    StreamOrder order_(c, c->stream);
     GH_TRY(upload(c, c->pos64, pos, 3 * np));
     GH_TRY(upload(c, c->mass64, mass, np));
     GH_TRY(ensure(c, c->acc64, sizeof(double) * 3 * (size_t)(np > 0 ? np : 1)));
@@ -533,6 +560,7 @@ int gh_tree_force_position(gh_ctx* c, const double* pos, const double* mass, int
 {
     GH_TRY(check_common(c, np, nf, eps, precision));
     GH_HIP(c, hipSetDevice(c->device));
+    StreamOrder order_(c, c->stream);
     GH_TRY(upload(c, c->pos64, pos, 3 * np));
     GH_TRY(upload(c, c->mass64, mass, np));
     GH_TRY(upload(c, c->fpos64, fpos, 3 * nf));
@@ -552,6 +580,7 @@ int gh_run_dev(gh_ctx* c, double* d_pos, double* d_vel, const double* d_mass, in
         return fail(c, GH_EARG, "unknown algorithm %d", algorithm);
     if (nsteps < 0) return fail(c, GH_EARG, "negative step count");
     GH_HIP(c, hipSetDevice(c->device));
+    StreamOrder order_(c, pick_stream(c, stream));
     return run_steps(c, d_pos, d_vel, d_mass, np, dt, eps, algorithm, theta, precision, nsteps,
                      d_hist_pos, d_hist_vel, pick_stream(c, stream));
 }
@@ -567,6 +596,7 @@ int gh_run(gh_ctx* c, double* pos, double* vel, const double* mass, int64_t np, 
     if (np == 0 || nsteps == 0) return 0;
     GH_HIP(c, hipSetDevice(c->device));
     hipStream_t s = c->stream;
+    StreamOrder order_(c, s);
     const int64_t n3 = 3 * np;
     GH_TRY(upload(c, c->xs, pos, n3));
     GH_TRY(upload(c, c->vs, vel, n3));
@@ -627,6 +657,7 @@ int gh_run_ext(gh_ctx* c, double* pos, double* vel, const double* mass, int64_t 
     if (np == 0 || nsteps == 0) return 0;
     GH_HIP(c, hipSetDevice(c->device));
     hipStream_t s = c->stream;
+    StreamOrder order_(c, s);
     const int64_t n3 = 3 * np;
     const size_t bytes = sizeof(double) * (size_t)n3;
     if (!c->copy_stream) GH_HIP(c, hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
@@ -734,6 +765,7 @@ int gh_energy(gh_ctx* c, const double* pos, const double* vel, const double* mas
 {
     GH_TRY(check_common(c, np, 0, eps, GH_FP64));
     GH_HIP(c, hipSetDevice(c->device));
+    StreamOrder order_(c, c->stream);
     GH_TRY(upload(c, c->pos64, pos, 3 * np));
     GH_TRY(upload(c, c->vs, vel, 3 * np));
     GH_TRY(upload(c, c->mass64, mass, np));

@@ -141,6 +141,7 @@ int gh_ensemble_create(gh_ctx* c, int n_sims, const int64_t* counts, const doubl
     if (ipt != 1 && ipt != 2 && ipt != 4)
         return fail(c, GH_EARG, "particles per lane must be 1, 2 or 4 (got %d)", ipt);
     GH_HIP(c, hipSetDevice(c->device));
+    StreamOrder order_(c, c->stream);
     gh_ensemble* e = new gh_ensemble();
     e->c = c;
     e->nsims = n_sims;
@@ -260,6 +261,7 @@ int gh_ensemble_run(gh_ensemble* e, int64_t nsteps, int use_graph)
     if (nsteps == 0) return 0;
     GH_HIP(c, hipSetDevice(c->device));
     hipStream_t s = c->stream;
+    StreamOrder order_(c, s);
     hipLaunchKernelGGL(k_ens_half_drift, dim3(blocks_for(e->n_pad, kBlock)), dim3(kBlock), 0, s,
                        e->x.as<double>(), e->v.as<double>(), e->n_pad, e->blk,
                        e->valid_blk.as<int>(), e->dt_blk.as<double>(), e->xh.as<double>());
@@ -291,6 +293,7 @@ int gh_ensemble_get(gh_ensemble* e, double* pos, double* vel)
     gh_ctx* c = e->c;
     GH_HIP(c, hipSetDevice(c->device));
     hipStream_t s = c->stream;
+    StreamOrder order_(c, s);
     int64_t off = 0;
     for (int k = 0; k < e->nsims; k++) {
         const int64_t st = e->start[k], n = e->count[k];

@@ -72,6 +72,11 @@ struct gh_ctx {
     int adj_nb = -1, adj_ipt = -1;
     gh::DevBuf slot;  // history slot counter of graph-replayed runs
     gh::OrderBufs ord_p, ord_t;  // spatial order of sources and of targets (FP32 paths)
+    // the stream of the last entry-point call and an event recorded at its end: a call
+    // on another stream waits for it first (StreamOrder), since all calls share the
+    // context's buffers
+    hipStream_t last_stream = nullptr;
+    hipEvent_t last_event = nullptr;
     // gh_run_ext: copy stream, pinned host staging (x_half, v, external acceleration)
     // and the device copy of the external acceleration
     hipStream_t copy_stream = nullptr;
@@ -124,6 +129,17 @@ struct TimedScope {
     hipEvent_t b = nullptr;
     TimedScope(gh_ctx* c_, hipStream_t s_, int cls_);
     ~TimedScope();
+};
+// Orders an entry point's work on stream s after the previous call's, whatever stream
+// that one used (the _dev entry points take the caller's stream, the others the
+// context's).  Construct after selecting the device; the destructor records the end of
+// this call.  Inactive on a stream under graph capture.
+struct StreamOrder {
+    gh_ctx* c;
+    hipStream_t s;
+    bool live = false;
+    StreamOrder(gh_ctx* c_, hipStream_t s_);
+    ~StreamOrder();
 };
 int check_launch(gh_ctx* c, const char* what);
 int zero_dev(gh_ctx* c, double* p, int64_t n, hipStream_t s);

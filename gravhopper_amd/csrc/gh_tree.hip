@@ -880,6 +880,11 @@ __device__ __forceinline__ void walk_lists(
         const float R = (float)rec.w * inv_theta;  // +inf for theta = 0
         const float R2 = R * R;
         if (d2 > R2) return true;
+        // every target lies within the box's far corner distance of the centre: if that is
+        // within R too, no target can pass
+        const float fx = fabsf(cx - bcx) + bhx, fy = fabsf(cy - bcy) + bhy,
+                    fz = fabsf(cz - bcz) + bhz;
+        if (!(__builtin_fmaf(fx, fx, __builtin_fmaf(fy, fy, fz * fz)) > R2)) return false;
         for (int i = 0; i < cnt; i++) {
             const float4 t = tg[i];
             const float ex = cx - t.x, ey = cy - t.y, ez = cz - t.z;
@@ -887,12 +892,14 @@ __device__ __forceinline__ void walk_lists(
         }
         return true;
     };
-    // the list entry of node n: centre of mass and mass, re-based to the frame
-    auto entry_of = [&](int n) -> Entry {
-        const double4 m = *(const double4*)((const double*)(nodes + n) + 4);
+    // the list entry of a node: centre of mass and mass, re-based to the frame
+    auto entry_of = [&](const double4 m) -> Entry {
         Entry e;
         e.x = m.x - ox; e.y = m.y - oy; e.z = m.z - oz; e.w = m.w;
         return e;
+    };
+    auto com_of = [&](int n) -> double4 {
+        return *(const double4*)((const double*)(nodes + n) + 4);
     };
 
     __syncthreads();  // the previous walk of this wavefront is done with stack and list
@@ -911,7 +918,7 @@ __device__ __forceinline__ void walk_lists(
             while (p < end) {
                 const double4 rec = *(const double4*)(nodes + p);
                 if (accepts(rec)) {
-                    if (lane == 0) list[(head + held) & (kListCap - 1)] = entry_of(p);
+                    if (lane == 0) list[(head + held) & (kListCap - 1)] = entry_of(com_of(p));
                     held += 1;
                     listed += 1;
                     p = skip[p];
@@ -932,8 +939,11 @@ __device__ __forceinline__ void walk_lists(
         bool acc_n = false;
         int4 ka = make_int4(-1, -1, -1, -1), kb = make_int4(-1, -1, -1, -1);
         int nk = 0;
+        double4 com = make_double4(0.0, 0.0, 0.0, 0.0);
         if (active) {
+            // the whole 64-byte record in one load: geometry and moments
             const double4 rec = *(const double4*)(nodes + n);
+            com = com_of(n);
             acc_n = accepts(rec);
             if (!acc_n) {
                 ka = *(const int4*)(kids + 8 * (int64_t)n);
@@ -945,7 +955,7 @@ __device__ __forceinline__ void walk_lists(
         // appends to the list, in lane order
         const unsigned long long am = __ballot(acc_n);
         Entry e;
-        if (acc_n) e = entry_of(n);
+        if (acc_n) e = entry_of(com);
         const int before = (int)__popcll(am & ((1ull << lane) - 1ull));
         const int added = (int)__popcll(am);
         // pushes: the popped ids' slots are reused, children in lane order
@@ -987,7 +997,7 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_group(
     typedef typename GroupEntry<F64>::T Entry;
     __shared__ int stack[kStackCap];
     __shared__ Entry list[kListCap];
-    __shared__ float4 tg[16];
+    __shared__ float4 tg[64];
     const int lane = threadIdx.x;
     const int64_t first = xcd_block(per_xcd) * G;
     if (first >= nt) return;
@@ -1022,8 +1032,8 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_group(
     // consecutive targets (max norm), through binary exponents.  A group whose sorted
     // targets jump between distant systems has targets far from its frame, whose FP32
     // coordinates would then carry 2^-24 times that distance; such a group is walked
-    // target by target instead.  (With 16 targets one long jump among 15 gaps moves the
-    // geometric mean little; groups of 2 to 8 are not checked.)
+    // target by target instead.  (With 16 or more targets one long jump moves the
+    // geometric mean of the gaps little; groups of 2 to 8 are not checked.)
     const float nx = __shfl(tx, (lane + 1) & 63), ny = __shfl(ty, (lane + 1) & 63),
                 nz = __shfl(tz, (lane + 1) & 63);
     int eg = lane < cnt - 1 && lane + 1 < G
@@ -1031,7 +1041,7 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_group(
                  : 0;
     for (int s = 32; s > 0; s >>= 1) eg += __shfl_xor(eg, s);
     const int ext = exponent_f(fmaxf(hix - lox, fmaxf(hiy - loy, hiz - loz)));
-    const bool loose = G == 16 && cnt > 1 && ext * (cnt - 1) - eg > kLooseLog2 * (cnt - 1);
+    const bool loose = G >= 16 && cnt > 1 && ext * (cnt - 1) - eg > kLooseLog2 * (cnt - 1);
 
     if (!loose) {
         // the group's targets in the frame, for the per-target test
@@ -1461,7 +1471,7 @@ static int walk(gh_ctx* c, const double* tgt, bool self, int64_t nt, double eps,
     // there are too few targets for kWalkUnits groups of 16 (then every target walks in
     // its own frame with the reference's opening test)
     int G = c->tune[GH_TUNE_WALK_GROUP];
-    if (!(G == 1 || G == 2 || G == 4 || G == 8 || G == 16))
+    if (!(G == 1 || G == 2 || G == 4 || G == 8 || G == 16 || G == 32 || G == 64))
         G = (nt >= 16 * kWalkUnits) ? 16 : 1;
     const bool f64eval = c->tune[GH_TUNE_WALK_F64EVAL] > 0;
     const bool guard = fp32_guard(eps);
@@ -1530,6 +1540,7 @@ extern "C" int gh_tree_dump(gh_ctx* c, const double* pos, const double* mass, in
     if (np < 1) return fail(c, GH_EARG, "the tree needs at least one particle");
     GH_HIP(c, hipSetDevice(c->device));
     hipStream_t s = c->stream;
+    StreamOrder order_(c, s);
     GH_TRY(ensure(c, c->pos64, sizeof(double) * 3 * (size_t)np));
     GH_TRY(ensure(c, c->mass64, sizeof(double) * (size_t)np));
     GH_HIP(c, hipMemcpyAsync(c->pos64.p, pos, sizeof(double) * 3 * (size_t)np,
@@ -1581,6 +1592,7 @@ static int tree_counts(gh_ctx* c, const double* pos, const double* mass, int64_t
     const int64_t nt = self ? np : nf;
     GH_HIP(c, hipSetDevice(c->device));
     hipStream_t s = c->stream;
+    StreamOrder order_(c, s);
     GH_TRY(ensure(c, c->pos64, sizeof(double) * 3 * (size_t)np));
     GH_TRY(ensure(c, c->mass64, sizeof(double) * (size_t)np));
     GH_TRY(ensure(c, c->acc64, sizeof(double) * 3 * (size_t)(nt > 0 ? nt : 1)));

@@ -16,6 +16,11 @@
  *   - "_dev" entry points take device pointers and a hipStream_t (passed as void*;
  *     NULL selects the context's stream) and return without synchronising.  The
  *     others take host pointers and return when the result is in host memory.
+ *   - Calls on one context run in call order whatever streams they use: a call whose
+ *     stream differs from the previous call's first makes its stream wait for the end
+ *     of that call (an event), since both use the context's arenas.  On a stream under
+ *     graph capture this ordering is left to the caller.  gh_sync waits for the
+ *     context's stream and for the last call.
  */
 #ifndef GRAVHOP_H
 #define GRAVHOP_H
@@ -215,8 +220,9 @@ int gh_timing_read(gh_ctx *ctx, int kernel_class, double *total_ms, int64_t *lau
  *   GH_TUNE_DIRECT_SELF  FP32 self field: 0 automatic, -1 one-sided kernel,
  *                        1/2/4 pair-once kernel with that many particles per lane
  *   GH_TUNE_DIRECT_IPT   FP32 one-sided kernel: 0 automatic, 1/2/4 targets per lane
- *   GH_TUNE_WALK_GROUP   FP32 tree walk: 0 automatic, 1/2/4/8/16 targets per group
- *                        (the targets that share one opening test and interaction list)
+ *   GH_TUNE_WALK_GROUP   FP32 tree walk: 0 automatic, 1/2/4/8/16/32/64 targets per
+ *                        group (the targets that share one opening test and interaction
+ *                        list)
  *   GH_TUNE_RUN_GRAPH    direct-sum gh_run / gh_run_dev: 0 replay two captured steps as
  *                        one HIP graph (from 4 steps on), -1 launch every kernel eagerly
  *   GH_TUNE_WALK_F64EVAL FP32 tree walk: 1 evaluates the same interaction lists in FP64
