@@ -13,7 +13,8 @@ At N = 2^23 the checks are properties that do not depend on size:
   monopole of all particles (closed form);
 - bitwise repeatability.
 The FP32 pair-once direct sum runs at N = 2^20 against FP64 direct sums on sampled
-targets and the balance of pair forces.
+targets and the balance of pair forces.  The 128-sphere ensemble runs at full size:
+eight spheres bitwise against single runs, two against the FP64 oracle.
 """
 import numpy as np
 import pytest
@@ -126,3 +127,37 @@ def test_direct_fp32_pair_once_1m(ctx):
     scale = (mass[:, None] * np.abs(a)).sum(axis=0)
     print("net force / sum of magnitudes:", net / scale)
     assert np.all(np.abs(net) < 1e-6 * scale)
+
+
+def test_ensemble_config4_full_size(ctx, oracle):
+    # the ensemble configuration itself: 128 Plummer spheres, N log-uniform in
+    # [1024, 65536], advanced two steps as one graph-replayed persistent batch
+    sims = workloads.ensemble(seed=4)
+    ipt = 4
+    print("ensemble: %d sims, %d particles" % (len(sims), sum(len(s["mass"]) for s in sims)),
+          flush=True)
+    ens = _native.Ensemble(ctx, sims, ipt=ipt, persistent=True)
+    ens.run(2, graph=True)
+    pos, vel = ens.state()
+    ens.close()
+    order = np.argsort([len(s["mass"]) for s in sims])
+    # 8 sims across the size range, bitwise against single runs of the same kernel
+    pick = order[np.linspace(0, len(sims) - 1, 8).round().astype(int)]
+    ctx.set_tuning(_native.TUNE_DIRECT_SELF, ipt)
+    try:
+        for k in pick:
+            s = sims[k]
+            x, v = ctx.run(s["pos"], s["vel"], s["mass"], s["dt"], s["eps"], 2, precision=FP32)
+            assert_bitwise(pos[k], x, "sim %d (N=%d) pos" % (k, len(s["mass"])))
+            assert_bitwise(vel[k], v, "sim %d (N=%d) vel" % (k, len(s["mass"])))
+    finally:
+        ctx.set_tuning(_native.TUNE_DIRECT_SELF, 0)
+    # the smallest and a middle-sized sim against the FP64 oracle
+    for k in (order[0], order[len(order) // 2]):
+        s = sims[k]
+        xo, _ = oracle.dkd_direct(s["pos"], s["vel"], s["mass"], s["dt"], s["eps"], 2)
+        disp = np.linalg.norm(xo - s["pos"], axis=1)
+        err = np.linalg.norm(pos[k] - xo, axis=1) / disp
+        print("sim %d (N=%d): median %.2e p99 %.2e" % (k, len(s["mass"]), np.median(err),
+                                                      np.percentile(err, 99)), flush=True)
+        assert np.median(err) < 1e-5, k

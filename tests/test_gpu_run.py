@@ -135,6 +135,39 @@ def test_simulation_external_forces(ctx):
                                np.cos(big_omega / 2), atol=2e-3)
 
 
+def test_dkd_map_powers_harmonic(ctx):
+    # One particle (no N-body force) in a harmonic external field a = -K x, started
+    # off the axis with a velocity.  With u = v D (kpc/Myr) and w^2 = K D, one DKD step
+    # (gravhopper.py:405-416) is the linear map
+    #   M = [[1 - (wh)^2/2, h (1 - (wh)^2/4)], [-w^2 h, 1 - (wh)^2/2]]
+    # per coordinate, det M = 1, so M^n = (sin(n p) M - sin((n-1) p) I) / sin p with
+    # cos p = 1 - (wh)^2/2.  Every snapshot must equal (x0, u0) M^n to 1e-12 of the
+    # amplitude: a kick and drift in the other order, a dropped half-drift or the
+    # force taken at x instead of the half-step position all fail by far more.
+    D = 1.022712165045695e-03
+    w, h, n = 0.05, 1.0, 300
+    K = w * w / D
+    x0 = np.array([[1.0, -0.5, 0.3]])
+    v0 = np.array([[30.0, 10.0, -20.0]])
+    sim = Simulation(dt=h, eps=0.1, algorithm='direct')
+    sim.add_IC({'pos': x0, 'vel': v0, 'mass': np.ones(1)})
+    sim.add_external_force(lambda pos, args: -K * pos)
+    sim.run(n)
+    x, v = sim.positions[:, 0, :], sim.velocities[:, 0, :]
+    M = np.array([[1 - (w * h) ** 2 / 2, h * (1 - (w * h) ** 2 / 4)],
+                  [-w * w * h, 1 - (w * h) ** 2 / 2]])
+    p = np.arccos(M[0, 0])
+    steps = np.arange(n + 1)
+    s1 = np.sin(steps * p)[:, None, None]
+    s0 = np.sin((steps - 1) * p)[:, None, None]
+    Mn = (s1 * M - s0 * np.eye(2)) / np.sin(p)                   # (n+1, 2, 2)
+    z0 = np.stack([x0[0], v0[0] * D])                            # (2, 3): x and u rows
+    z = np.einsum("nij,jk->nik", Mn, z0)                         # (n+1, 2, 3)
+    amp = np.sqrt(z0[0] ** 2 + (z0[1] / w) ** 2)                 # per coordinate
+    assert np.all(np.abs(x - z[:, 0]) <= 1e-12 * amp), np.abs(x - z[:, 0]).max()
+    assert np.all(np.abs(v * D - z[:, 1]) <= 1e-12 * amp * w), np.abs(v * D - z[:, 1]).max()
+
+
 def test_simulation_fp32_close_to_fp64(ctx):
     ic = workloads.plummer(4096, seed=34)
     res = {}
