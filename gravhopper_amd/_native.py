@@ -66,6 +66,7 @@ SIGNATURES = {
     "gh_adjoint_backward": ([_P, _P, _P, _I64, _D, _D, _I, _I64, _I64, _P, _P, _P, _P], _I),
     "gh_timing_read": ([_P, _I, ctypes.POINTER(_D), ctypes.POINTER(_I64)], _I),
     "gh_energy": ([_P, _P, _P, _P, _I64, _D, _P, _P, _P], _I),
+    "gh_energy_dev_noalloc": ([_P, _P, _P, _I64, _D, _P, _P], _I),
 }
 
 # gh_set_tuning keys

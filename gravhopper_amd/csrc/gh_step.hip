@@ -125,21 +125,30 @@ __device__ __forceinline__ double block_sum(double v, double* sh)
     return r;
 }
 
+// PACKED: particles as (x, y, z, m) records (src); otherwise read in place from the
+// (n,3) positions and the masses.  The arithmetic is the same either way.
+template <bool PACKED>
 __global__ __launch_bounds__(kBlock) void k_energy_partials(const double4* __restrict__ src,
+                                                            const double* __restrict__ pos,
+                                                            const double* __restrict__ mass,
                                                             const double* __restrict__ vel,
                                                             int64_t n, double eps2,
                                                             double* __restrict__ partial)
 {
     __shared__ double4 tile[kBlock];
     __shared__ double red[kBlock];
+    auto load = [&](int64_t j) -> double4 {
+        if (PACKED) return src[j];
+        return make_double4(pos[3 * j], pos[3 * j + 1], pos[3 * j + 2], mass[j]);
+    };
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const bool live = i < n;
     double4 me = make_double4(0, 0, 0, 0);
-    if (live) me = src[i];
+    if (live) me = load(i);
     double row = 0.0;
     for (int64_t base = 0; base < n; base += kBlock) {
         __syncthreads();
-        if (base + threadIdx.x < n) tile[threadIdx.x] = src[base + threadIdx.x];
+        if (base + threadIdx.x < n) tile[threadIdx.x] = load(base + threadIdx.x);
         __syncthreads();
         const int cnt = (int)((n - base) < kBlock ? (n - base) : kBlock);
         if (!live) continue;
@@ -197,8 +206,9 @@ int energy_dev(gh_ctx* c, const double* d_pos, const double* d_vel, const double
     if (n > 0) {
         hipLaunchKernelGGL(k_pack_src64_e, dim3(nb), dim3(kBlock), 0, s, d_pos, d_mass, n, src);
         GH_TRY(check_launch(c, "k_pack_src64_e"));
-        hipLaunchKernelGGL(k_energy_partials, dim3(nb), dim3(kBlock), 0, s, src, d_vel, n,
-                           eps * eps, part);
+        hipLaunchKernelGGL(k_energy_partials<true>, dim3(nb), dim3(kBlock), 0, s, src,
+                           (const double*)nullptr, (const double*)nullptr, d_vel, n, eps * eps,
+                           part);
         GH_TRY(check_launch(c, "k_energy_partials"));
     }
     hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(kBlock), 0, s, part, nb, d_out5);
@@ -206,3 +216,52 @@ int energy_dev(gh_ctx* c, const double* d_pos, const double* d_vel, const double
 }
 
 }  // namespace gh
+
+// ------------------------------------------------------------------------------
+// allocation-free diagnostics (gh_energy_dev_noalloc)
+// ------------------------------------------------------------------------------
+// The per-block partial sums live in this file-scope device array instead of an arena:
+// kNoAllocBlocks blocks of 256 particles, i.e. up to 4194304 particles, 640 KiB held on
+// every device the library runs on until the process ends.  Host code never passes the
+// array's name to a kernel: each device's address of it comes from hipGetSymbolAddress,
+// once per device, and is kept in g_scratch[device].
+namespace gh {
+namespace {
+constexpr int64_t kNoAllocBlocks = 16384;
+constexpr int kNoAllocDevices = 64;
+__device__ __align__(256) double scratch[kNoAllocBlocks * kDiag];
+double* g_scratch[kNoAllocDevices] = {nullptr};
+}  // namespace
+
+}  // namespace gh
+
+using namespace gh;
+
+extern "C" int gh_energy_dev_noalloc(const double* d_pos, const double* d_vel,
+                                     const double* d_mass, int64_t np, double eps,
+                                     double* d_out5, void* stream)
+{
+    if (np < 0 || np > kNoAllocBlocks * kBlock || !d_out5) return GH_EARG;
+    if (np > 0 && (!d_pos || !d_vel || !d_mass)) return GH_EARG;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kNoAllocDevices) return GH_EHIP;
+    double* part = __atomic_load_n(&g_scratch[dev], __ATOMIC_ACQUIRE);
+    if (!part) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return GH_EHIP;
+        part = static_cast<double*>(p);
+        __atomic_store_n(&g_scratch[dev], part, __ATOMIC_RELEASE);
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int64_t nb = blocks_for(np, kBlock);
+    (void)hipGetLastError();  // a launch error below is this call's, not an earlier one
+    if (np > 0) {
+        // every partial slot read below (blocks 0 .. nb-1) is written first, so the
+        // previous call's contents never show
+        hipLaunchKernelGGL(k_energy_partials<false>, dim3((unsigned)nb), dim3(kBlock), 0, s,
+                           (const double4*)nullptr, d_pos, d_mass, d_vel, np, eps * eps, part);
+        if (hipGetLastError() != hipSuccess) return GH_EHIP;
+    }
+    hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(kBlock), 0, s, part, nb, d_out5);
+    return hipGetLastError() == hipSuccess ? GH_OK : GH_EHIP;
+}

@@ -258,6 +258,16 @@ int gh_energy(gh_ctx *ctx, const double *pos, const double *vel, const double *m
               int64_t np, double eps, double *kinetic, double *potential,
               double *momentum);
 
+/* gh_energy on device pointers that allocates nothing: no hipMalloc, no context (no
+ * reference counterpart; for callers that cannot allocate, such as code under HIP graph
+ * capture or serving loops that forbid it).  Runs on the current device and on stream
+ * (NULL: the default stream); out5 (device, 5 doubles) receives kinetic, potential and
+ * the 3 momentum components, bitwise gh_energy's.  Its per-block partial sums live in a
+ * fixed device array of the library, 640 KiB per device: np is at most 4194304
+ * (GH_EARG beyond).  Calls that overlap on one device race on that array. */
+int gh_energy_dev_noalloc(const double *d_pos, const double *d_vel, const double *d_mass,
+                          int64_t np, double eps, double *d_out5, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
