@@ -1,9 +1,10 @@
 """Drop-in for `gravhopper.Simulation`: the N-body driver of the hot path.
 
-Same public surface as gravhopper/gravhopper.py:91-473 and :853-904 (constructor,
-set_/get_ dt, eps and algorithm, add_IC, run, init_run, reset, snap, current_snap,
-prev_snap, perform_timestep, calculate_acceleration, calculate_extra_acceleration and
-the add_external_*force registrations), with the same drift-kick-drift leapfrog.
+Same public surface as gravhopper/gravhopper.py:91-473, :853-904 and :958-1162
+(constructor, set_/get_ dt, eps and algorithm, add_IC, run, init_run, reset, snap,
+current_snap, prev_snap, perform_timestep, calculate_acceleration,
+calculate_extra_acceleration, the add_external_*force registrations, plot_particles and
+movie_particles), with the same drift-kick-drift leapfrog.
 
 What changes is where the steps run:
   - without external forces, run(N) hands the whole loop to the GPU (gh_run): the
@@ -381,6 +382,122 @@ class Simulation(object):
             raise ExternalPackageException(
                 "External forces must be plain callables; galpy, gala and agama "
                 "potential objects are not supported by this engine.")
+
+    # -- plots and movies (gravhopper.py:958-1062, 1065-1088, 1091-1162) -----------
+    def _plot_series(self, parm):
+        if parm == 'pos':
+            return self.positions, 'kpc'
+        if parm == 'vel':
+            return self.velocities, 'km/s'
+        raise ValueError("parm must be 'pos' or 'vel'")
+
+    @staticmethod
+    def _in_unit(data, unit, default):
+        """Plain values of data in `unit`.  With astropy, any unit data converts to;
+        without it, data are plain numbers in the internal unit, so only that unit (or
+        None) is accepted."""
+        if is_quantity(data):
+            return data.to(unit).value
+        if unit is not None and str(unit) != default:
+            raise ValueError("without astropy, data can only be plotted in %s" % default)
+        return np.asarray(data)
+
+    def _plot_xy(self, snapnum, sl=None):
+        """(x, y) of the particles in the stored plot settings at snapshot(s) snapnum."""
+        pp = self._plot_parms
+        data, default = self._plot_series(pp['data_parm'])
+        lo, hi = pp['particle_range']
+        rows = data[snapnum, lo:hi] if sl is None else data[:, lo:hi]
+        x = self._in_unit(rows[..., pp['xindex']], pp['unit'], default)
+        y = self._in_unit(rows[..., pp['yindex']], pp['unit'], default)
+        return x, y
+
+    def plot_particles(self, parm='pos', coords='xy', snap='final', xlim=None, ylim=None,
+                       s=0.2, unit=None, ax=None, timeformat='{0:.1f}', nolabels=False,
+                       particle_range=None, **kwargs):
+        """Scatter plot of one snapshot's positions ('pos') or velocities ('vel') in the
+        projection coords ('xy', 'xz', 'yz', ...).  snap: a snapshot number, 'final'
+        or 'IC'.  unit: the unit to plot in (default: that of the data, kpc or km/s).
+        particle_range: (first, end) indices to plot.  timeformat: format of the title
+        (the snapshot's time), False for none.  Extra keywords go to ax.scatter.
+        Returns the PathCollection (gravhopper.py:958-1062)."""
+        import matplotlib.pyplot as plt
+
+        data, default = self._plot_series(parm)
+        if len(coords) != 2:
+            raise ValueError("coords must have length 2")
+        if coords[0] not in 'xyz' or coords[1] not in 'xyz':
+            raise ValueError("coords characters must be x, y, or z")
+        if ax is None:
+            ax = plt.subplot(111, aspect=1.0)
+        snapnum = self.timestep if snap == 'final' else 0 if snap == 'IC' else snap
+        if unit is None:
+            unit = data.unit if is_quantity(data) else default
+        if particle_range is None:
+            particle_range = (0, self.Np)
+        self._plot_parms = {'data_parm': parm, 'particle_range': particle_range,
+                            'xindex': 'xyz'.index(coords[0]),
+                            'yindex': 'xyz'.index(coords[1]), 'unit': unit}
+        x, y = self._plot_xy(snapnum)
+        output = ax.scatter(x, y, s=s, **kwargs)
+        if xlim is not None:
+            ax.set_xlim(xlim)
+        if ylim is not None:
+            ax.set_ylim(ylim)
+        if not nolabels:
+            pre = '' if parm == 'pos' else 'v_'
+            ax.set_xlabel('${0}{1}$ ({2})'.format(pre, coords[0], unit))
+            ax.set_ylabel('${0}{1}$ ({2})'.format(pre, coords[1], unit))
+        self._plot_particles_settitle(ax, snapnum, timeformat)
+        return output
+
+    def _plot_particles_settitle(self, ax, snapnum, timeformat=False):
+        """Titles ax with the time of snapshot snapnum in timeformat (False: no title)."""
+        if timeformat is not False:
+            ax.set_title(timeformat.format(self.times[snapnum]))
+
+    def _plot_particles_setoffsets(self, scatterplot, snapnum):
+        """Moves the points of a plot_particles() scatter plot to snapshot snapnum."""
+        x, y = self._plot_xy(snapnum)
+        scatterplot.set_offsets(np.column_stack([x, y]))
+
+    def movie_particles(self, fname, fps=25, ax=None, skip=None, timeformat='{0:.1f}',
+                        *args, **kwargs):
+        """Writes an animation of plot_particles() over the snapshots to fname: every
+        skip-th snapshot from the ICs, fps frames per second.  Unless xlim / ylim are
+        given, the axes span every value the plotted particles take over the run.  With
+        ax None a new figure is made and closed afterwards.  Other arguments go to
+        plot_particles (gravhopper.py:1091-1162)."""
+        import matplotlib.pyplot as plt
+        from matplotlib.animation import FuncAnimation
+
+        if ax is None:
+            fig = plt.figure()
+            ax = fig.add_subplot(111, aspect=1.0)
+            close_plot = True
+        else:
+            fig = ax.get_figure()
+            close_plot = False
+        skip = 1 if skip is None else skip
+        particles = self.plot_particles(*args, ax=ax, snap='IC', timeformat=timeformat,
+                                        **kwargs)
+        xall, yall = self._plot_xy(None, sl=True)
+        if 'xlim' not in kwargs:
+            ax.set_xlim([np.min(xall), np.max(xall)])
+        if 'ylim' not in kwargs:
+            ax.set_ylim([np.min(yall), np.max(yall)])
+
+        def animate(frame):
+            snapnum = frame * skip
+            self._plot_particles_setoffsets(particles, snapnum)
+            self._plot_particles_settitle(ax, snapnum, timeformat)
+            return particles
+
+        anim = FuncAnimation(fig, animate, frames=(self.timestep + 1) // skip,
+                             interval=1000 / fps)
+        anim.save(fname)
+        if close_plot:
+            plt.close(fig)
 
     # -- initial conditions (gravhopper.py:848-904) --------------------------------
     def nrows(self, array):
