@@ -28,6 +28,8 @@ PEAK_FP32_TFLOPS = 157.3      # MI355X published FP32 peak
 PEAK_HBM_GBS = 8000.0         # MI355X published HBM3E bandwidth
 FLOP_PER_PAIR = 20            # 3 sub, 3 mul + 3 add (r^2 + eps^2), rsqrt = 1, 2 mul (r^-3),
                               # 1 mul (m r^-3), 3 FMA = 6
+PEAK_VALU_LANE_OPS = PEAK_FP32_TFLOPS * 1e12 / 2  # lane instructions/s (an FMA is 2 FLOP)
+G_KMS2_KPC_MSUN = 4.398600412921223e-09 / 1.022712165045695e-03  # G in (km/s)^2 kpc / Msun
 N_DIRECT = 262144
 EPS_DIRECT = 1e-5             # kpc (0.01 pc)
 DT_DIRECT = 0.001             # Myr
@@ -49,6 +51,27 @@ def traffic_from_profiles(name):
             return json.load(f).get("hbm_bytes_per_launch")
     except (OSError, ValueError):
         return None
+
+
+def counters_from_profiles(name):
+    """Per-launch counters of the headline kernel measured with rocprofv3 --pmc,
+    committed under profiles/ (None when absent)."""
+    p = os.path.join(ROOT, "profiles", name)
+    try:
+        with open(p) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
+
+
+def rel_err_stats(got, want):
+    e = np.linalg.norm(got - want, axis=1) / np.linalg.norm(want, axis=1)
+    return float(np.median(e)), float(np.percentile(e, 99))
+
+
+def total_energy(ctx, pos, vel, mass, eps):
+    ke, pe, _ = ctx.energy(pos, vel, mass, eps)
+    return ke + G_KMS2_KPC_MSUN * pe
 
 
 def host_description():
@@ -222,7 +245,7 @@ def bench_tree(ctx, torch, steps, warmup, lite=False):
     mass = g["mass"]
     # total-energy drift over the warmup and timed steps (FP64 pair sums)
     ke1, pe1, _ = ctx.energy(pos, v.cpu().numpy(), mass, eps)
-    G = 4.398600412921223e-09 / 1.022712165045695e-03  # (km/s)^2 kpc / Msun
+    G = G_KMS2_KPC_MSUN
     e0, e1 = ke0 + G * pe0, ke1 + G * pe1
     out["energy_rel_drift"] = abs(e1 - e0) / abs(e0)
     out["energy_drift_steps"] = warmup + steps
@@ -267,6 +290,15 @@ def bench_tree(ctx, torch, steps, warmup, lite=False):
                       "force_rel_err_median": float(np.median(e)),
                       "force_rel_err_p99": float(np.percentile(e, 99))})
     out["theta_sweep"] = sweep
+    # total-energy drift over the configuration's 50 steps from the initial state
+    xs = torch.tensor(g["pos"], dtype=torch.float64, device=dev)
+    vs = torch.tensor(g["vel"], dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    ctx.run_dev(xs.data_ptr(), vs.data_ptr(), m.data_ptr(), n, dt, eps, 50,
+                algorithm=TREE, theta=theta, precision=FP32)
+    ctx.sync()
+    e50 = total_energy(ctx, xs.cpu().numpy(), vs.cpu().numpy(), mass, eps)
+    out["energy_rel_drift_50_steps"] = abs(e50 - e0) / abs(e0)
     # the FP64 walk (the default precision of Simulation): the reference's accepted
     # nodes, FP64 monopoles
     xs = torch.tensor(pos, dtype=torch.float64, device=dev)
@@ -306,6 +338,7 @@ def bench_ensemble(ctx, torch, steps):
     ens.run(steps, graph=True)
     ctx.sync()
     el = time.perf_counter() - t0
+    pos_end, vel_end = ens.state()
     ens.close()
     pairs = info["pairs_per_step"]
     out = {"workload": "128 Plummer spheres, N log-uniform [1024, 65536], eps=0.02 b, "
@@ -337,6 +370,27 @@ def bench_ensemble(ctx, torch, steps):
     out["sequential_sim_step_ms_p50"] = float(np.percentile(per, 50) * 1e3)
     out["sequential_sim_step_ms_p99"] = float(np.percentile(per, 99) * 1e3)
     out["speedup_over_sequential"] = float(per.sum()) / (el / steps)
+    # accuracy: every sphere advances bitwise as the single-sphere FP32 path does (GPU
+    # test), so its force error is that path's against the FP64 kernel, on 8 spheres
+    # across the size range; energy drift of every sphere over the 1 + steps steps
+    order = np.argsort([len(s["mass"]) for s in sims])
+    errs = []
+    for k in order[np.linspace(0, len(sims) - 1, 8).round().astype(int)]:
+        s = sims[k]
+        a32 = ctx.direct(s["pos"], s["mass"], s["eps"], _native.GH_FP32)
+        a64 = ctx.direct(s["pos"], s["mass"], s["eps"], _native.GH_FP64)
+        errs.append(np.linalg.norm(a32 - a64, axis=1) / np.linalg.norm(a64, axis=1))
+    e = np.concatenate(errs)
+    out["force_rel_err_median"] = float(np.median(e))
+    out["force_rel_err_p99"] = float(np.percentile(e, 99))
+    drift = []
+    for k, s in enumerate(sims):
+        e0 = total_energy(ctx, s["pos"], s["vel"], s["mass"], s["eps"])
+        e1 = total_energy(ctx, pos_end[k], vel_end[k], s["mass"], s["eps"])
+        drift.append(abs(e1 - e0) / abs(e0))
+    out["energy_rel_drift_median"] = float(np.median(drift))
+    out["energy_rel_drift_max"] = float(np.max(drift))
+    out["energy_drift_steps"] = 1 + steps
     return out
 
 
@@ -369,7 +423,7 @@ def bench_adjoint(ctx, torch, steps=200, every=20):
     x0.grad = v0.grad = m.grad = None
     ctx.timing_enable(True)
     t0 = time.perf_counter()
-    x, _ = leapfrog(x0, v0, m, dt, eps, steps, checkpoint_every=every)
+    x, v_end = leapfrog(x0, v0, m, dt, eps, steps, checkpoint_every=every)
     loss = ((x - xt) ** 2).sum(-1).mean()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
@@ -380,15 +434,53 @@ def bench_adjoint(ctx, torch, steps=200, every=20):
     f_ms, f_n = ctx.timing_read(_native.T_DIRECT_F32)
     ctx.timing_enable(False)
     fwd, bwd = (t1 - t0) / steps, (t2 - t1) / steps
+    # accuracy: FP32 force against the FP64 kernel at the initial state, energy drift of
+    # the forward run, and the gradients against the same adjoint in FP64
+    f_med, f_p99 = rel_err_stats(ctx.direct(ic["pos"], ic["mass"], eps, _native.GH_FP32),
+                                 ctx.direct(ic["pos"], ic["mass"], eps, _native.GH_FP64))
+    e0 = total_energy(ctx, ic["pos"], ic["vel"], ic["mass"], eps)
+    e1 = total_energy(ctx, x.detach().cpu().numpy(), v_end.detach().cpu().numpy(), ic["mass"],
+                      eps)
+    g32 = [t.grad.detach().cpu().numpy() for t in (x0, v0, m)]
+
+    def grads(nsteps, precision):
+        x0.grad = v0.grad = m.grad = None
+        xe, ve = leapfrog(x0, v0, m, dt, eps, nsteps, checkpoint_every=every,
+                          precision=precision)
+        ((xe - xt) ** 2).sum(-1).mean().backward()
+        torch.cuda.synchronize()
+        return xe.detach().cpu().numpy(), ve.detach().cpu().numpy(), [
+            t.grad.detach().cpu().numpy() for t in (x0, v0, m)]
+
+    def grad_err(ga, gb):
+        return {name: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+                for name, a, b in zip(("x0", "v0", "m"), ga, gb)}
+
+    # the same run in FP64: at dt = 0.005 Myr (a third of a crossing time of the 1 pc
+    # sphere) the 200-step run is chaotic, so FP32 and FP64 trajectories, and with them
+    # the gradients, part ways; over one checkpoint segment (20 steps) they agree
+    x64, v64, g64 = grads(steps, "fp64")
+    e64 = total_energy(ctx, x64, v64, ic["mass"], eps)
+    x32 = x.detach().cpu().numpy()
+    disp = np.linalg.norm(x64 - ic["pos"], axis=1)
+    _, _, g32s = grads(every, "fp32")
+    _, _, g64s = grads(every, "fp64")
     return {"workload": "Plummer N=16384, eps=0.05 pc, dt=0.005 Myr, 200 steps, checkpoint "
                         "every 20, FP32 forces and VJP pairs, FP64 state",
+            "force_rel_err_median": f_med, "force_rel_err_p99": f_p99,
+            "energy_rel_drift": abs(e1 - e0) / abs(e0), "energy_drift_steps": steps,
+            "energy_rel_drift_fp64_forward": abs(e64 - e0) / abs(e0),
+            "final_pos_diff_vs_fp64_rel_median": float(np.median(
+                np.linalg.norm(x32 - x64, axis=1) / disp)),
+            "grad_rel_err_vs_fp64_adjoint_%d_steps" % steps: grad_err(g32, g64),
+            "grad_rel_err_vs_fp64_adjoint_%d_steps" % every: grad_err(g32s, g64s),
             "loss": float(loss.detach()), "forward_ms_per_step": fwd * 1e3,
             "backward_ms_per_step": bwd * 1e3,
             "adjoint_over_forward": bwd / fwd,
             "vjp_kernel_ms": vjp_ms / max(vjp_n, 1),
             "force_kernel_ms": f_ms / max(f_n, 1),
             "vjp_over_force_kernel": (vjp_ms / max(vjp_n, 1)) / (f_ms / max(f_n, 1)),
-            "grad_norms": [float(t.grad.norm()) for t in (x0, v0, m)]}
+            "grad_norms": [float(np.linalg.norm(g)) for g in g32]}
 
 
 def main():
@@ -504,6 +596,18 @@ def main():
                      "kernel": "k_direct_sym_f32<4,false>", "kernel_avg_ms": k_avg_s * 1e3,
                      "flop_per_pair": FLOP_PER_PAIR},
     }
+    cnt = counters_from_profiles("direct_f32_n262144_traffic.json")
+    if cnt and cnt.get("valu_wave_instructions_per_launch") and n == cnt.get("n", n):
+        # VALU issue from the counters: SQ_INSTS_VALU wave instructions per launch x 64
+        # lanes over this run's kernel time, against the lane-instruction rate of the
+        # FP32 peak (one FMA per lane = 2 FLOP)
+        lane_ops = cnt["valu_wave_instructions_per_launch"] * 64 / k_avg_s
+        result["roofline"].update({
+            "valu_counter": "SQ_INSTS_VALU",
+            "valu_wave_instructions_per_launch": cnt["valu_wave_instructions_per_launch"],
+            "valu_lane_ops_per_s": lane_ops,
+            "valu_issue_frac": lane_ops / PEAK_VALU_LANE_OPS,
+            "valu_busy_frac_counter": cnt.get("valu_busy_frac")})
     extras = {"update_kernel_avg_ms": u_ms / max(u_n, 1)}
 
     if not args.no_extras:
@@ -525,7 +629,7 @@ def main():
         a32 = acc32.cpu().numpy()
         err = np.linalg.norm(a32 - a64, axis=1) / np.linalg.norm(a64, axis=1)
         ke1, pe1, _ = ctx.energy(xs, vs, ic["mass"], EPS_DIRECT)
-        G = 4.398600412921223e-09 / 1.022712165045695e-03
+        G = G_KMS2_KPC_MSUN
         e0, e1 = ke0 + G * pe0, ke1 + G * pe1
         extras.update({
             "force_rel_err_median": float(np.median(err)),

@@ -799,29 +799,43 @@ struct GroupEntry<true> {
     typedef double4 T;
 };
 
-__device__ __forceinline__ int wave_incl_sum(int v, int lane)
+// Lanes' exclusive prefix sum and the wave total of a count 0..15, from four ballots
+// (no cross-lane data movement); the total is wave-uniform.
+__device__ __forceinline__ int wave_excl_count(int v, int& total)
 {
-    for (int o = 1; o < 64; o <<= 1) {
-        const int t = __shfl_up(v, o);
-        if (lane >= o) v += t;
+    int pre = 0;
+    total = 0;
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+        const u64 m = __ballot((v >> b) & 1);
+        pre += (int)__builtin_amdgcn_mbcnt_hi((u32)(m >> 32),
+                                              __builtin_amdgcn_mbcnt_lo((u32)m, 0u)) << b;
+        total += __popcll(m) << b;
     }
-    return v;
+    return pre;
+}
+
+__device__ __forceinline__ int lanes_below(u64 m)
+{
+    return (int)__builtin_amdgcn_mbcnt_hi((u32)(m >> 32), __builtin_amdgcn_mbcnt_lo((u32)m, 0u));
 }
 
 // One walk with the opening box B (centre bc, half-sizes bh) in the frame o: traversal
 // and evaluation as described above.  This lane's target is (dx0, dy0, dz0) = x - o
-// (FP64; t its FP32 rounding), its slice of each flush `slice` of `nslices`.  Adds the
-// lane's partial field to (ax, ay, az); listed = the list length.
+// (FP64; t its FP32 rounding), its slice of each flush `slice` of `nslices`, and the
+// group has cnt targets.  pend and fails are LDS scratch of 64 entries.  Adds the lane's
+// partial field to (ax, ay, az); listed = the list length.
 template <bool F64, bool GUARD>
 __device__ __forceinline__ void walk_lists(
     const NodeD* __restrict__ nodes, const int* __restrict__ kids, const int* __restrict__ skip,
     int n_nodes, double ox, double oy, double oz, float bcx, float bcy, float bcz, float bhx,
     float bhy, float bhz, double dx0, double dy0, double dz0, int slice, int nslices,
     float inv_theta, double eps2, int* stack, typename GroupEntry<F64>::T* list,
-    const float4* tg, int cnt, double& ax, double& ay, double& az, int& listed)
+    float4* pend, u64* fails, int cnt, double& ax, double& ay, double& az, int& listed)
 {
     typedef typename GroupEntry<F64>::T Entry;
     const int lane = threadIdx.x;
+    const int G = 64 / nslices;  // targets of the group; lane l holds target l mod G
     const float tx = (float)dx0, ty = (float)dy0, tz = (float)dz0;
     int head = 0, held = 0;  // list ring: first entry, entries held
     const float eps2f = in_vgpr((float)eps2);
@@ -865,33 +879,6 @@ __device__ __forceinline__ void walk_lists(
         head = (head + take) & (kListCap - 1);
         held -= take;
     };
-    // the group opening test (identical in both instances); rec = (centre, size with the
-    // leaf's sign bit).  The box decides first; a node it rejects is still accepted when
-    // every target passes the reference's test against it (tg: the targets in the frame),
-    // which keeps a group whose box spans a region none of its targets is near (two halo
-    // particles on either side of a disk) from opening that region.
-    auto accepts = [&](const double4 rec) -> bool {
-        if (__double_as_longlong(rec.w) < 0) return true;  // leaf
-        const float cx = (float)(rec.x - ox), cy = (float)(rec.y - oy), cz = (float)(rec.z - oz);
-        const float ddx = fmaxf(0.f, fabsf(cx - bcx) - bhx);
-        const float ddy = fmaxf(0.f, fabsf(cy - bcy) - bhy);
-        const float ddz = fmaxf(0.f, fabsf(cz - bcz) - bhz);
-        const float d2 = __builtin_fmaf(ddx, ddx, __builtin_fmaf(ddy, ddy, ddz * ddz));
-        const float R = (float)rec.w * inv_theta;  // +inf for theta = 0
-        const float R2 = R * R;
-        if (d2 > R2) return true;
-        // every target lies within the box's far corner distance of the centre: if that is
-        // within R too, no target can pass
-        const float fx = fabsf(cx - bcx) + bhx, fy = fabsf(cy - bcy) + bhy,
-                    fz = fabsf(cz - bcz) + bhz;
-        if (!(__builtin_fmaf(fx, fx, __builtin_fmaf(fy, fy, fz * fz)) > R2)) return false;
-        for (int i = 0; i < cnt; i++) {
-            const float4 t = tg[i];
-            const float ex = cx - t.x, ey = cy - t.y, ez = cz - t.z;
-            if (!(__builtin_fmaf(ex, ex, __builtin_fmaf(ey, ey, ez * ez)) > R2)) return false;
-        }
-        return true;
-    };
     // the list entry of a node: centre of mass and mass, re-based to the frame
     auto entry_of = [&](const double4 m) -> Entry {
         Entry e;
@@ -900,6 +887,63 @@ __device__ __forceinline__ void walk_lists(
     };
     auto com_of = [&](int n) -> double4 {
         return *(const double4*)((const double*)(nodes + n) + 4);
+    };
+    // The group opening test (identical in both instances) of the node with record rec =
+    // (centre, size with the leaf's sign bit): accept = leaf, or the box B clears the
+    // node's opening radius R; need = B does not clear it but its far corner does, so
+    // that every target may still pass the reference's test on its own (then the
+    // per-target tests below decide).  c: the centre in the frame; R2 = R^2.
+    auto box_test = [&](const double4 rec, float& cx, float& cy, float& cz, float& R2,
+                        bool& need) -> bool {
+        cx = (float)(rec.x - ox);
+        cy = (float)(rec.y - oy);
+        cz = (float)(rec.z - oz);
+        const float R = (float)rec.w * inv_theta;  // +inf for theta = 0
+        R2 = R * R;
+        // branch-free, so that the whole record is loaded at once whatever the node is
+        const bool leaf = __double_as_longlong(rec.w) < 0;
+        const float ax_ = fabsf(cx - bcx), ay_ = fabsf(cy - bcy), az_ = fabsf(cz - bcz);
+        const float ddx = fmaxf(0.f, ax_ - bhx), ddy = fmaxf(0.f, ay_ - bhy),
+                    ddz = fmaxf(0.f, az_ - bhz);
+        const bool clear = __builtin_fmaf(ddx, ddx, __builtin_fmaf(ddy, ddy, ddz * ddz)) > R2;
+        const float fx = ax_ + bhx, fy = ay_ + bhy, fz = az_ + bhz;
+        need = !leaf && !clear && cnt > 1 &&
+               __builtin_fmaf(fx, fx, __builtin_fmaf(fy, fy, fz * fz)) > R2;
+        return leaf || clear;
+    };
+    // Per-target tests of the nodes whose lanes have `need` set, spread over the wave as
+    // (node, target) pairs: 64 / G nodes per pass, lane l testing node l / G of the pass
+    // against target l mod G (its own).  A node is accepted when none of its G lanes
+    // fails.  Returns the acceptance of this lane's node (false without `need`).
+    auto target_tests = [&](bool need, float cx, float cy, float cz, float R2) -> bool {
+        const u64 needm = __ballot(need);
+        if (needm == 0) return false;
+        const int rank = lanes_below(needm);
+        if (need) pend[rank] = make_float4(cx, cy, cz, R2);
+        __syncthreads();
+        const int m = (int)__popcll(needm);
+        const int per = 64 / G;  // nodes per pass
+        int pass = 0;
+        for (int base = 0; base < m; base += per, pass++) {
+            const int j = base + lane / G;
+            bool fail = false;
+            if (j < m) {
+                const float4 q = pend[j];
+                const float ex = q.x - tx, ey = q.y - ty, ez = q.z - tz;
+                fail = !(__builtin_fmaf(ex, ex, __builtin_fmaf(ey, ey, ez * ez)) > q.w);
+            }
+            const u64 fm = __ballot(fail);
+            if (lane == 0) fails[pass] = fm;
+        }
+        __syncthreads();
+        bool ok = false;
+        if (need) {
+            const int ps = rank / per, sl = rank - ps * per;
+            const u64 field =
+                G == 64 ? fails[ps] : (fails[ps] >> (sl * G)) & ((1ull << G) - 1ull);
+            ok = field == 0;
+        }
+        return ok;
     };
 
     __syncthreads();  // the previous walk of this wavefront is done with stack and list
@@ -917,7 +961,17 @@ __device__ __forceinline__ void walk_lists(
             const int end = skip[n0];
             while (p < end) {
                 const double4 rec = *(const double4*)(nodes + p);
-                if (accepts(rec)) {
+                float cx, cy, cz, R2;
+                bool need;
+                bool acc = box_test(rec, cx, cy, cz, R2, need);
+                if (!acc && need) {
+                    // every lane's target against the node (lanes past the group's
+                    // count repeat its last target)
+                    const float ex = cx - tx, ey = cy - ty, ez = cz - tz;
+                    acc = __ballot(!(__builtin_fmaf(ex, ex, __builtin_fmaf(ey, ey, ez * ez)) >
+                                     R2)) == 0;
+                }
+                if (acc) {
                     if (lane == 0) list[(head + held) & (kListCap - 1)] = entry_of(com_of(p));
                     held += 1;
                     listed += 1;
@@ -935,40 +989,37 @@ __device__ __forceinline__ void walk_lists(
         }
         if (k > room) k = room;
         const bool active = lane < k;
+        // the popped node's record and children, loaded together (inactive lanes load the
+        // root's, and ignore them)
         const int n = active ? stack[top - 1 - lane] : 0;
-        bool acc_n = false;
-        int4 ka = make_int4(-1, -1, -1, -1), kb = make_int4(-1, -1, -1, -1);
-        int nk = 0;
-        double4 com = make_double4(0.0, 0.0, 0.0, 0.0);
-        if (active) {
-            // the whole 64-byte record in one load: geometry and moments
-            const double4 rec = *(const double4*)(nodes + n);
-            com = com_of(n);
-            acc_n = accepts(rec);
-            if (!acc_n) {
-                ka = *(const int4*)(kids + 8 * (int64_t)n);
-                kb = *(const int4*)(kids + 8 * (int64_t)n + 4);
-                nk = (ka.x >= 0) + (ka.y >= 0) + (ka.z >= 0) + (ka.w >= 0) + (kb.x >= 0) +
-                     (kb.y >= 0) + (kb.z >= 0) + (kb.w >= 0);
-            }
-        }
+        const double4 rec = *(const double4*)(nodes + n);
+        const double4 com = com_of(n);
+        const int4 ka = *(const int4*)(kids + 8 * (int64_t)n);
+        const int4 kb = *(const int4*)(kids + 8 * (int64_t)n + 4);
+        float cx, cy, cz, R2;
+        bool need;
+        bool acc_n = box_test(rec, cx, cy, cz, R2, need) && active;
+        need = need && active;
+        if (target_tests(need, cx, cy, cz, R2)) acc_n = true;
+        // children come first in kids[8 n ..], then -1s: 8 plus the sign bits
+        int nk = 8 + (ka.x >> 31) + (ka.y >> 31) + (ka.z >> 31) + (ka.w >> 31) + (kb.x >> 31) +
+                 (kb.y >> 31) + (kb.z >> 31) + (kb.w >> 31);
+        if (!active || acc_n) nk = 0;
         // appends to the list, in lane order
-        const unsigned long long am = __ballot(acc_n);
-        Entry e;
-        if (acc_n) e = entry_of(com);
-        const int before = (int)__popcll(am & ((1ull << lane) - 1ull));
+        const u64 am = __ballot(acc_n);
+        const int before = lanes_below(am);
         const int added = (int)__popcll(am);
         // pushes: the popped ids' slots are reused, children in lane order
-        const int incl = wave_incl_sum(nk, lane);
-        const int total = __shfl(incl, 63);
+        int total;
+        const int pre = wave_excl_count(nk, total);
         __syncthreads();  // every lane has read its popped id
-        if (acc_n) list[(head + held + before) & (kListCap - 1)] = e;
+        if (acc_n) list[(head + held + before) & (kListCap - 1)] = entry_of(com);
         if (nk > 0) {
-            int w = top - k + incl - nk;
+            const int w = top - k + pre;
             const int kk[8] = {ka.x, ka.y, ka.z, ka.w, kb.x, kb.y, kb.z, kb.w};
 #pragma unroll
             for (int q = 0; q < 8; q++)
-                if (kk[q] >= 0) stack[w++] = kk[q];
+                if (q < nk) stack[w + q] = kk[q];
         }
         held += added;
         listed += added;
@@ -997,7 +1048,8 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_group(
     typedef typename GroupEntry<F64>::T Entry;
     __shared__ int stack[kStackCap];
     __shared__ Entry list[kListCap];
-    __shared__ float4 tg[64];
+    __shared__ float4 pend[64];
+    __shared__ u64 fails[64];
     const int lane = threadIdx.x;
     const int64_t first = xcd_block(per_xcd) * G;
     if (first >= nt) return;
@@ -1044,8 +1096,6 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_group(
     const bool loose = G >= 16 && cnt > 1 && ext * (cnt - 1) - eg > kLooseLog2 * (cnt - 1);
 
     if (!loose) {
-        // the group's targets in the frame, for the per-target test
-        if (lane < cnt) tg[lane] = make_float4(tx, ty, tz, 0.f);
         // box centre and half-sizes, the half-sizes widened by a few ulp so that rounding
         // never shrinks the box
         const float bcx = 0.5f * (lox + hix), bcy = 0.5f * (loy + hiy),
@@ -1057,7 +1107,7 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_group(
         int listed = 0;
         walk_lists<F64, GUARD>(nodes, kids, skip, n_nodes, o.x, o.y, o.z, bcx, bcy, bcz, bhx,
                                bhy, bhz, dx0, dy0, dz0, lane / G, 64 / G, inv_theta, eps2,
-                               stack, list, tg, cnt, ax, ay, az, listed);
+                               stack, list, pend, fails, cnt, ax, ay, az, listed);
         // add the slices of each target (lanes ti, ti + G, ...), in FP64
         for (int s = G; s < 64; s <<= 1) {
             ax += __shfl_xor(ax, s);
@@ -1069,14 +1119,13 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_group(
     }
     // target by target, each in its own frame with a point as box (the reference's own
     // opening test), 64 slices
-    if (lane == 0) tg[0] = make_float4(0.f, 0.f, 0.f, 0.f);
     for (int k = 0; k < cnt; k++) {
         const double4 p = target(first + k);
         double ax = 0.0, ay = 0.0, az = 0.0;
         int listed = 0;
         walk_lists<F64, GUARD>(nodes, kids, skip, n_nodes, p.x, p.y, p.z, 0.f, 0.f, 0.f, 0.f,
                                0.f, 0.f, 0.0, 0.0, 0.0, lane, 64, inv_theta, eps2, stack, list,
-                               tg, 1, ax, ay, az, listed);
+                               pend, fails, 1, ax, ay, az, listed);
         for (int s = 1; s < 64; s <<= 1) {
             ax += __shfl_xor(ax, s);
             ay += __shfl_xor(ay, s);
