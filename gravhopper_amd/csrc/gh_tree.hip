@@ -747,8 +747,9 @@ __global__ __launch_bounds__(kFillBlock) void k_fill_nodes(FillArgs a, int64_t n
 // Scheduling (both walks).  Workgroups are dealt to the 8 XCDs round-robin (workgroup
 // b runs on XCD b mod 8); xcd_block() renumbers them so that XCD x takes the x-th of 8
 // contiguous, Morton-ordered ranges of walk units, and the L2 of XCD x holds the nodes
-// of one region of the tree.  One wavefront per workgroup, so that the hardware refills
-// a SIMD slot as soon as its walk ends rather than when the slowest of several does.
+// of one region of the tree.  The FP64 walk runs one wavefront per workgroup, so that the
+// hardware refills a SIMD slot as soon as its walk ends rather than when the slowest of
+// several does; the FP32 group walk runs two independent ones per workgroup (below).
 constexpr int kXcds = 8;
 constexpr int64_t kWalkUnits = 4096;  // walks in flight for small target counts
 constexpr int64_t kSortTargetsMin = 4096;  // position targets sorted from this many on
@@ -787,8 +788,12 @@ __device__ __forceinline__ int64_t xcd_block(int64_t per_xcd)
 // slices are added in FP64 at the end.  F64 evaluates the same lists (the traversal and
 // its tests are identical) in FP64, to check the FP32 arithmetic.  counts (nullable)
 // receives each target's list length.
-constexpr int kStackCap = 1024;
+constexpr int kStackCap = 768;
 constexpr int kListCap = 128;
+// Two wavefronts per workgroup, each walking its own group: a CU holds at most 16
+// workgroups, so one-wavefront workgroups would cap it at 16 walks.
+constexpr int kGroupWaves = 2;
+constexpr int kGroupBlock = 64 * kGroupWaves;
 
 template <bool F64>
 struct GroupEntry {
@@ -815,6 +820,17 @@ __device__ __forceinline__ int wave_excl_count(int v, int& total)
     return pre;
 }
 
+// Orders this wavefront's LDS accesses around it.  A wavefront's LDS instructions
+// execute in program order, so within one wavefront this fence (no s_barrier) is all
+// that a write by one lane followed by a read by another needs; the group walk keeps
+// each wavefront's LDS to itself.
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ __forceinline__ int lanes_below(u64 m)
 {
     return (int)__builtin_amdgcn_mbcnt_hi((u32)(m >> 32), __builtin_amdgcn_mbcnt_lo((u32)m, 0u));
@@ -823,21 +839,24 @@ __device__ __forceinline__ int lanes_below(u64 m)
 // One walk with the opening box B (centre bc, half-sizes bh) in the frame o: traversal
 // and evaluation as described above.  This lane's target is (dx0, dy0, dz0) = x - o
 // (FP64; t its FP32 rounding), its slice of each flush `slice` of `nslices`, and the
-// group has cnt targets.  pend and fails are LDS scratch of 64 entries.  Adds the lane's
-// partial field to (ax, ay, az); listed = the list length.
+// group has cnt targets.  list holds kListCap entries; fails is LDS scratch of 64
+// entries.  Adds the lane's partial field to (ax, ay, az); listed = the list length.
 template <bool F64, bool GUARD>
 __device__ __forceinline__ void walk_lists(
     const NodeD* __restrict__ nodes, const int* __restrict__ kids, const int* __restrict__ skip,
     int n_nodes, double ox, double oy, double oz, float bcx, float bcy, float bcz, float bhx,
     float bhy, float bhz, double dx0, double dy0, double dz0, int slice, int nslices,
     float inv_theta, double eps2, int* stack, typename GroupEntry<F64>::T* list,
-    float4* pend, u64* fails, int cnt, double& ax, double& ay, double& az, int& listed)
+    u64* fails, int cnt, double& ax, double& ay, double& az, int& listed)
 {
     typedef typename GroupEntry<F64>::T Entry;
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    // the per-target tests' scratch: the list's upper half, free whenever they run (the
+    // list then holds fewer than 64 entries)
+    float4* const pend = (float4*)(list + 64);
     const int G = 64 / nslices;  // targets of the group; lane l holds target l mod G
     const float tx = (float)dx0, ty = (float)dy0, tz = (float)dz0;
-    int head = 0, held = 0;  // list ring: first entry, entries held
+    int held = 0;  // entries in the list (list[0 .. held))
     const float eps2f = in_vgpr((float)eps2);
 
     // evaluates `take` entries (<= 64) from the list head
@@ -845,7 +864,7 @@ __device__ __forceinline__ void walk_lists(
         if (F64) {
             double fx = 0.0, fy = 0.0, fz = 0.0;
             for (int j = slice; j < take; j += nslices) {
-                const Entry e = list[(head + j) & (kListCap - 1)];
+                const Entry e = list[j];
                 const double ex = (double)e.x - dx0, ey = (double)e.y - dy0,
                              ez = (double)e.z - dz0;
                 const double d = ex * ex + ey * ey + ez * ez + eps2;
@@ -862,7 +881,7 @@ __device__ __forceinline__ void walk_lists(
         } else {
             float fx = 0.f, fy = 0.f, fz = 0.f;
             for (int j = slice; j < take; j += nslices) {
-                const Entry e = list[(head + j) & (kListCap - 1)];
+                const Entry e = list[j];
                 const float ex = (float)e.x - tx, ey = (float)e.y - ty, ez = (float)e.z - tz;
                 float r2 = __builtin_fmaf(ex, ex, eps2f);
                 r2 = __builtin_fmaf(ey, ey, r2);
@@ -876,8 +895,15 @@ __device__ __forceinline__ void walk_lists(
             }
             ax += (double)fx; ay += (double)fy; az += (double)fz;
         }
-        head = (head + take) & (kListCap - 1);
-        held -= take;
+        // the entries after the evaluated ones move to the front (fewer than 64, so the
+        // two ranges do not overlap)
+        const int rest = held - take;
+        Entry keep;
+        if (lane < rest) keep = list[take + lane];
+        wave_sync();
+        if (lane < rest) list[lane] = keep;
+        held = rest;
+        wave_sync();
     };
     // the list entry of a node: centre of mass and mass, re-based to the frame
     auto entry_of = [&](const double4 m) -> Entry {
@@ -920,7 +946,7 @@ __device__ __forceinline__ void walk_lists(
         if (needm == 0) return false;
         const int rank = lanes_below(needm);
         if (need) pend[rank] = make_float4(cx, cy, cz, R2);
-        __syncthreads();
+        wave_sync();
         const int m = (int)__popcll(needm);
         const int per = 64 / G;  // nodes per pass
         int pass = 0;
@@ -935,7 +961,7 @@ __device__ __forceinline__ void walk_lists(
             const u64 fm = __ballot(fail);
             if (lane == 0) fails[pass] = fm;
         }
-        __syncthreads();
+        wave_sync();
         bool ok = false;
         if (need) {
             const int ps = rank / per, sl = rank - ps * per;
@@ -946,10 +972,10 @@ __device__ __forceinline__ void walk_lists(
         return ok;
     };
 
-    __syncthreads();  // the previous walk of this wavefront is done with stack and list
+    wave_sync();  // the previous walk of this wavefront is done with stack and list
     if (lane == 0) stack[0] = 0;
     int top = n_nodes > 0 ? 1 : 0;
-    __syncthreads();
+    wave_sync();
     while (top > 0) {
         int k = top < 64 ? top : 64;
         const int room = (kStackCap - top) / 7;
@@ -972,17 +998,17 @@ __device__ __forceinline__ void walk_lists(
                                      R2)) == 0;
                 }
                 if (acc) {
-                    if (lane == 0) list[(head + held) & (kListCap - 1)] = entry_of(com_of(p));
+                    if (lane == 0) list[held] = entry_of(com_of(p));
                     held += 1;
                     listed += 1;
                     p = skip[p];
                 } else {
                     p = p + 1;
                 }
-                __syncthreads();
+                wave_sync();
                 if (held >= 64) {
                     flush(64);
-                    __syncthreads();
+                    wave_sync();
                 }
             }
             continue;
@@ -1012,8 +1038,8 @@ __device__ __forceinline__ void walk_lists(
         // pushes: the popped ids' slots are reused, children in lane order
         int total;
         const int pre = wave_excl_count(nk, total);
-        __syncthreads();  // every lane has read its popped id
-        if (acc_n) list[(head + held + before) & (kListCap - 1)] = entry_of(com);
+        wave_sync();  // every lane has read its popped id
+        if (acc_n) list[held + before] = entry_of(com);
         if (nk > 0) {
             const int w = top - k + pre;
             const int kk[8] = {ka.x, ka.y, ka.z, ka.w, kb.x, kb.y, kb.z, kb.w};
@@ -1024,9 +1050,9 @@ __device__ __forceinline__ void walk_lists(
         held += added;
         listed += added;
         top = top - k + total;
-        __syncthreads();
+        wave_sync();
         while (held >= 64) flush(64);
-        __syncthreads();
+        wave_sync();
     }
     if (held > 0) flush(held);
 }
@@ -1039,19 +1065,22 @@ __device__ __forceinline__ int exponent_f(float x)
 }
 
 template <bool F64, bool GUARD>
-__global__ __launch_bounds__(kWalkBlock) void k_walk_group(
+__global__ __launch_bounds__(kGroupBlock) void k_walk_group(
     const NodeD* __restrict__ nodes, const int* __restrict__ kids, const int* __restrict__ skip,
     int n_nodes, const double* __restrict__ tgt, const double4* __restrict__ stgt,
     const u32* __restrict__ order, int64_t nt, int G, float inv_theta, double eps2,
     double* __restrict__ acc, int* __restrict__ counts, int64_t per_xcd)
 {
     typedef typename GroupEntry<F64>::T Entry;
-    __shared__ int stack[kStackCap];
-    __shared__ Entry list[kListCap];
-    __shared__ float4 pend[64];
-    __shared__ u64 fails[64];
-    const int lane = threadIdx.x;
-    const int64_t first = xcd_block(per_xcd) * G;
+    __shared__ int stacks[kGroupWaves][kStackCap];
+    __shared__ Entry lists[kGroupWaves][kListCap];
+    __shared__ u64 failss[kGroupWaves][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int* const stack = stacks[wave];
+    Entry* const list = lists[wave];
+    u64* const fails = failss[wave];
+    // each wavefront walks its own group (no synchronisation between them)
+    const int64_t first = (xcd_block(per_xcd) * kGroupWaves + wave) * G;
     if (first >= nt) return;
     const int cnt = (int)(nt - first < G ? nt - first : G);
     const int ti = lane & (G - 1);  // this lane's target within the group
@@ -1107,7 +1136,7 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_group(
         int listed = 0;
         walk_lists<F64, GUARD>(nodes, kids, skip, n_nodes, o.x, o.y, o.z, bcx, bcy, bcz, bhx,
                                bhy, bhz, dx0, dy0, dz0, lane / G, 64 / G, inv_theta, eps2,
-                               stack, list, pend, fails, cnt, ax, ay, az, listed);
+                               stack, list, fails, cnt, ax, ay, az, listed);
         // add the slices of each target (lanes ti, ti + G, ...), in FP64
         for (int s = G; s < 64; s <<= 1) {
             ax += __shfl_xor(ax, s);
@@ -1125,7 +1154,7 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_group(
         int listed = 0;
         walk_lists<F64, GUARD>(nodes, kids, skip, n_nodes, p.x, p.y, p.z, 0.f, 0.f, 0.f, 0.f,
                                0.f, 0.f, 0.0, 0.0, 0.0, lane, 64, inv_theta, eps2, stack, list,
-                               pend, fails, 1, ax, ay, az, listed);
+                               fails, 1, ax, ay, az, listed);
         for (int s = 1; s < 64; s <<= 1) {
             ax += __shfl_xor(ax, s);
             ay += __shfl_xor(ay, s);
@@ -1526,7 +1555,8 @@ static int walk(gh_ctx* c, const double* tgt, bool self, int64_t nt, double eps,
     const bool guard = fp32_guard(eps);
     const float inv_theta = theta > 0.0 ? (float)(1.0 / theta) : __builtin_huge_valf();
     const int64_t n_groups = (nt + G - 1) / G;
-    const int64_t per = (n_groups + kXcds - 1) / kXcds;
+    const int64_t n_blocks = (n_groups + kGroupWaves - 1) / kGroupWaves;
+    const int64_t per = (n_blocks + kXcds - 1) / kXcds;
     const dim3 grid((unsigned)(per * kXcds));
     const NodeD* nd = tb->nodes64.as<NodeD>();
     const int* kd = tb->kids.as<int>();
@@ -1534,7 +1564,7 @@ static int walk(gh_ctx* c, const double* tgt, bool self, int64_t nt, double eps,
     const int T = (int)tb->n_nodes;
     const double eps2 = eps * eps;
 #define GH_GWALK(F, GU)                                                                    \
-    hipLaunchKernelGGL((k_walk_group<F, GU>), grid, dim3(kWalkBlock), 0, s, nd, kd, sk, T, tgt, \
+    hipLaunchKernelGGL((k_walk_group<F, GU>), grid, dim3(kGroupBlock), 0, s, nd, kd, sk, T, tgt, \
                        stgt, order, nt, G, inv_theta, eps2, d_acc, counts, per)
     if (f64eval)
         GH_GWALK(true, false);
