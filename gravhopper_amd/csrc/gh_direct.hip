@@ -438,7 +438,9 @@ __global__ __launch_bounds__(kBlock) void k_direct_f32(const double4* __restrict
 //
 // Results are partial fields part[K][p] (FP32, 3 per particle): particle p of block B
 // receives one partial from every block K (K = B for its diagonal); k_reduce_sym adds
-// them in block order in FP64.
+// them in block order in FP64.  The j-side partials can go to a second array
+// (SymOut): a pass over the block pairs of a band of columns J keeps only the band's
+// slots (see direct_sym).
 constexpr int kRotFromNext = 0x134;  // DPP wave_rol:1 -- lane l reads lane l+1 (mod 64)
 
 __device__ __forceinline__ float rot_from_next(float v)
@@ -448,6 +450,16 @@ __device__ __forceinline__ float rot_from_next(float v)
                                                    false));
 }
 
+// Where the partial fields go: the i-side of an item to part[slot_i][p] (stride np_pad
+// particles), the j-side to jpart[slot_j][p - joff] (stride jstride particles).  One
+// array for both (jpart = part, jstride = np_pad, joff = 0) outside banded passes.
+struct SymOut {
+    float* part;
+    float* jpart;
+    int64_t jstride;
+    int64_t joff;
+};
+
 // One block pair.  Work item {I, J, slot_i, slot_j}: global block indices I <= J (of
 // one simulation), the partial slot the i-side field goes to (J's index within its
 // simulation) and the slot of the j-side field (I's index within its simulation).
@@ -455,7 +467,7 @@ template <int IPT, bool GUARD, bool DS>
 __device__ __forceinline__ void sym_item(const float4* __restrict__ src,
                                          const double4* __restrict__ s64,
                                          const double4* __restrict__ frame, int64_t np_pad,
-                                         int4 it, float eps2, float* __restrict__ part,
+                                         int4 it, float eps2, const SymOut out,
                                          float4* tile, float4* tile_lo, float (*jp)[3][64])
 {
     constexpr int BLK = kBlock * IPT;
@@ -591,14 +603,15 @@ __device__ __forceinline__ void sym_item(const float4* __restrict__ src,
                 float sacc = jp[0][comp][t];
 #pragma unroll
                 for (int w = 1; w < kBlock / 64; w++) sacc += jp[w][comp][t];
-                part[((int64_t)it.w * np_pad + jbase + c * 64 + t) * 3 + comp] = -sacc;
+                out.jpart[((int64_t)it.w * out.jstride + (jbase - out.joff) + c * 64 + t) * 3 +
+                          comp] = -sacc;
             }
             __syncthreads();
         }
     }
 #pragma unroll
     for (int k = 0; k < IPT; k++) {
-        float* o = part + ((int64_t)it.z * np_pad + ibase + k * 64) * 3;
+        float* o = out.part + ((int64_t)it.z * np_pad + ibase + k * 64) * 3;
         o[0] = ax[k];
         o[1] = ay[k];
         o[2] = az[k];
@@ -617,7 +630,7 @@ __global__ __launch_bounds__(kBlock) void k_direct_sym_f32(const float4* __restr
                                                            const int4* __restrict__ work,
                                                            int64_t items,
                                                            const float* __restrict__ eps2_blk,
-                                                           float* __restrict__ part,
+                                                           const SymOut out,
                                                            int* __restrict__ queue)
 {
     constexpr int NCH = kBlock * IPT / 64;
@@ -629,7 +642,7 @@ __global__ __launch_bounds__(kBlock) void k_direct_sym_f32(const float4* __restr
     if (!queue) {
         const int4 it = work[blockIdx.x];
         if (flag[it.x] | flag[it.y]) return;
-        sym_item<IPT, GUARD, false>(src, nullptr, frame, np_pad, it, eps2_blk[it.x], part, tile,
+        sym_item<IPT, GUARD, false>(src, nullptr, frame, np_pad, it, eps2_blk[it.x], out, tile,
                                     nullptr, jp);
         return;
     }
@@ -641,7 +654,7 @@ __global__ __launch_bounds__(kBlock) void k_direct_sym_f32(const float4* __restr
         if (k >= items) break;
         const int4 it = work[k];
         if (flag[it.x] | flag[it.y]) continue;
-        sym_item<IPT, GUARD, false>(src, nullptr, frame, np_pad, it, eps2_blk[it.x], part, tile,
+        sym_item<IPT, GUARD, false>(src, nullptr, frame, np_pad, it, eps2_blk[it.x], out, tile,
                                     nullptr, jp);
         __syncthreads();
     }
@@ -658,7 +671,7 @@ __global__ __launch_bounds__(kBlock) void k_direct_sym_ds(const double4* __restr
                                                           const int4* __restrict__ work,
                                                           int64_t items,
                                                           const float* __restrict__ eps2_blk,
-                                                          float* __restrict__ part)
+                                                          const SymOut out)
 {
     constexpr int NCH = kBlock * IPT / 64;
     __shared__ float4 tile[NCH * 128];
@@ -668,7 +681,7 @@ __global__ __launch_bounds__(kBlock) void k_direct_sym_ds(const double4* __restr
     for (int64_t k = blockIdx.x; k < items; k += gridDim.x) {
         const int4 it = work[k];
         if (!(flag[it.x] | flag[it.y])) continue;
-        sym_item<IPT, GUARD, true>(nullptr, s64, frame, np_pad, it, eps2_blk[it.x], part, tile,
+        sym_item<IPT, GUARD, true>(nullptr, s64, frame, np_pad, it, eps2_blk[it.x], out, tile,
                                    tile_lo, jp);
         __syncthreads();
     }
@@ -708,6 +721,36 @@ __global__ __launch_bounds__(kBlock) void k_reduce_sym(const float* __restrict__
             acc[perm ? 3 * (int64_t)perm[q] + (k - 3 * q) : k] = s[t];
         }
     }
+}
+
+// One pass of the banded pair-once self field (direct_sym): the block pairs (I, J) with
+// J in the band [c0, c0 + w) of blocks.  Particle p of block P has this pass's partials
+// from partners q (blocks) in partner order: the j-side of (q, P) for q < P when P is
+// in the band (partj[q][p - c0 blk], jw blocks of particles per slot), then the i-side
+// of (P, q) for q >= P in the band (parti[q - c0][p]).  Their FP64 sum is added to
+// acc[perm[p]] (written on the first pass).  w: the band's width in blocks.
+__global__ __launch_bounds__(kBlock) void k_reduce_band(const float* __restrict__ parti,
+                                                        const float* __restrict__ partj,
+                                                        int blk, int c0, int w, int jw,
+                                                        int64_t np_pad,
+                                                        int64_t n3, const u32* __restrict__ perm,
+                                                        bool first, double* __restrict__ acc)
+{
+    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (k >= n3) return;
+    const int64_t p = k / 3;
+    const int comp = (int)(k - 3 * p);
+    const int P = (int)(p / blk);
+    double sum = 0.0;
+    const bool in_band = P >= c0 && P < c0 + w;
+    if (in_band) {
+        const int64_t pj = p - (int64_t)c0 * blk;
+        for (int q = 0; q < P; q++) sum += (double)partj[((int64_t)q * jw * blk + pj) * 3 + comp];
+    }
+    for (int q = P > c0 ? P : c0; q < c0 + w; q++)
+        sum += (double)parti[((int64_t)(q - c0) * np_pad + p) * 3 + comp];
+    const int64_t o = 3 * (int64_t)perm[p] + comp;
+    acc[o] = first ? sum : acc[o] + sum;
 }
 
 // Sum of per-split partial fields (sorted target order), in split order, written to
@@ -800,9 +843,10 @@ bool fp32_guard(double eps)
 // Symmetric (pair-once) kernel for the self field: particles per lane, or 0 for the
 // one-sided kernel.  Block pairs must number a few thousand to fill 256 CUs.
 // The pair-once kernel keeps one FP32 partial field per block: 12 B x np x (np / block)
-// of scratch, which grows as np^2 (12.9 GB at np = 2^20).  Beyond this size the
-// one-sided kernel (no scratch, 1.45x the VALU work per pair) is used instead.
-constexpr double kSymScratchMax = 16.0 * (1 << 30);
+// of scratch, which grows as np^2 (12.9 GB at np = 2^20).  Above kSymScratchMax the
+// block pairs are taken in bands of columns, one pass each, whose partials take at most
+// that much (direct_sym).
+constexpr double kSymScratchMax = 13.0 * (1 << 30);
 
 int sym_ipt(gh_ctx* c, int64_t np)
 {
@@ -817,11 +861,6 @@ int sym_ipt(gh_ctx* c, int64_t np)
         ipt = 2;
     else if (np >= 16384)
         ipt = 1;
-    if (ipt > 0) {
-        const double blk = (double)kBlock * ipt;
-        const double np_pad = std::ceil((double)np / blk) * blk;
-        if (12.0 * np_pad * (np_pad / blk) > kSymScratchMax) return 0;
-    }
     return ipt;
 }
 
@@ -829,34 +868,40 @@ template <int IPT>
 static void launch_sym_t(bool guard, unsigned grid, hipStream_t s, const float4* src,
                          const double4* s64, const double4* frame, const int* flag,
                          const int* nflag, int64_t np_pad, const int4* work, int64_t items,
-                         const float* eps2_blk, float* part, int* queue, unsigned ds_grid)
+                         const float* eps2_blk, const SymOut out, int* queue, unsigned ds_grid)
 {
     if (guard) {
         hipLaunchKernelGGL((k_direct_sym_f32<IPT, true>), dim3(grid), dim3(kBlock), 0, s, src,
-                           frame, flag, np_pad, work, items, eps2_blk, part, queue);
+                           frame, flag, np_pad, work, items, eps2_blk, out, queue);
         hipLaunchKernelGGL((k_direct_sym_ds<IPT, true>), dim3(ds_grid), dim3(kBlock), 0, s, s64,
-                           frame, flag, nflag, np_pad, work, items, eps2_blk, part);
+                           frame, flag, nflag, np_pad, work, items, eps2_blk, out);
     } else {
         hipLaunchKernelGGL((k_direct_sym_f32<IPT, false>), dim3(grid), dim3(kBlock), 0, s, src,
-                           frame, flag, np_pad, work, items, eps2_blk, part, queue);
+                           frame, flag, np_pad, work, items, eps2_blk, out, queue);
         hipLaunchKernelGGL((k_direct_sym_ds<IPT, false>), dim3(ds_grid), dim3(kBlock), 0, s,
-                           s64, frame, flag, nflag, np_pad, work, items, eps2_blk, part);
+                           s64, frame, flag, nflag, np_pad, work, items, eps2_blk, out);
     }
 }
 
 int launch_sym(gh_ctx* c, int ipt, bool guard, const float4* src, const double4* s64,
                const double4* frame, const int* flag, const int* nflag, int64_t np_pad,
                const int4* work, int64_t items, const float* eps2_blk, float* part, int* queue,
-               unsigned persistent_grid, hipStream_t s)
+               unsigned persistent_grid, hipStream_t s, float* jpart, int64_t jstride,
+               int64_t joff)
 {
+    SymOut out;
+    out.part = part;
+    out.jpart = jpart ? jpart : part;
+    out.jstride = jpart ? jstride : np_pad;
+    out.joff = jpart ? joff : 0;
     if (items > 0x7fffffff) return fail(c, GH_EARG, "too many block pairs for one launch");
     const unsigned grid = queue ? persistent_grid : (unsigned)items;
     if (grid == 0) return 0;
     const unsigned ds_grid = (unsigned)std::min<int64_t>(items, 2 * (int64_t)c->n_cu);
     switch (ipt) {
-        case 1: launch_sym_t<1>(guard, grid, s, src, s64, frame, flag, nflag, np_pad, work, items, eps2_blk, part, queue, ds_grid); break;
-        case 2: launch_sym_t<2>(guard, grid, s, src, s64, frame, flag, nflag, np_pad, work, items, eps2_blk, part, queue, ds_grid); break;
-        case 4: launch_sym_t<4>(guard, grid, s, src, s64, frame, flag, nflag, np_pad, work, items, eps2_blk, part, queue, ds_grid); break;
+        case 1: launch_sym_t<1>(guard, grid, s, src, s64, frame, flag, nflag, np_pad, work, items, eps2_blk, out, queue, ds_grid); break;
+        case 2: launch_sym_t<2>(guard, grid, s, src, s64, frame, flag, nflag, np_pad, work, items, eps2_blk, out, queue, ds_grid); break;
+        case 4: launch_sym_t<4>(guard, grid, s, src, s64, frame, flag, nflag, np_pad, work, items, eps2_blk, out, queue, ds_grid); break;
         default: return fail(c, GH_EARG, "unsupported particles-per-lane %d", ipt);
     }
     return check_launch(c, "k_direct_sym_f32");
@@ -885,23 +930,40 @@ int direct_sym(gh_ctx* c, const double* d_pos, const double* d_mass, int64_t np,
     const int64_t blk = (int64_t)kBlock * ipt;
     const int64_t np_pad = pad_to(np, blk);
     const int nb = (int)(np_pad / blk);
+    // band width in blocks: all of them when one partial per block fits kSymScratchMax,
+    // else as many as keep the band's partials (2 w np_pad of 12 bytes) within it
+    int w = nb;
+    if (c->tune[GH_TUNE_SYM_BAND] > 0)
+        w = std::min(nb, c->tune[GH_TUNE_SYM_BAND]);
+    else if (12.0 * (double)np_pad * nb > kSymScratchMax)
+        w = (int)std::max<double>(1.0, std::floor(kSymScratchMax / (24.0 * (double)np_pad)));
+    const int bands = (nb + w - 1) / w;
     const int64_t items = (int64_t)nb * (nb + 1) / 2;
     if (items > 0x7fffffff) return fail(c, GH_EARG, "too many particles for one launch");
-    GH_TRY(ensure(c, c->scratch, sizeof(float) * 3 * (size_t)np_pad * nb));
+    const size_t part_floats = bands == 1 ? 3 * (size_t)np_pad * nb : 6 * (size_t)np_pad * w;
+    GH_TRY(ensure(c, c->scratch, sizeof(float) * part_floats));
     GH_TRY(ensure(c, c->eps_blk, sizeof(float) * (size_t)nb));
     const float eps2 = (float)(eps * eps);
-    if (c->sym_nb != nb || c->sym_ipt != ipt) {
-        // block pairs (I, J), I <= J, ordered by I; slots are the block indices
-        std::vector<int4> w((size_t)items);
+    if (c->sym_nb != nb || c->sym_ipt != ipt || c->sym_w != w) {
+        // block pairs (I, J), I <= J, band by band of columns J, then ordered by I;
+        // slots: the partner block, within the band for the i-side
+        std::vector<int4> wl((size_t)items);
+        c->sym_band_start.assign(1, 0);
         size_t k = 0;
-        for (int I = 0; I < nb; I++)
-            for (int J = I; J < nb; J++) w[k++] = make_int4(I, J, J, I);
+        for (int b = 0; b < bands; b++) {
+            const int c0 = b * w, c1 = std::min(nb, c0 + w);
+            for (int I = 0; I < c1; I++)
+                for (int J = std::max(I, c0); J < c1; J++)
+                    wl[k++] = make_int4(I, J, bands == 1 ? J : J - c0, I);
+            c->sym_band_start.push_back((int64_t)k);
+        }
         GH_TRY(ensure(c, c->work, sizeof(int4) * (size_t)items));
-        GH_HIP(c, hipMemcpyAsync(c->work.p, w.data(), sizeof(int4) * (size_t)items,
+        GH_HIP(c, hipMemcpyAsync(c->work.p, wl.data(), sizeof(int4) * (size_t)items,
                                  hipMemcpyHostToDevice, s));
         GH_HIP(c, hipStreamSynchronize(s));
         c->sym_nb = nb;
         c->sym_ipt = ipt;
+        c->sym_w = w;
         c->sym_eps2 = -1.f;
     }
     if (c->sym_eps2 != eps2) {  // the block list is rebuilt above whenever nb changes
@@ -914,14 +976,34 @@ int direct_sym(gh_ctx* c, const double* d_pos, const double* d_mass, int64_t np,
     GH_TRY(spatial_order(c, o, d_pos, np, s));
     GH_TRY(gather_frames(c, o, d_pos, d_mass, o.perm(), np, np_pad, (int)blk, nullptr, true, s));
     float* part = c->scratch.as<float>();
-    {
-        TimedScope ts(c, s, GH_T_DIRECT_F32);
-        GH_TRY(launch_sym(c, ipt, fp32_guard(eps), o.loc.as<float4>(), o.s64.as<double4>(),
-                          o.frame.as<double4>(), o.flag.as<int>(), o.nflag.as<int>(), np_pad,
-                          c->work.as<int4>(),
-                          items, c->eps_blk.as<float>(), part, nullptr, 0, s));
+    const bool guard = fp32_guard(eps);
+    if (bands == 1) {
+        {
+            TimedScope ts(c, s, GH_T_DIRECT_F32);
+            GH_TRY(launch_sym(c, ipt, guard, o.loc.as<float4>(), o.s64.as<double4>(),
+                              o.frame.as<double4>(), o.flag.as<int>(), o.nflag.as<int>(), np_pad,
+                              c->work.as<int4>(), items, c->eps_blk.as<float>(), part, nullptr,
+                              0, s));
+        }
+        return reduce_sym(c, part, nb, nullptr, (int)blk, np_pad, np, o.perm(), d_acc, s);
     }
-    return reduce_sym(c, part, nb, nullptr, (int)blk, np_pad, np, o.perm(), d_acc, s);
+    float* partj = part + 3 * (size_t)np_pad * w;
+    for (int b = 0; b < bands; b++) {
+        const int c0 = b * w, wb = std::min(nb, c0 + w) - c0;
+        const int64_t i0 = c->sym_band_start[b], ni = c->sym_band_start[b + 1] - i0;
+        {
+            TimedScope ts(c, s, GH_T_DIRECT_F32);
+            GH_TRY(launch_sym(c, ipt, guard, o.loc.as<float4>(), o.s64.as<double4>(),
+                              o.frame.as<double4>(), o.flag.as<int>(), o.nflag.as<int>(), np_pad,
+                              c->work.as<int4>() + i0, ni, c->eps_blk.as<float>(), part, nullptr,
+                              0, s, partj, (int64_t)w * blk, (int64_t)c0 * blk));
+        }
+        hipLaunchKernelGGL(k_reduce_band, dim3(blocks_for(3 * np, kBlock)), dim3(kBlock), 0, s,
+                           part, partj, (int)blk, c0, wb, w, np_pad, 3 * np, o.perm(), b == 0,
+                           d_acc);
+        GH_TRY(check_launch(c, "k_reduce_band"));
+    }
+    return 0;
 }
 
 // One-sided FP32 field at nt targets.  src64: the sources as (x, y, z, m), np_pad of

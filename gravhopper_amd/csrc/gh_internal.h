@@ -65,7 +65,8 @@ struct gh_ctx {
     size_t arena_bytes = 0;         // device memory held by the context's arenas
     gh::DevBuf work;                // block-pair list of the pair-once direct kernel
     gh::DevBuf eps_blk;             // its per-block squared softening
-    int sym_nb = -1, sym_ipt = -1;
+    int sym_nb = -1, sym_ipt = -1, sym_w = -1;  // shape of the block-pair list
+    std::vector<int64_t> sym_band_start;        // its bands' first items
     float sym_eps2 = -1.f;          // the softening eps_blk holds
     // adjoint workspaces (gh_adjoint.hip)
     gh::DevBuf adj_work, adj_src, adj_mu, adj_mv, adj_part, adj_seg;
@@ -182,10 +183,13 @@ int direct_sym(gh_ctx* c, const double* d_pos, const double* d_mass, int64_t np,
 // The pair-once kernel over a block-pair list (src: local FP32 coordinates per sorted
 // particle, s64: the same particles in FP64, frame/flag: per block), followed by its
 // double-single instance for the items with a flagged block.
+// jpart (nullable): the j-side partials go to jpart[slot_j][p - joff] with a stride of
+// jstride particles instead of part.
 int launch_sym(gh_ctx* c, int ipt, bool guard, const float4* src, const double4* s64,
                const double4* frame, const int* flag, const int* nflag, int64_t np_pad,
                const int4* work, int64_t items, const float* eps2_blk, float* part, int* queue,
-               unsigned persistent_grid, hipStream_t s);
+               unsigned persistent_grid, hipStream_t s, float* jpart = nullptr,
+               int64_t jstride = 0, int64_t joff = 0);
 // acc[perm[p]] (perm nullable) = the FP64 sum of particle p's partial slots
 int reduce_sym(gh_ctx* c, const float* part, int nb, const int* nb_blk, int blk,
                int64_t np_pad, int64_t n, const u32* perm, double* acc, hipStream_t s);

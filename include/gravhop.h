@@ -236,6 +236,10 @@ int gh_timing_read(gh_ctx *ctx, int kernel_class, double *total_ms, int64_t *lau
  *   GH_TUNE_ALLOC_LIMIT_MB  cap, in MiB, on the device memory the context's arenas may
  *                        hold (0: no cap); a call that would exceed it fails with
  *                        GH_ENOMEM and leaves the context usable
+ *   GH_TUNE_SYM_BAND     FP32 pair-once self field: 0 automatic (one pass while one FP32
+ *                        partial per particle and block fits 13 GiB, else bands of
+ *                        block columns, one pass each, whose partials fit it), n > 0
+ *                        bands of n block columns
  * The FP32 direct-sum keys and GH_TUNE_WALK_F64EVAL change results only by rounding;
  * GH_TUNE_WALK_GROUP changes which nodes the FP32 walk accepts (never fewer than the
  * reference's opening test accepts for any target); GH_TUNE_DIRECT_F64_TPW and
@@ -248,7 +252,8 @@ int gh_timing_read(gh_ctx *ctx, int kernel_class, double *total_ms, int64_t *lau
 #define GH_TUNE_WALK_F64EVAL 4
 #define GH_TUNE_ALLOC_LIMIT_MB 5
 #define GH_TUNE_DIRECT_F64_TPW 6
-#define GH_TUNE_COUNT 7
+#define GH_TUNE_SYM_BAND 7
+#define GH_TUNE_COUNT 8
 int gh_set_tuning(gh_ctx *ctx, int key, int value);
 
 /* FP64 diagnostics (no reference counterpart): kinetic energy sum 0.5 m v^2

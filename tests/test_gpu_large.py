@@ -161,3 +161,24 @@ def test_ensemble_config4_full_size(ctx, oracle):
         print("sim %d (N=%d): median %.2e p99 %.2e" % (k, len(s["mass"]), np.median(err),
                                                       np.percentile(err, 99)), flush=True)
         assert np.median(err) < 1e-5, k
+
+
+def test_direct_fp32_pair_once_banded_2m(ctx):
+    # N = 2^21: one partial per particle and block would take 51.5 GB, so the pair-once
+    # kernel runs in 8 bands of at most 277 block columns (13 GiB of partials per pass)
+    n = 1 << 21
+    print("plummer N=%d" % n, flush=True)
+    p = workloads.plummer(n, b=1e-3, mass=1e6, seed=94)
+    pos, mass, eps = p["pos"], p["mass"], 1e-5
+    a = ctx.direct(pos, mass, eps, FP32)
+    print("fp32 banded pair-once done", flush=True)
+    rng = np.random.default_rng(95)
+    idx = rng.choice(n, size=256, replace=False)
+    want = ctx.direct(pos, mass, eps, FP64, force_pos=pos[idx])
+    e = rel_err(a[idx], want)
+    print("fp32 vs fp64 at 256 particles: median %.3g max %.3g" % (np.median(e), e.max()))
+    assert np.median(e) < 1e-6
+    net = (mass[:, None] * a).sum(axis=0)
+    scale = (mass[:, None] * np.abs(a)).sum(axis=0)
+    print("net force / sum of magnitudes:", net / scale)
+    assert np.all(np.abs(net) < 1e-6 * scale)
