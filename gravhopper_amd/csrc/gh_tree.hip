@@ -29,13 +29,15 @@
 //                     end+1
 //   6. moments        node mass and first moment as differences of double-double prefix
 //                     sums over the sorted particles (error far below one FP64 ulp)
-//   7. walk           stackless pre-order walk with skip pointers.  A wavefront walks up
-//                     to 64 targets in lockstep: the wave visits nodes in increasing
-//                     order (next = n+1 if any waiting lane opened n, else skip[n]); a
-//                     lane waits for every node from the end of the last subtree it
-//                     accepted on, so each lane accepts exactly the nodes the
+//   7. walks          FP64: stackless pre-order walk with skip pointers.  A wavefront
+//                     walks up to 64 targets in lockstep: the wave visits nodes in
+//                     increasing order (next = n+1 if any waiting lane opened n, else
+//                     skip[n]); a lane waits for every node from the end of the last
+//                     subtree it accepted on, so each lane accepts exactly the nodes the
 //                     reference's recursive walk (:487-541) accepts, while the node index
 //                     stays wave-uniform and node records arrive through scalar loads.
+//                     FP32: a group walk with one interaction list per 16 targets whose
+//                     lists refine the reference's accepted nodes (k_walk_group).
 #include <algorithm>
 #include <cmath>
 
@@ -1246,7 +1248,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_f64(const NodeD* __restrict__ n
     const double eps2 = eps * eps;
     double ax = 0.0, ay = 0.0, az = 0.0;
     int cnt = 0;
-    int my_skip = live ? 0 : 0x7fffffff;  // as in the FP32 walk: waiting iff node >= my_skip
+    int my_skip = live ? 0 : 0x7fffffff;  // waiting iff node >= my_skip
     const int n_nodes32 = (int)n_nodes;  // below 2^31 (checked by the build)
     int node = 0;
     while (node < n_nodes32) {
