@@ -150,6 +150,7 @@ class Context:
                                "(%d device(s) visible, code %d)" % (device, n.value, rc))
         self.handle = h
         self.device = device
+        self._tuning = {}  # values set through set_tuning (0 = automatic otherwise)
 
     def close(self):
         if getattr(self, "handle", None) is not None and self.handle.value:
@@ -309,6 +310,11 @@ class Context:
 
     def set_tuning(self, key, value):
         self.check(self.lib.gh_set_tuning(self.handle, int(key), int(value)))
+        self._tuning[int(key)] = int(value)
+
+    def get_tuning(self, key):
+        """The value last set for a tuning key on this context (0: automatic)."""
+        return self._tuning.get(int(key), 0)
 
     def timing_enable(self, on=True):
         self.check(self.lib.gh_timing_enable(self.handle, 1 if on else 0))

@@ -87,14 +87,21 @@ class Simulation(object):
         compensated sums, FP64 state)
     theta : opening angle of the tree (the reference always uses 0.7,
         gravhopper.py:446 with jbgrav.py:52)
+    autotune : before the first FP32 run, time the candidate kernel shapes on this
+        simulation's particles (gravhopper_amd.autotune.tune_for_run: the self-field
+        kernel for 'direct', the walk's group size for 'tree') and use the fastest
+        for this simulation's runs (FP32 rounding may differ between shapes; FP64
+        runs are unaffected)
     device_loop : with external forces, keep the state on the GPU and call the
         external-force callbacks once per step from the library's loop (True), or
         step in a host loop as the reference does (False); same trajectories
     """
 
     def __init__(self, dt=None, eps=None, algorithm='tree', precision='fp64', theta=0.7,
-                 device_loop=True):
+                 device_loop=True, autotune=False):
         self.device_loop = bool(device_loop)
+        self.autotune = bool(autotune)
+        self._tuned = None  # (algorithm, Np, tuning key, value) once tuned
         self.ICarrays = False
         self.Np = 0
         self.Nsnap = 0
@@ -188,15 +195,43 @@ class Simulation(object):
 
         if N <= 0:
             return
-        if not self._has_external_forces():
-            self._run_on_device(N)
-        elif self.device_loop:
-            self._run_on_device_external(N)
-        else:
-            for _ in range(self.timestep, self.timestep + N):
-                self.timestep += 1
-                self.perform_timestep()
-                self._times[self.timestep] = self._times[self.timestep - 1] + self._dt()
+        ctx = _native.default_context()
+        tuned = self._tuning_for_run(ctx)
+        if tuned is not None:
+            saved = ctx.get_tuning(tuned[0])
+            ctx.set_tuning(*tuned)
+        try:
+            if not self._has_external_forces():
+                self._run_on_device(N)
+            elif self.device_loop:
+                self._run_on_device_external(N)
+            else:
+                for _ in range(self.timestep, self.timestep + N):
+                    self.timestep += 1
+                    self.perform_timestep()
+                    self._times[self.timestep] = self._times[self.timestep - 1] + self._dt()
+        finally:
+            if tuned is not None:
+                ctx.set_tuning(tuned[0], saved)
+
+    def _tuning_for_run(self, ctx):
+        """(tuning key, value) this simulation's runs use, from the autotuner, or None."""
+        if not self.autotune or self.params['precision'] != 'fp32':
+            return None
+        algo = self.params['algorithm']
+        if self._tuned is None or self._tuned[:2] != (algo, self.Np):
+            from . import autotune
+            t = self.timestep
+            saved = {k: ctx.get_tuning(k) for k in (_native.TUNE_DIRECT_SELF,
+                                                    _native.TUNE_WALK_GROUP)}
+            try:
+                kv = autotune.tune_for_run(ctx, self._pos[t], self._mass, self._eps(), algo,
+                                           'fp32', theta=self.theta)
+            finally:
+                for k, v in saved.items():
+                    ctx.set_tuning(k, v)
+            self._tuned = (algo, self.Np) + tuple(kv)
+        return self._tuned[2:]
 
     def _has_external_forces(self):
         return bool(self.extra_force_functions or self.extra_timedependent_force_functions
