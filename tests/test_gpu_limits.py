@@ -62,3 +62,24 @@ def test_arena_cap_above_need_is_transparent(ctx, capped):
         capped.tree(big, np.ones(len(big)), 1e-3, 0.7, FP32)
     got = capped.tree(small["pos"], small["mass"], 1e-3, 0.7, FP64)
     assert_bitwise(got, ctx.tree(small["pos"], small["mass"], 1e-3, 0.7, FP64))
+
+
+def test_ensemble_create_destroy_under_cap(capped):
+    # every ensemble's buffers go back to the arena count when it is destroyed, so a
+    # cap that fits one ensemble fits any number of them made one after another
+    sims = []
+    for k in range(4):
+        ic = workloads.plummer(20000, seed=860 + k)
+        ic["eps"], ic["dt"] = 2e-5, 1e-3
+        sims.append(ic)
+    capped.set_tuning(_native.TUNE_ALLOC_LIMIT_MB, 256)
+    first = None
+    for rep in range(12):
+        ens = _native.Ensemble(capped, sims, ipt=2)
+        ens.run(1)
+        pos, _ = ens.state()
+        ens.close()
+        if first is None:
+            first = pos
+        for a, b in zip(pos, first):
+            assert_bitwise(a, b, "repeat %d" % rep)
