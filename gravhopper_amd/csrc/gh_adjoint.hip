@@ -20,7 +20,9 @@
 // (g_k = m_k A - B mu_k + C, lm_k = -L), and the j side, moving one lane per step,
 //   A' += s^-3 m_k,  B' += s^-3 mu_k,  C' += s^-3 u d,  L' += s^-3 (mu_k . d)
 // (g_j = -mu_j A' + m_j B' - C', lm_j = L').  Pair arithmetic in FP32, partial sums
-// added in FP64.  The trajectory is re-run from checkpoints taken every k steps.
+// added in FP64.  The trajectory is re-run from checkpoints taken every k steps; the
+// re-run keeps each step's half-step positions and, for FP32 forces, the spatial order
+// its force evaluation made, which the VJP of the same positions reuses.
 #include <algorithm>
 
 #include "gh_internal.h"
@@ -338,6 +340,25 @@ static int copy_d(gh_ctx* c, const double* a, double* b, int64_t n, hipStream_t 
     return check_launch(c, "k_copy_d");
 }
 
+__global__ __launch_bounds__(kBlock) void k_copy_u32(const u32* __restrict__ a,
+                                                     u32* __restrict__ b, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
+// Puts back a spatial order that the forward replay saved for the same positions and
+// clears the count of flagged groups as k_morton does: the order buffers are then as
+// spatial_order leaves them, without sorting again.
+__global__ __launch_bounds__(kBlock) void k_restore_order(const u32* __restrict__ saved,
+                                                          u32* __restrict__ perm, int64_t n,
+                                                          int* __restrict__ nflag)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i == 0) *nflag = 0;
+    if (i < n) perm[i] = saved[i];
+}
+
 static int vjp_ipt(int64_t n)
 {
     if (n >= 65536) return 4;
@@ -399,10 +420,11 @@ static int adj_prepare(gh_ctx* c, int64_t n, AdjWork& w, hipStream_t s)
 }
 
 // One reverse step at half-step positions xh: (lx, lv) after the step -> before it;
-// lm accumulates.
+// lm accumulates.  saved, if not null, is the spatial order of xh that the forward
+// replay's force evaluation made.
 static int reverse_step(gh_ctx* c, const AdjWork& w, const double* xh, const double* m,
                         int64_t n, double dt, double eps, double* lx, double* lv, double* lm,
-                        hipStream_t s)
+                        const u32* saved, hipStream_t s)
 {
     const double cdrift = 0.5 * dt * kDriftScale;
     const double dtA = dt * kAccelScale;
@@ -410,7 +432,14 @@ static int reverse_step(gh_ctx* c, const AdjWork& w, const double* xh, const dou
     double* mv = c->adj_mv.as<double>();
     float4* part = c->adj_part.as<float4>();
     OrderBufs& o = c->ord_p;
-    GH_TRY(spatial_order(c, o, xh, n, s));
+    if (saved && o.sorted_n == n) {
+        hipLaunchKernelGGL(k_restore_order, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s,
+                           saved, o.idx[o.sel].as<u32>(), n, o.nflag.as<int>());
+        GH_TRY(check_launch(c, "k_restore_order"));
+        o.sorted = xh;
+    } else {
+        GH_TRY(spatial_order(c, o, xh, n, s));
+    }
     GH_TRY(gather_frames(c, o, xh, m, o.perm(), n, w.n_pad, kBlock * w.ipt, nullptr, true, s));
     hipLaunchKernelGGL(k_adj_pre, dim3(blocks_for(w.n_pad, kBlock)), dim3(kBlock), 0, s, lx, lv,
                        n, w.n_pad, o.perm(), cdrift, dtA, mv, mu);
@@ -432,23 +461,40 @@ static int reverse_step(gh_ctx* c, const AdjWork& w, const double* xh, const dou
     return check_launch(c, "k_adj_post");
 }
 
-// forward over [0, len) steps from (x, v); xh of every step saved to hx (len x n3)
+// forward over [0, len) steps from (x, v); or, when hx is not null, xh of every step saved
+// to hx (len x n3), stopping at the last step's xh (x and v are then left part way) and,
+// when hperm is not null and the force evaluation ordered the particles (FP32), that
+// order to hperm (len x n), with have[st] = 1
 static int forward_segment(gh_ctx* c, double* x, double* v, const double* m, int64_t n,
                            double dt, double eps, int precision, int64_t len, double* hx,
-                           hipStream_t s)
+                           u32* hperm, std::vector<char>* have, hipStream_t s)
 {
     const int64_t n3 = 3 * n;
     GH_TRY(ensure(c, c->xh, sizeof(double) * (size_t)n3));
     GH_TRY(ensure(c, c->acc64, sizeof(double) * (size_t)n3));
     double* xh = c->xh.as<double>();
     double* acc = c->acc64.as<double>();
+    if (have) have->assign((size_t)len, 0);
     GH_TRY(half_drift(c, x, v, n, dt, xh, s));
     for (int64_t st = 0; st < len; st++) {
-        if (hx) GH_TRY(copy_d(c, xh, hx + st * n3, n3, s));
-        if (n > 1)
+        if (hx) {
+            GH_TRY(copy_d(c, xh, hx + st * n3, n3, s));
+            // the reverse sweep needs only the last step's half-step positions
+            if (st == len - 1) break;
+        }
+        if (n > 1) {
+            OrderBufs& o = c->ord_p;
+            o.sorted = nullptr;
             GH_TRY(direct_dev(c, xh, m, n, nullptr, n, eps, precision, acc, s));
-        else
+            if (hperm && have && o.sorted == xh && o.sorted_n == n) {
+                hipLaunchKernelGGL(k_copy_u32, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s,
+                                   o.perm(), hperm + st * n, n);
+                GH_TRY(check_launch(c, "k_copy_u32"));
+                (*have)[(size_t)st] = 1;
+            }
+        } else {
             GH_TRY(zero_dev(c, acc, n3, s));
+        }
         GH_TRY(kick_drift(c, acc, n, dt, xh, v, x, nullptr, nullptr, s));
     }
     return 0;
@@ -476,7 +522,8 @@ int gh_adjoint_forward(gh_ctx* c, double* d_x, double* d_v, const double* d_m, i
         GH_TRY(copy_d(c, d_x, d_ckpt + (2 * seg) * n3, n3, s));
         GH_TRY(copy_d(c, d_v, d_ckpt + (2 * seg + 1) * n3, n3, s));
         const int64_t len = std::min(every, nsteps - s0);
-        GH_TRY(forward_segment(c, d_x, d_v, d_m, n, dt, eps, precision, len, nullptr, s));
+        GH_TRY(forward_segment(c, d_x, d_v, d_m, n, dt, eps, precision, len, nullptr, nullptr,
+                               nullptr, s));
     }
     return 0;
 }
@@ -504,15 +551,22 @@ int gh_adjoint_backward(gh_ctx* c, const double* d_ckpt, const double* d_m, int6
     double* segx = c->adj_seg.as<double>();            // every x n3 half-step positions
     double* x = segx + every * n3;                      // replay state
     double* v = x + n3;
+    u32* segp = nullptr;                                // every x n spatial orders
+    if (precision == GH_FP32 && n > 1) {
+        GH_TRY(ensure(c, c->adj_perm, sizeof(u32) * (size_t)n * (size_t)every));
+        segp = c->adj_perm.as<u32>();
+    }
+    std::vector<char> have;
     const int64_t nseg = (nsteps + every - 1) / every;
     for (int64_t seg = nseg - 1; seg >= 0; seg--) {
         const int64_t s0 = seg * every;
         const int64_t len = std::min(every, nsteps - s0);
         GH_TRY(copy_d(c, d_ckpt + (2 * seg) * n3, x, n3, s));
         GH_TRY(copy_d(c, d_ckpt + (2 * seg + 1) * n3, v, n3, s));
-        GH_TRY(forward_segment(c, x, v, d_m, n, dt, eps, precision, len, segx, s));
+        GH_TRY(forward_segment(c, x, v, d_m, n, dt, eps, precision, len, segx, segp, &have, s));
         for (int64_t st = len - 1; st >= 0; st--)
-            GH_TRY(reverse_step(c, w, segx + st * n3, d_m, n, dt, eps, d_lx, d_lv, d_lm, s));
+            GH_TRY(reverse_step(c, w, segx + st * n3, d_m, n, dt, eps, d_lx, d_lv, d_lm,
+                                have[(size_t)st] ? segp + st * n : nullptr, s));
     }
     return 0;
 }

@@ -453,7 +453,7 @@ void gh_ctx_destroy(gh_ctx* c)
                       &c->src32a, &c->src32b, &c->tgt32,  &c->xs,    &c->vs,
                       &c->xh,     &c->hist_p, &c->hist_v, &c->scratch, &c->counters,
                       &c->work,   &c->eps_blk, &c->adj_work, &c->adj_src, &c->adj_mu,
-                      &c->adj_mv, &c->adj_part, &c->adj_seg, &c->slot};
+                      &c->adj_mv, &c->adj_part, &c->adj_seg, &c->adj_perm, &c->slot};
     for (DevBuf* b : bufs) release(c, *b);
     release(c, c->ext);
     for (OrderBufs* o : {&c->ord_p, &c->ord_t}) {

@@ -70,6 +70,7 @@ __global__ __launch_bounds__(kBlock) void k_morton(const double* __restrict__ po
 
 int spatial_order(gh_ctx* c, OrderBufs& o, const double* d_pos, int64_t n, hipStream_t s)
 {
+    o.sorted = nullptr;
     if (n <= 0) return 0;
     for (int k = 0; k < 2; k++) {
         GH_TRY(ensure(c, o.keys[k], sizeof(u64) * (size_t)n));
@@ -86,7 +87,10 @@ int spatial_order(gh_ctx* c, OrderBufs& o, const double* d_pos, int64_t n, hipSt
     // blocks' compactness, and so the forces' rounding, by little (Plummer sphere,
     // 65536 particles, 1024-particle blocks: median relative error 3.0e-8 with 10 levels,
     // 3.4e-8 with 16) and cost two more passes.
-    return radix_sort(c, o.keys, o.idx, o.hist, n, 1, kKeyShift, s, &o.sel, 63);
+    GH_TRY(radix_sort(c, o.keys, o.idx, o.hist, n, 1, kKeyShift, s, &o.sel, 63));
+    o.sorted = d_pos;
+    o.sorted_n = n;
+    return 0;
 }
 
 // Root box (as k_bbox_final with eps = 0) of each segment's real points, one workgroup
@@ -166,6 +170,7 @@ int spatial_order_seg(gh_ctx* c, OrderBufs& o, const double* d_pos, int64_t n_pa
                       int nseg, const int* seg, const int64_t* start, const int64_t* count,
                       hipStream_t s)
 {
+    o.sorted = nullptr;
     if (n_pad <= 0) return 0;
     for (int k = 0; k < 2; k++) {
         GH_TRY(ensure(c, o.keys[k], sizeof(u64) * (size_t)n_pad));

@@ -47,6 +47,8 @@ struct OrderBufs {
     DevBuf flag;   // int per group: 1 if the group is not compact (see k_frames)
     DevBuf nflag;  // int: number of flagged groups (cleared by the key kernels)
     int sel = 0;   // which of idx[0] / idx[1] holds the order
+    const double* sorted = nullptr;  // positions spatial_order last ordered (n of them)
+    int64_t sorted_n = 0;
     const u32* perm() const { return idx[sel].as<u32>(); }
 };
 
@@ -69,7 +71,7 @@ struct gh_ctx {
     std::vector<int64_t> sym_band_start;        // its bands' first items
     float sym_eps2 = -1.f;          // the softening eps_blk holds
     // adjoint workspaces (gh_adjoint.hip)
-    gh::DevBuf adj_work, adj_src, adj_mu, adj_mv, adj_part, adj_seg;
+    gh::DevBuf adj_work, adj_src, adj_mu, adj_mv, adj_part, adj_seg, adj_perm;
     int adj_nb = -1, adj_ipt = -1;
     gh::DevBuf slot;  // history slot counter of graph-replayed runs
     gh::OrderBufs ord_p, ord_t;  // spatial order of sources and of targets (FP32 paths)
