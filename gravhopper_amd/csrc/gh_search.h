@@ -96,26 +96,46 @@ GH_HD EndSearch end_search(const unsigned char* base, int64_t n)
     return es;
 }
 
-// Last sorted position of the node of level L that starts at s.
-GH_HD int64_t node_end_dp(const EndSearch& es, int64_t n, int64_t s, int L)
+// The first groups a search from s reads, on levels 0..2: they depend on s only, so a
+// caller ending several nodes that start at s loads them once.
+struct EndStart {
+    int64_t s, g0, p1, g1, p2, g2;
+    Bytes16 w0, w1, w2;
+};
+
+GH_HD EndStart end_start(const EndSearch& es, int64_t s)
 {
-    const int64_t g0 = s >> 4, p1 = g0 + 1, g1 = p1 >> 4, p2 = g1 + 1, g2 = p2 >> 4;
-    const Bytes16 w0 = load16(es.base, g0), w1 = load16(es.m1, g1), w2 = load16(es.m2, g2);
-    int r = first_le16(w0, (int)(s & 15), L);
-    if (r < 16) return (g0 << 4) + r;
-    r = first_le16(w1, (int)(p1 & 15), L);
+    EndStart st;
+    st.s = s;
+    st.g0 = s >> 4;
+    st.p1 = st.g0 + 1;
+    st.g1 = st.p1 >> 4;
+    st.p2 = st.g1 + 1;
+    st.g2 = st.p2 >> 4;
+    st.w0 = load16(es.base, st.g0);
+    st.w1 = load16(es.m1, st.g1);
+    st.w2 = load16(es.m2, st.g2);
+    return st;
+}
+
+// Last sorted position of the node of level L that starts at st.s.
+GH_HD int64_t node_end_from(const EndSearch& es, int64_t n, const EndStart& st, int L)
+{
+    int r = first_le16(st.w0, (int)(st.s & 15), L);
+    if (r < 16) return (st.g0 << 4) + r;
+    r = first_le16(st.w1, (int)(st.p1 & 15), L);
     if (r < 16) {
-        const int64_t q1 = (g1 << 4) + r;
+        const int64_t q1 = (st.g1 << 4) + r;
         return (q1 << 4) + first_le16(load16(es.base, q1), 0, L);
     }
-    r = first_le16(w2, (int)(p2 & 15), L);
+    r = first_le16(st.w2, (int)(st.p2 & 15), L);
     if (r < 16) {
-        const int64_t q2 = (g2 << 4) + r;
+        const int64_t q2 = (st.g2 << 4) + r;
         const int64_t q1 = (q2 << 4) + first_le16(load16(es.m1, q2), 0, L);
         return (q1 << 4) + first_le16(load16(es.base, q1), 0, L);
     }
     // large nodes: up one level at a time, then down through every level
-    int64_t p = g2 + 1;
+    int64_t p = st.g2 + 1;
     for (int k = 3; k < es.levels; k++) {
         const int64_t g = p >> 4;
         r = first_le16(load16(es.base + search_level_offset(n, k), g), (int)(p & 15), L);
@@ -128,6 +148,12 @@ GH_HD int64_t node_end_dp(const EndSearch& es, int64_t n, int64_t s, int L)
         p = g + 1;
     }
     return n - 1;  // not reached: the top level's one group holds position n-1's entry
+}
+
+// Last sorted position of the node of level L that starts at s.
+GH_HD int64_t node_end_dp(const EndSearch& es, int64_t n, int64_t s, int L)
+{
+    return node_end_from(es, n, end_start(es, s), L);
 }
 
 }  // namespace gh

@@ -670,29 +670,22 @@ constexpr int kFillF64 = 1;   // NodeD + leaf flags (both walks)
 constexpr int kFillKids = 2;  // children lists (FP32 group walk)
 constexpr int kFillDump = 4;  // levels and branches (canonical dump)
 
-__device__ void emit_node(const FillArgs& a, int64_t n, int64_t id, int lv, int br, double cx,
-                          double cy, double cz, double size, int64_t s, int64_t e, bool is_leaf,
-                          u32 p)
+// Node id of level lv and branch br with centre c and size, ending at sorted position
+// e, centre of mass mc and mass m; p = the particle of a leaf.
+__device__ __forceinline__ void emit_node(const FillArgs& a, int64_t id, int lv, int br,
+                                          double cx, double cy, double cz, double size,
+                                          int64_t e, bool is_leaf, u32 p, double mcx,
+                                          double mcy, double mcz, double m)
 {
     NodeD d;
     d.cx = cx;
     d.cy = cy;
     d.cz = cz;
     d.size = is_leaf ? -size : size;
-    if (is_leaf) {
-        const double4 q = a.spos[s];
-        d.mass = q.w;
-        d.mx = q.x;
-        d.my = q.y;
-        d.mz = q.z;
-    } else {
-        const DD4 hi = a.pref.at(e + 1), lo = a.pref.at(s);
-        const double m = dd_diff(hi, lo, 0);
-        d.mass = m;
-        d.mx = dd_diff(hi, lo, 1) / m;
-        d.my = dd_diff(hi, lo, 2) / m;
-        d.mz = dd_diff(hi, lo, 3) / m;
-    }
+    d.mx = mcx;
+    d.my = mcy;
+    d.mz = mcz;
+    d.mass = m;
     a.skip[id] = (int)a.off.at(e + 1);
     if (a.what & (kFillF64 | kFillDump)) {
         a.nodes64[id] = d;
@@ -732,15 +725,24 @@ __global__ __launch_bounds__(kFillBlock) void k_fill_nodes(FillArgs a, int64_t n
         br = descend(x, y, z, cx, cy, cz, h);
         size = h;
     }
-    for (int k = 0; k < n_int; k++) {
-        const int lv = first + k;
-        const int64_t e = node_end_dp(a.es, n, i, lv);
-        emit_node(a, n, base + k, lv, br, cx, cy, cz, size, i, e, false, p);
-        const double h = 0.5 * size;
-        br = descend(x, y, z, cx, cy, cz, h);
-        size = h;
+    if (n_int > 0) {
+        // what every internal node starting at i shares: the first end-search groups and
+        // the moment prefix at i
+        const EndStart st = end_start(a.es, i);
+        const DD4 lo = a.pref.at(i);
+        for (int k = 0; k < n_int; k++) {
+            const int lv = first + k;
+            const int64_t e = node_end_from(a.es, n, st, lv);
+            const DD4 hi = a.pref.at(e + 1);
+            const double m = dd_diff(hi, lo, 0);
+            emit_node(a, base + k, lv, br, cx, cy, cz, size, e, false, p,
+                      dd_diff(hi, lo, 1) / m, dd_diff(hi, lo, 2) / m, dd_diff(hi, lo, 3) / m, m);
+            const double h = 0.5 * size;
+            br = descend(x, y, z, cx, cy, cz, h);
+            size = h;
+        }
     }
-    emit_node(a, n, base + n_int, first + n_int, br, cx, cy, cz, size, i, i, true, p);
+    emit_node(a, base + n_int, first + n_int, br, cx, cy, cz, size, i, true, p, x, y, z, q.w);
 }
 
 // ------------------------------------------------------------------------------
@@ -1017,18 +1019,22 @@ __device__ __forceinline__ void walk_lists(
         }
         if (k > room) k = room;
         const bool active = lane < k;
-        // the popped node's record and children, loaded together (inactive lanes load the
-        // root's, and ignore them)
+        // the popped node's record (inactive lanes load the root's, and ignore it); its
+        // children only where it is opened, so that the lanes of accepted nodes and
+        // leaves (most of them) send no request for them
         const int n = active ? stack[top - 1 - lane] : 0;
         const double4 rec = *(const double4*)(nodes + n);
         const double4 com = com_of(n);
-        const int4 ka = *(const int4*)(kids + 8 * (int64_t)n);
-        const int4 kb = *(const int4*)(kids + 8 * (int64_t)n + 4);
         float cx, cy, cz, R2;
         bool need;
         bool acc_n = box_test(rec, cx, cy, cz, R2, need) && active;
         need = need && active;
         if (target_tests(need, cx, cy, cz, R2)) acc_n = true;
+        int4 ka = make_int4(-1, -1, -1, -1), kb = ka;
+        if (active && !acc_n) {
+            ka = *(const int4*)(kids + 8 * (int64_t)n);
+            kb = *(const int4*)(kids + 8 * (int64_t)n + 4);
+        }
         // children come first in kids[8 n ..], then -1s: 8 plus the sign bits
         int nk = 8 + (ka.x >> 31) + (ka.y >> 31) + (ka.z >> 31) + (ka.w >> 31) + (kb.x >> 31) +
                  (kb.y >> 31) + (kb.z >> 31) + (kb.w >> 31);
@@ -1166,23 +1172,22 @@ __global__ __launch_bounds__(kGroupBlock) void k_walk_group(
     }
 }
 
-// Children of every internal node, in pre-order: kids[8 n + k], -1 after the last.  The
-// children of n are n + 1 and then, while below skip[n], the skip pointer of the previous
-// child.
-__global__ __launch_bounds__(kBlock) void k_kids(const NodeD* __restrict__ nodes,
-                                                 const int* __restrict__ skip, int n_nodes,
+// Children of every node, in pre-order: kids[8 n + k], -1 after the last.  The children
+// of n are n + 1 and then, while below skip[n], the skip pointer of the previous child; a
+// leaf (skip[n] = n + 1) has none.  The walk reads only the lists of nodes it opens, but
+// leaves' lists are written too: they share cache lines with internal nodes' lists, and
+// a line left partly unwritten costs the walk a fetch of the rest from memory.
+__global__ __launch_bounds__(kBlock) void k_kids(const int* __restrict__ skip, int n_nodes,
                                                  int* __restrict__ kids)
 {
     const int n = (int)(blockIdx.x * kBlock + threadIdx.x);
     if (n >= n_nodes) return;
+    const int end = skip[n];
     int k[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
-    if (__double_as_longlong(nodes[n].size) >= 0) {  // internal
-        const int end = skip[n];
-        int c = n + 1, q = 0;
-        while (c < end && q < 8) {
-            k[q++] = c;
-            c = skip[c];
-        }
+    int c = n + 1, q = 0;
+    while (c < end && q < 8) {
+        k[q++] = c;
+        c = skip[c];
     }
     int4* o = (int4*)(kids + 8 * (int64_t)n);
     o[0] = make_int4(k[0], k[1], k[2], k[3]);
@@ -1483,8 +1488,7 @@ static int build_tree(gh_ctx* c, const double* d_pos, const double* d_mass, int6
     GH_TRY(check_launch(c, "k_fill_nodes"));
     if (what & kFillKids) {
         hipLaunchKernelGGL(k_kids, dim3(blocks_for(total, kBlock)), dim3(kBlock), 0, s,
-                           tb->nodes64.as<NodeD>(), tb->skip.as<int>(), (int)total,
-                           tb->kids.as<int>());
+                           tb->skip.as<int>(), (int)total, tb->kids.as<int>());
         GH_TRY(check_launch(c, "k_kids"));
     }
     c->last_tree_nodes = total;
