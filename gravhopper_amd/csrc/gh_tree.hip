@@ -76,6 +76,7 @@ struct TreeBufs {
     DevBuf nodes64, kids, skip, level, branch, leaf;
     DevBuf tkeys[2], tidx[2];  // Morton order of position targets (tree_force_position)
     long long* readback = nullptr;  // pinned host words: flags[2], node count
+    long long* readback_dev = nullptr;  // the same words' device address
     int64_t n_nodes = 0;
     int words = 0;
     int key_sel = 0;  // which of keys[0]/keys[1] holds the sorted keys
@@ -471,8 +472,8 @@ __global__ __launch_bounds__(kBlock) void k_delta(DeltaArgs a, int64_t n, int wo
 
 // End-search levels 3 and up, each from the level below, by one workgroup (the levels
 // above 2 hold at most n / 4096 entries).
-__global__ __launch_bounds__(kBlock) void k_search_levels(unsigned char* __restrict__ dp,
-                                                          int64_t n, int levels)
+__device__ __forceinline__ void search_levels_block(unsigned char* __restrict__ dp, int64_t n,
+                                                    int levels)
 {
     for (int k = 3; k < levels; k++) {
         const unsigned char* lo = dp + search_level_offset(n, k - 1);
@@ -543,15 +544,16 @@ struct OpDD4 {
 constexpr int kScanItems = 4;
 constexpr int kScanChunk = kBlock * kScanItems;
 
-// in[0..n) -> out[0..n] exclusive (out[0] = 0, out[n] = total); phase 1 writes the
-// block-local exclusive prefix and the block totals.
+// Local phase of an exclusive scan of in[0..n), for this workgroup's chunk: out[i] =
+// sum of the chunk's elements before i, and sums[b] = the chunk's total.  sh is LDS for
+// kBlock elements.
 template <class Op>
-__global__ __launch_bounds__(kBlock) void k_scan_local(const typename Op::In* __restrict__ in,
-                                                       int64_t n, typename Op::T* __restrict__ out,
-                                                       typename Op::T* __restrict__ sums)
+__device__ __forceinline__ void scan_local_block(const typename Op::In* __restrict__ in,
+                                                 int64_t n, typename Op::T* __restrict__ out,
+                                                 typename Op::T* __restrict__ sums,
+                                                 typename Op::T* sh)
 {
     typedef typename Op::T T;
-    __shared__ T sh[kBlock];
     const int64_t b = (int64_t)blockIdx.x * kScanChunk + (int64_t)threadIdx.x * kScanItems;
     T v[kScanItems];
     T run = Op::zero();
@@ -575,14 +577,15 @@ __global__ __launch_bounds__(kBlock) void k_scan_local(const typename Op::In* __
     if (threadIdx.x == kBlock - 1) sums[blockIdx.x] = sh[kBlock - 1];
 }
 
-// Exclusive scan of the nb block totals by one workgroup (each thread a contiguous
-// run), total written to *total.
+// Exclusive scan in place of the nb chunk totals by one workgroup (each thread a
+// contiguous run); the total goes to *total and is returned to every thread.
 template <class Op>
-__global__ __launch_bounds__(kBlock) void k_scan_sums(typename Op::T* __restrict__ sums,
-                                                      int64_t nb, typename Op::T* __restrict__ total)
+__device__ __forceinline__ typename Op::T scan_sums_block(typename Op::T* __restrict__ sums,
+                                                          int64_t nb,
+                                                          typename Op::T* __restrict__ total,
+                                                          typename Op::T* sh)
 {
     typedef typename Op::T T;
-    __shared__ T sh[kBlock];
     const int64_t per = (nb + kBlock - 1) / kBlock;
     const int64_t b = (int64_t)threadIdx.x * per;
     const int64_t e = b + per < nb ? b + per : nb;
@@ -604,7 +607,44 @@ __global__ __launch_bounds__(kBlock) void k_scan_sums(typename Op::T* __restrict
         sums[k] = acc;
         acc = Op::add(acc, t);
     }
-    if (threadIdx.x == kBlock - 1) *total = sh[kBlock - 1];
+    const T tot = sh[kBlock - 1];
+    if (threadIdx.x == kBlock - 1) *total = tot;
+    return tot;
+}
+
+// The fill's two scans over the sorted particles, local phase: node counts cnt -> first
+// node ids off, and moments (m, m x, m y, m z) of spos -> double-double prefixes pref.
+__global__ __launch_bounds__(kBlock) void k_scan_local_fill(
+    const long long* __restrict__ cnt, const double4* __restrict__ spos, int64_t n,
+    long long* __restrict__ off, long long* __restrict__ off_sums, DD4* __restrict__ pref,
+    DD4* __restrict__ pref_sums)
+{
+    __shared__ long long shi[kBlock];
+    __shared__ DD4 shd[kBlock];
+    scan_local_block<OpI64>(cnt, n, off, off_sums, shi);
+    scan_local_block<OpDD4>(spos, n, pref, pref_sums, shd);
+}
+
+// One workgroup after k_scan_local_fill: both scans' chunk totals (totals to off[n] and
+// pref[n]), the end-search levels 3 and up, and the words the host reads back, written
+// to its pinned memory: the two flag counts of k_delta and the node count.
+__global__ __launch_bounds__(kBlock) void k_scan_sums_fill(
+    long long* __restrict__ off_sums, DD4* __restrict__ pref_sums, int64_t nb,
+    long long* __restrict__ off_total, DD4* __restrict__ pref_total,
+    unsigned char* __restrict__ dp, int64_t n, int levels, const int* __restrict__ flags,
+    long long* __restrict__ host)
+{
+    __shared__ long long shi[kBlock];
+    __shared__ DD4 shd[kBlock];
+    const long long nodes = scan_sums_block<OpI64>(off_sums, nb, off_total, shi);
+    scan_sums_block<OpDD4>(pref_sums, nb, pref_total, shd);
+    search_levels_block(dp, n, levels);
+    if (threadIdx.x == 0) {
+        int* hf = (int*)host;
+        hf[0] = flags[0];
+        hf[1] = flags[1];
+        host[1] = nodes;
+    }
 }
 
 // A scan kept in two parts, block-local exclusive prefixes and the exclusive prefix of
@@ -621,26 +661,6 @@ struct ScanView {
         return j == n ? local[n] : Op::add(blocks[j / kScanChunk], local[j]);
     }
 };
-
-// Exclusive scan of in[0..n) in two parts (ScanView): out must hold n+1 elements and
-// tmp at least blocks+1 elements.
-template <class Op>
-static int scan_exclusive(gh_ctx* c, const typename Op::In* in, int64_t n, typename Op::T* out,
-                          typename Op::T* tmp, hipStream_t s, ScanView<Op>* view)
-{
-    const int64_t nb = (n + kScanChunk - 1) / kScanChunk;
-    if (nb > 0) {
-        hipLaunchKernelGGL(k_scan_local<Op>, dim3((unsigned)nb), dim3(kBlock), 0, s, in, n, out,
-                           tmp);
-        GH_TRY(check_launch(c, "k_scan_local"));
-    }
-    hipLaunchKernelGGL(k_scan_sums<Op>, dim3(1), dim3(kBlock), 0, s, tmp, nb, out + n);
-    GH_TRY(check_launch(c, "k_scan_sums"));
-    view->local = out;
-    view->blocks = tmp;
-    view->n = n;
-    return 0;
-}
 
 // ------------------------------------------------------------------------------
 // 5./6. node ends, moments, node records
@@ -1382,8 +1402,12 @@ static int build_tree(gh_ctx* c, const double* d_pos, const double* d_mass, int6
     unsigned char* const dp = tb->delta.as<unsigned char>();
     const int search_lv = search_levels(n);
     GH_TRY(ensure(c, tb->flags, sizeof(int) * 4));
-    if (!tb->readback)
-        GH_HIP(c, hipHostMalloc((void**)&tb->readback, 4 * sizeof(long long), hipHostMallocDefault));
+    if (!tb->readback) {
+        // coherent, so that the kernel's stores reach the host without any cache flush
+        GH_HIP(c, hipHostMalloc((void**)&tb->readback, 4 * sizeof(long long),
+                                hipHostMallocMapped | hipHostMallocCoherent));
+        GH_HIP(c, hipHostGetDevicePointer((void**)&tb->readback_dev, tb->readback, 0));
+    }
     const int64_t nb = (n + kScanChunk - 1) / kScanChunk;
     GH_TRY(ensure(c, tb->off, sizeof(long long) * (size_t)(2 * n + 2)));
     GH_TRY(ensure(c, tb->block_sums, sizeof(DD4) * (size_t)(nb + 1)));
@@ -1427,21 +1451,32 @@ static int build_tree(gh_ctx* c, const double* d_pos, const double* d_mass, int6
         hipLaunchKernelGGL(k_delta, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, da, n,
                            words);
         GH_TRY(check_launch(c, "k_delta"));
-        if (search_lv > 3) {
-            hipLaunchKernelGGL(k_search_levels, dim3(1), dim3(kBlock), 0, s, dp, n, search_lv);
-            GH_TRY(check_launch(c, "k_search_levels"));
+        // 5./6. first node ids (scan of the node counts) and moments (double-double
+        // prefix sums of (m, m x, m y, m z)), each kept as chunk prefixes plus the
+        // prefix of the chunk totals (ScanView); the end-search levels above 2; and the
+        // readback of (flags, node count)
+        if (nb > 0) {
+            hipLaunchKernelGGL(k_scan_local_fill, dim3((unsigned)nb), dim3(kBlock), 0, s, cnt,
+                               tb->spos.as<double4>(), n, off,
+                               tb->block_counts.as<long long>(), pref,
+                               tb->block_sums.as<DD4>());
+            GH_TRY(check_launch(c, "k_scan_local_fill"));
         }
-        // 5. numbering
-        GH_TRY(scan_exclusive<OpI64>(c, cnt, n, off, tb->block_counts.as<long long>(), s,
-                                     &off_view));
-        // 6. moments: exclusive double-double prefix sums of (m, m x, m y, m z)
-        GH_TRY(scan_exclusive<OpDD4>(c, tb->spos.as<double4>(), n, pref,
-                                     tb->block_sums.as<DD4>(), s, &pref_view));
-        GH_HIP(c, hipMemcpyAsync(tb->readback, tb->flags.p, 2 * sizeof(int),
-                                 hipMemcpyDeviceToHost, s));
-        GH_HIP(c, hipMemcpyAsync(tb->readback + 1, off + n, sizeof(long long),
-                                 hipMemcpyDeviceToHost, s));
+        tb->readback[0] = tb->readback[1] = -1;
+        hipLaunchKernelGGL(k_scan_sums_fill, dim3(1), dim3(kBlock), 0, s,
+                           tb->block_counts.as<long long>(), tb->block_sums.as<DD4>(), nb,
+                           off + n, pref + n, dp, n, search_lv, tb->flags.as<int>(),
+                           tb->readback_dev);
+        GH_TRY(check_launch(c, "k_scan_sums_fill"));
+        off_view.local = off;
+        off_view.blocks = tb->block_counts.as<long long>();
+        off_view.n = n;
+        pref_view.local = pref;
+        pref_view.blocks = tb->block_sums.as<DD4>();
+        pref_view.n = n;
         GH_HIP(c, hipStreamSynchronize(s));
+        if (tb->readback[1] < 0)
+            return fail(c, GH_EHIP, "internal: the tree build's readback was not written");
         const int* got = (const int*)tb->readback;
         if (got[1] > 0)
             return fail(c, GH_ECOINCIDENT,
