@@ -64,6 +64,18 @@ struct NodeD {
 typedef double f64x8 __attribute__((ext_vector_type(8)));  // a NodeD as one load
 static_assert(sizeof(NodeD) == sizeof(f64x8), "NodeD is 64 bytes");
 
+// FP32 group walk's node record, 48 bytes read as three 16-byte loads: the FP64 centre
+// (a leaf's is its particle's position, as a leaf is accepted without a test), the FP32
+// size with a leaf's sign bit set, the FP32 mass, and the FP32 offset of the centre of
+// mass from the centre (zero for a leaf, so that its entry is its exact position).
+// Rounding the offset moves a monopole by at most 2^-24 of its node's size.
+struct alignas(16) NodeW {
+    double cx, cy, cz;
+    float size, mass;
+    float ox, oy, oz, pad;
+};
+static_assert(sizeof(NodeW) == 48, "NodeW is 48 bytes");
+
 struct DD4 {
     double h[4], l[4];
 };
@@ -73,7 +85,7 @@ struct TreeBufs {
     DevBuf idx[2];
     DevBuf hist, bbox_part, box, flags, delta, off, prefix, block_sums, block_counts, packed;
     DevBuf spos;  // (x, y, z, m) of the particles in sorted order
-    DevBuf nodes64, kids, skip, level, branch, leaf;
+    DevBuf nodes64, nodesw, kids, skip, level, branch, leaf;
     DevBuf tkeys[2], tidx[2];  // Morton order of position targets (tree_force_position)
     long long* readback = nullptr;  // pinned host words: flags[2], node count
     long long* readback_dev = nullptr;  // the same words' device address
@@ -679,6 +691,7 @@ struct FillArgs {
     const double* box;
     ScanView<OpDD4> pref;  // moment prefix sums, n+1 entries
     NodeD* nodes64;
+    NodeW* nodesw;
     int* skip;
     short* level;
     signed char* branch;
@@ -686,9 +699,10 @@ struct FillArgs {
     int what;   // kFill* bits: which records this build needs
 };
 
-constexpr int kFillF64 = 1;   // NodeD + leaf flags (both walks)
+constexpr int kFillF64 = 1;   // NodeD + leaf flags (FP64 walk, dump)
 constexpr int kFillKids = 2;  // children lists (FP32 group walk)
 constexpr int kFillDump = 4;  // levels and branches (canonical dump)
+constexpr int kFillW = 8;     // NodeW records (FP32 group walk)
 
 // Node id of level lv and branch br with centre c and size, ending at sorted position
 // e, centre of mass mc and mass m; p = the particle of a leaf.
@@ -714,6 +728,27 @@ __device__ __forceinline__ void emit_node(const FillArgs& a, int64_t id, int lv,
     if (a.what & kFillDump) {
         a.level[id] = (short)lv;
         a.branch[id] = (signed char)br;
+    }
+    if (a.what & kFillW) {
+        NodeW w;
+        if (is_leaf) {
+            w.cx = mcx;
+            w.cy = mcy;
+            w.cz = mcz;
+            w.ox = w.oy = w.oz = 0.f;
+            w.size = -(float)size;
+        } else {
+            w.cx = cx;
+            w.cy = cy;
+            w.cz = cz;
+            w.ox = (float)(mcx - cx);
+            w.oy = (float)(mcy - cy);
+            w.oz = (float)(mcz - cz);
+            w.size = (float)size;
+        }
+        w.mass = (float)m;
+        w.pad = 0.f;
+        a.nodesw[id] = w;
     }
 }
 
@@ -799,9 +834,10 @@ __device__ __forceinline__ int64_t xcd_block(int64_t per_xcd)
 // leaves are accepted as in the reference.
 //
 // Traversal: an LDS stack of node ids; each step the 64 lanes take up to 64 ids from
-// its top and test them in parallel (centre and size from the FP64 record, re-based to
-// the frame).  An accepted node's centre of mass, re-based from FP64, goes to the
-// group's interaction list; an opened node's children (kids[8 n ..]) are pushed.  Each
+// its top and test them in parallel (the FP64 centre of the node's NodeW record,
+// re-based to the frame, and its FP32 size).  An accepted node's centre of mass (centre
+// plus offset, re-based in FP64) goes to the group's interaction list; an opened node's
+// children (kids[8 n ..]) are pushed.  Each
 // popped id pushes at most 8, so a step pops at most (kStackCap - top) / 7 ids; with
 // less room left than one id needs, the wave walks the top id's subtree one node at a
 // time with the skip pointers instead, pushing nothing.
@@ -860,6 +896,30 @@ __device__ __forceinline__ int lanes_below(u64 m)
     return (int)__builtin_amdgcn_mbcnt_hi((u32)(m >> 32), __builtin_amdgcn_mbcnt_lo((u32)m, 0u));
 }
 
+// A NodeW in registers.
+struct WalkRec {
+    double cx, cy, cz;
+    float size, mass, ox, oy, oz;
+};
+
+__device__ __forceinline__ WalkRec load_rec(const NodeW* __restrict__ nodes, int n)
+{
+    const char* p = (const char*)(nodes + n);
+    const double2 a = *(const double2*)p;        // cx, cy
+    const int4 b = *(const int4*)(p + 16);       // cz, size, mass
+    const float4 c = *(const float4*)(p + 32);   // offset of the centre of mass
+    WalkRec r;
+    r.cx = a.x;
+    r.cy = a.y;
+    r.cz = __hiloint2double(b.y, b.x);
+    r.size = __int_as_float(b.z);
+    r.mass = __int_as_float(b.w);
+    r.ox = c.x;
+    r.oy = c.y;
+    r.oz = c.z;
+    return r;
+}
+
 // One walk with the opening box B (centre bc, half-sizes bh) in the frame o: traversal
 // and evaluation as described above.  This lane's target is (dx0, dy0, dz0) = x - o
 // (FP64; t its FP32 rounding), its slice of each flush `slice` of `nslices`, and the
@@ -867,7 +927,7 @@ __device__ __forceinline__ int lanes_below(u64 m)
 // entries.  Adds the lane's partial field to (ax, ay, az); listed = the list length.
 template <bool F64, bool GUARD>
 __device__ __forceinline__ void walk_lists(
-    const NodeD* __restrict__ nodes, const int* __restrict__ kids, const int* __restrict__ skip,
+    const NodeW* __restrict__ nodes, const int* __restrict__ kids, const int* __restrict__ skip,
     int n_nodes, double ox, double oy, double oz, float bcx, float bcy, float bcz, float bhx,
     float bhy, float bhz, double dx0, double dy0, double dz0, int slice, int nslices,
     float inv_theta, double eps2, int* stack, typename GroupEntry<F64>::T* list,
@@ -929,29 +989,30 @@ __device__ __forceinline__ void walk_lists(
         held = rest;
         wave_sync();
     };
-    // the list entry of a node: centre of mass and mass, re-based to the frame
-    auto entry_of = [&](const double4 m) -> Entry {
+    // the list entry of a node: centre of mass (centre plus offset) and mass, re-based to
+    // the frame in FP64
+    auto entry_of = [&](const WalkRec& r) -> Entry {
         Entry e;
-        e.x = m.x - ox; e.y = m.y - oy; e.z = m.z - oz; e.w = m.w;
+        e.x = (r.cx - ox) + (double)r.ox;
+        e.y = (r.cy - oy) + (double)r.oy;
+        e.z = (r.cz - oz) + (double)r.oz;
+        e.w = r.mass;
         return e;
     };
-    auto com_of = [&](int n) -> double4 {
-        return *(const double4*)((const double*)(nodes + n) + 4);
-    };
-    // The group opening test (identical in both instances) of the node with record rec =
+    // The group opening test (identical in both instances) of the node with record rec
     // (centre, size with the leaf's sign bit): accept = leaf, or the box B clears the
     // node's opening radius R; need = B does not clear it but its far corner does, so
     // that every target may still pass the reference's test on its own (then the
     // per-target tests below decide).  c: the centre in the frame; R2 = R^2.
-    auto box_test = [&](const double4 rec, float& cx, float& cy, float& cz, float& R2,
+    auto box_test = [&](const WalkRec& rec, float& cx, float& cy, float& cz, float& R2,
                         bool& need) -> bool {
-        cx = (float)(rec.x - ox);
-        cy = (float)(rec.y - oy);
-        cz = (float)(rec.z - oz);
-        const float R = (float)rec.w * inv_theta;  // +inf for theta = 0
+        cx = (float)(rec.cx - ox);
+        cy = (float)(rec.cy - oy);
+        cz = (float)(rec.cz - oz);
+        const float R = rec.size * inv_theta;  // +inf for theta = 0
         R2 = R * R;
         // branch-free, so that the whole record is loaded at once whatever the node is
-        const bool leaf = __double_as_longlong(rec.w) < 0;
+        const bool leaf = __float_as_int(rec.size) < 0;
         const float ax_ = fabsf(cx - bcx), ay_ = fabsf(cy - bcy), az_ = fabsf(cz - bcz);
         const float ddx = fmaxf(0.f, ax_ - bhx), ddy = fmaxf(0.f, ay_ - bhy),
                     ddz = fmaxf(0.f, az_ - bhz);
@@ -1010,7 +1071,7 @@ __device__ __forceinline__ void walk_lists(
             int p = n0;
             const int end = skip[n0];
             while (p < end) {
-                const double4 rec = *(const double4*)(nodes + p);
+                const WalkRec rec = load_rec(nodes, p);
                 float cx, cy, cz, R2;
                 bool need;
                 bool acc = box_test(rec, cx, cy, cz, R2, need);
@@ -1022,7 +1083,7 @@ __device__ __forceinline__ void walk_lists(
                                      R2)) == 0;
                 }
                 if (acc) {
-                    if (lane == 0) list[held] = entry_of(com_of(p));
+                    if (lane == 0) list[held] = entry_of(rec);
                     held += 1;
                     listed += 1;
                     p = skip[p];
@@ -1043,8 +1104,7 @@ __device__ __forceinline__ void walk_lists(
         // children only where it is opened, so that the lanes of accepted nodes and
         // leaves (most of them) send no request for them
         const int n = active ? stack[top - 1 - lane] : 0;
-        const double4 rec = *(const double4*)(nodes + n);
-        const double4 com = com_of(n);
+        const WalkRec rec = load_rec(nodes, n);
         float cx, cy, cz, R2;
         bool need;
         bool acc_n = box_test(rec, cx, cy, cz, R2, need) && active;
@@ -1067,7 +1127,7 @@ __device__ __forceinline__ void walk_lists(
         int total;
         const int pre = wave_excl_count(nk, total);
         wave_sync();  // every lane has read its popped id
-        if (acc_n) list[held + before] = entry_of(com);
+        if (acc_n) list[held + before] = entry_of(rec);
         if (nk > 0) {
             const int w = top - k + pre;
             const int kk[8] = {ka.x, ka.y, ka.z, ka.w, kb.x, kb.y, kb.z, kb.w};
@@ -1094,7 +1154,7 @@ __device__ __forceinline__ int exponent_f(float x)
 
 template <bool F64, bool GUARD>
 __global__ __launch_bounds__(kGroupBlock) void k_walk_group(
-    const NodeD* __restrict__ nodes, const int* __restrict__ kids, const int* __restrict__ skip,
+    const NodeW* __restrict__ nodes, const int* __restrict__ kids, const int* __restrict__ skip,
     int n_nodes, const double* __restrict__ tgt, const double4* __restrict__ stgt,
     const u32* __restrict__ order, int64_t nt, int G, float inv_theta, double eps2,
     double* __restrict__ acc, int* __restrict__ counts, int64_t per_xcd)
@@ -1499,6 +1559,7 @@ static int build_tree(gh_ctx* c, const double* d_pos, const double* d_mass, int6
         GH_TRY(ensure(c, tb->leaf, sizeof(int) * T));
     }
     if (what & kFillKids) GH_TRY(ensure(c, tb->kids, 8 * sizeof(int) * T));
+    if (what & kFillW) GH_TRY(ensure(c, tb->nodesw, sizeof(NodeW) * T));
     if (what & kFillDump) {
         GH_TRY(ensure(c, tb->level, sizeof(short) * T));
         GH_TRY(ensure(c, tb->branch, T));
@@ -1513,6 +1574,7 @@ static int build_tree(gh_ctx* c, const double* d_pos, const double* d_mass, int6
     fa.box = tb->box.as<double>();
     fa.pref = pref_view;
     fa.nodes64 = tb->nodes64.as<NodeD>();
+    fa.nodesw = tb->nodesw.as<NodeW>();
     fa.skip = tb->skip.as<int>();
     fa.level = tb->level.as<short>();
     fa.branch = tb->branch.as<signed char>();
@@ -1599,7 +1661,7 @@ static int walk(gh_ctx* c, const double* tgt, bool self, int64_t nt, double eps,
     const int64_t n_blocks = (n_groups + kGroupWaves - 1) / kGroupWaves;
     const int64_t per = (n_blocks + kXcds - 1) / kXcds;
     const dim3 grid((unsigned)(per * kXcds));
-    const NodeD* nd = tb->nodes64.as<NodeD>();
+    const NodeW* nd = tb->nodesw.as<NodeW>();
     const int* kd = tb->kids.as<int>();
     const int* sk = tb->skip.as<int>();
     const int T = (int)tb->n_nodes;
@@ -1628,7 +1690,7 @@ int tree_dev(gh_ctx* c, const double* d_pos, const double* d_mass, int64_t np,
     if (precision != GH_FP64 && precision != GH_FP32)
         return fail(c, GH_EARG, "unknown precision %d", precision);
     GH_TRY(build_tree(c, d_pos, d_mass, np, eps, theta,
-                      precision == GH_FP64 ? kFillF64 : (kFillF64 | kFillKids), s));
+                      precision == GH_FP64 ? kFillF64 : (kFillW | kFillKids), s));
     return walk(c, self ? d_pos : d_fpos, self, nt, eps, theta, precision, d_acc, nullptr, s);
 }
 
@@ -1640,7 +1702,7 @@ void tree_free(gh_ctx* c)
                       &tb->bbox_part, &tb->box,   &tb->flags,     &tb->delta,   &tb->off,
                       &tb->prefix,  &tb->spos,      &tb->block_sums, &tb->block_counts,
                       &tb->packed,
-                      &tb->nodes64, &tb->kids,
+                      &tb->nodes64, &tb->nodesw, &tb->kids,
                       &tb->skip,    &tb->level,   &tb->branch,    &tb->leaf,    &tb->tkeys[0],
                       &tb->tkeys[1], &tb->tidx[0], &tb->tidx[1]};
     for (DevBuf* b : bufs) release(c, *b);
@@ -1729,7 +1791,7 @@ static int tree_counts(gh_ctx* c, const double* pos, const double* mass, int64_t
         tgt = c->fpos64.as<double>();
     }
     GH_TRY(build_tree(c, c->pos64.as<double>(), c->mass64.as<double>(), np, eps, theta,
-                      precision == GH_FP64 ? kFillF64 : (kFillF64 | kFillKids), s));
+                      precision == GH_FP64 ? kFillF64 : (kFillW | kFillKids), s));
     GH_TRY(walk(c, tgt, self, nt, eps, theta, precision, c->acc64.as<double>(),
                 c->scratch.as<int>(), s));
     std::vector<int> cnt((size_t)(nt > 0 ? nt : 1));
