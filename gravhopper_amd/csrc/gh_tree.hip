@@ -637,9 +637,10 @@ __global__ __launch_bounds__(kBlock) void k_scan_local_fill(
     scan_local_block<OpDD4>(spos, n, pref, pref_sums, shd);
 }
 
-// One workgroup after k_scan_local_fill: both scans' chunk totals (totals to off[n] and
-// pref[n]), the end-search levels 3 and up, and the words the host reads back, written
-// to its pinned memory: the two flag counts of k_delta and the node count.
+// Two workgroups after k_scan_local_fill, side by side: workgroup 1 scans the moments'
+// chunk totals (total to pref[n]); workgroup 0 scans the node counts' (total to off[n]),
+// forms the end-search levels 3 and up, and writes the words the host reads back to its
+// pinned memory: the two flag counts of k_delta and the node count.
 __global__ __launch_bounds__(kBlock) void k_scan_sums_fill(
     long long* __restrict__ off_sums, DD4* __restrict__ pref_sums, int64_t nb,
     long long* __restrict__ off_total, DD4* __restrict__ pref_total,
@@ -648,8 +649,11 @@ __global__ __launch_bounds__(kBlock) void k_scan_sums_fill(
 {
     __shared__ long long shi[kBlock];
     __shared__ DD4 shd[kBlock];
+    if (blockIdx.x == 1) {
+        scan_sums_block<OpDD4>(pref_sums, nb, pref_total, shd);
+        return;
+    }
     const long long nodes = scan_sums_block<OpI64>(off_sums, nb, off_total, shi);
-    scan_sums_block<OpDD4>(pref_sums, nb, pref_total, shd);
     search_levels_block(dp, n, levels);
     if (threadIdx.x == 0) {
         int* hf = (int*)host;
@@ -1523,7 +1527,7 @@ static int build_tree(gh_ctx* c, const double* d_pos, const double* d_mass, int6
             GH_TRY(check_launch(c, "k_scan_local_fill"));
         }
         tb->readback[0] = tb->readback[1] = -1;
-        hipLaunchKernelGGL(k_scan_sums_fill, dim3(1), dim3(kBlock), 0, s,
+        hipLaunchKernelGGL(k_scan_sums_fill, dim3(2), dim3(kBlock), 0, s,
                            tb->block_counts.as<long long>(), tb->block_sums.as<DD4>(), nb,
                            off + n, pref + n, dp, n, search_lv, tb->flags.as<int>(),
                            tb->readback_dev);
