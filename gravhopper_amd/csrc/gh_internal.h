@@ -229,10 +229,23 @@ int gather_frames(gh_ctx* c, OrderBufs& o, const double* d_pos, const double* d_
                   bool want_s64, hipStream_t s);
 
 // tree helpers shared with the spatial order (gh_tree.hip)
+constexpr int kRsItems = 8;
+constexpr int kRsChunk = kBlock * kRsItems;  // elements per radix-sort workgroup
+// Sorts of at most this many chunks may take radix_sort_direct.
+constexpr int kRsDirectChunks = 32;
 int radix_sort(gh_ctx* c, DevBuf* keys, DevBuf* idx, DevBuf& hist, int64_t n, int words,
                int lo_shift, hipStream_t s, int* sel_out, int hi_shift = 64);
+// One-word stable LSD sort of (keys[0], idx[0]) by bits [lo_shift, hi_shift), 8 per pass,
+// one kernel per pass, for at most kRsDirectChunks chunks.  hist holds, per pass p, the
+// digit counts of each chunk of that pass's input at hist + p * 256 * nb (digit d, chunk
+// b at d * nb + b, nb = the chunk count): pass 0's counted by the caller, the later ones
+// zero on entry (each pass counts the next pass's).
+int radix_sort_direct(gh_ctx* c, DevBuf* keys, DevBuf* idx, DevBuf& hist, int64_t n,
+                      int lo_shift, int hi_shift, hipStream_t s, int* sel_out);
 int bbox_dev(gh_ctx* c, const double* d_pos, int64_t n, double eps, DevBuf& part, double* box,
              hipStream_t s);
+int bbox_partial_dev(gh_ctx* c, const double* d_pos, int64_t n, DevBuf& part, u32* zero,
+                     int64_t nzero, hipStream_t s, int* nparts);
 
 // tree (gh_tree.hip)
 int tree_dev(gh_ctx* c, const double* d_pos, const double* d_mass, int64_t np,

@@ -41,6 +41,7 @@
 #include <algorithm>
 #include <cmath>
 
+#include "gh_box.h"
 #include "gh_internal.h"
 #include "gh_search.h"
 
@@ -51,8 +52,6 @@ typedef unsigned int u32;
 
 constexpr int kLevelsPerWord = 21;
 constexpr int kMaxWords = 8;  // 168 levels
-constexpr int kRsItems = 8;
-constexpr int kRsChunk = kBlock * kRsItems;  // elements per radix-sort workgroup
 
 // FP64 node record of the walk and the dump
 // FP64 record; a leaf carries its size with the sign bit set, so that the FP64 walk
@@ -97,10 +96,16 @@ struct TreeBufs {
 // ------------------------------------------------------------------------------
 // 1. bounding box
 // ------------------------------------------------------------------------------
+// Extent of each workgroup's share of the points: part[6 b ..] = (min, max).  The
+// workgroups also clear zero[0 .. nzero) (the small sort's digit counts; may be null).
 __global__ __launch_bounds__(kBlock) void k_bbox_partial(const double* __restrict__ pos,
-                                                         int64_t n, double* __restrict__ part)
+                                                         int64_t n, double* __restrict__ part,
+                                                         u32* __restrict__ zero, int64_t nzero)
 {
     __shared__ double smin[3][kBlock], smax[3][kBlock];
+    for (int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x; j < nzero;
+         j += (int64_t)gridDim.x * kBlock)
+        zero[j] = 0u;
     double mn[3], mx[3];
     for (int k = 0; k < 3; k++) mn[k] = mx[k] = pos[k];  // particle 0 seeds, as :747-750
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
@@ -115,16 +120,7 @@ __global__ __launch_bounds__(kBlock) void k_bbox_partial(const double* __restric
         smin[k][threadIdx.x] = mn[k];
         smax[k][threadIdx.x] = mx[k];
     }
-    __syncthreads();
-    for (int w = kBlock / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w)
-            for (int k = 0; k < 3; k++) {
-                const double a = smin[k][threadIdx.x + w], b = smax[k][threadIdx.x + w];
-                if (a < smin[k][threadIdx.x]) smin[k][threadIdx.x] = a;
-                if (b > smax[k][threadIdx.x]) smax[k][threadIdx.x] = b;
-            }
-        __syncthreads();
-    }
+    block_extent(smin, smax);
     if (threadIdx.x == 0)
         for (int k = 0; k < 3; k++) {
             part[6 * blockIdx.x + k] = smin[k][0];
@@ -138,41 +134,9 @@ __global__ __launch_bounds__(kBlock) void k_bbox_final(const double* __restrict_
                                                        double* __restrict__ box)
 {
     __shared__ double smin[3][kBlock], smax[3][kBlock];
-    double mn[3], mx[3];
-    for (int k = 0; k < 3; k++) {
-        mn[k] = part[k];
-        mx[k] = part[3 + k];
-    }
-    for (int b = threadIdx.x; b < nparts; b += kBlock)
-        for (int k = 0; k < 3; k++) {
-            if (part[6 * b + k] < mn[k]) mn[k] = part[6 * b + k];
-            if (part[6 * b + 3 + k] > mx[k]) mx[k] = part[6 * b + 3 + k];
-        }
-    for (int k = 0; k < 3; k++) {
-        smin[k][threadIdx.x] = mn[k];
-        smax[k][threadIdx.x] = mx[k];
-    }
-    __syncthreads();
-    for (int w = kBlock / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w)
-            for (int k = 0; k < 3; k++) {
-                if (smin[k][threadIdx.x + w] < smin[k][threadIdx.x])
-                    smin[k][threadIdx.x] = smin[k][threadIdx.x + w];
-                if (smax[k][threadIdx.x + w] > smax[k][threadIdx.x])
-                    smax[k][threadIdx.x] = smax[k][threadIdx.x + w];
-            }
-        __syncthreads();
-    }
-    if (threadIdx.x != 0) return;
-    for (int k = 0; k < 3; k++) {
-        mn[k] = smin[k][0];
-        mx[k] = smax[k][0];
-    }
-    double bs = mx[0] - mn[0] + eps;
-    for (int k = 1; k < 3; k++)
-        if ((mx[k] - mn[k]) > bs) bs = mx[k] - mn[k] + eps;
-    for (int k = 0; k < 3; k++) box[k] = 0.5 * (mn[k] + mx[k]);
-    box[3] = bs;
+    double b[4];
+    box_from_parts(part, nparts, eps, smin, smax, b);
+    if (threadIdx.x < 4) box[threadIdx.x] = b[threadIdx.x];
 }
 
 // ------------------------------------------------------------------------------
@@ -279,18 +243,25 @@ __global__ __launch_bounds__(kBlock) void k_rs_scan(u32* __restrict__ hist, int 
 // wavefronts.  STAGE (one key word only): the block's elements are first placed in LDS
 // in output order and then written out by consecutive threads, so each digit's run of
 // the block leaves as contiguous stores; otherwise every element is stored directly.
-template <bool STAGE>
+// DIRECT (STAGE only, at most kRsDirectChunks blocks): hist holds the unscanned digit
+// counts of every block, and each block forms its bases from them itself (totals is
+// unused); when hnext is not null the block also counts, per output chunk, the digits
+// of the next pass (bits shift + 8 ..) into hnext, which must be zero beforehand.
+template <bool STAGE, bool DIRECT>
 __global__ __launch_bounds__(kBlock) void k_rs_scatter(
     const u64* __restrict__ kin, u64* __restrict__ kout, const u32* __restrict__ iin,
     u32* __restrict__ iout, int64_t n, int words, int wsel, int shift,
-    const u32* __restrict__ hist, int nblocks, const u32* __restrict__ totals)
+    const u32* __restrict__ hist, int nblocks, const u32* __restrict__ totals,
+    u32* __restrict__ hnext)
 {
+    static_assert(!DIRECT || STAGE, "the direct form stages its output");
     constexpr int kWaves = kBlock / 64;
     __shared__ u32 gbase0[256];          // this block's first output position per digit
     __shared__ u32 lstart[256];          // this block's first LDS slot per digit (STAGE)
     __shared__ u32 wcount[kWaves][256];  // per-wave running counts, then wave offsets
     __shared__ u64 stage_k[STAGE ? kRsChunk : 1];
     __shared__ u32 stage_i[STAGE ? kRsChunk : 1];
+    __shared__ u32 next_cnt[DIRECT ? kRsDirectChunks * 256 : 1];  // [output chunk][digit]
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int dd = threadIdx.x;  // one digit per thread in the per-digit steps
     const int64_t base = (int64_t)blockIdx.x * kRsChunk;
@@ -305,6 +276,30 @@ __global__ __launch_bounds__(kBlock) void k_rs_scatter(
         kw[it] = i < n ? kin[(int64_t)wsel * n + i] : 0ull;
         id[it] = i < n ? iin[i] : 0u;
     }
+    if (DIRECT) {
+        // this digit's count in every block: total, earlier blocks, this block
+        u32 tot = 0u, before = 0u, mine = 0u;
+        for (int b = 0; b < nblocks; b++) {
+            const u32 h = hist[dd * nblocks + b];
+            tot += h;
+            if (b < (int)blockIdx.x) before += h;
+            if (b == (int)blockIdx.x) mine = h;
+        }
+        gbase0[dd] = tot;
+        lstart[dd] = mine;
+        for (int j = threadIdx.x; j < kRsDirectChunks * 256; j += kBlock) next_cnt[j] = 0u;
+        __syncthreads();
+        for (int off = 1; off < 256; off <<= 1) {
+            const u32 t = dd >= off ? gbase0[dd - off] : 0u;
+            const u32 m = dd >= off ? lstart[dd - off] : 0u;
+            __syncthreads();
+            gbase0[dd] += t;
+            lstart[dd] += m;
+            __syncthreads();
+        }
+        gbase0[dd] += before - tot;  // exclusive base plus earlier blocks
+        lstart[dd] -= mine;
+    } else {
     // digit bases: exclusive scan of the 256 digit totals, plus earlier blocks
     {
         const u32 tot = totals[dd];
@@ -336,6 +331,7 @@ __global__ __launch_bounds__(kBlock) void k_rs_scatter(
             __syncthreads();
         }
         lstart[dd] -= cnt;
+    }
     }
     for (int w = 0; w < kWaves; w++) wcount[w][dd] = 0u;
     __syncthreads();
@@ -396,6 +392,16 @@ __global__ __launch_bounds__(kBlock) void k_rs_scatter(
             const u32 dst = gbase0[d] + (u32)(e - lstart[d]);
             kout[dst] = k;
             iout[dst] = stage_i[e];
+            if (DIRECT && hnext)
+                atomicAdd(&next_cnt[(dst / kRsChunk) * 256 + (u32)((k >> (shift + 8)) & 255u)],
+                          1u);
+        }
+        if (DIRECT && hnext) {
+            __syncthreads();
+            for (int j = threadIdx.x; j < nblocks * 256; j += kBlock) {
+                const u32 v = next_cnt[j];
+                if (v) atomicAdd(&hnext[(j & 255) * nblocks + (j >> 8)], v);
+            }
         }
     }
 }
@@ -1415,15 +1421,15 @@ int radix_sort(gh_ctx* c, DevBuf* keys, DevBuf* idx, DevBuf& hist, int64_t n, in
                                hist.as<u32>() + (int64_t)256 * nblocks);
             GH_TRY(check_launch(c, "k_rs_scan"));
             if (words == 1)
-                hipLaunchKernelGGL(k_rs_scatter<true>, dim3(nblocks), dim3(kBlock), 0, s, kin,
-                                   keys[sel ^ 1].as<u64>(), idx[sel].as<u32>(),
+                hipLaunchKernelGGL((k_rs_scatter<true, false>), dim3(nblocks), dim3(kBlock), 0,
+                                   s, kin, keys[sel ^ 1].as<u64>(), idx[sel].as<u32>(),
                                    idx[sel ^ 1].as<u32>(), n, words, w, shift, hist.as<u32>(),
-                                   nblocks, hist.as<u32>() + (int64_t)256 * nblocks);
+                                   nblocks, hist.as<u32>() + (int64_t)256 * nblocks, nullptr);
             else
-                hipLaunchKernelGGL(k_rs_scatter<false>, dim3(nblocks), dim3(kBlock), 0, s, kin,
-                                   keys[sel ^ 1].as<u64>(), idx[sel].as<u32>(),
+                hipLaunchKernelGGL((k_rs_scatter<false, false>), dim3(nblocks), dim3(kBlock), 0,
+                                   s, kin, keys[sel ^ 1].as<u64>(), idx[sel].as<u32>(),
                                    idx[sel ^ 1].as<u32>(), n, words, w, shift, hist.as<u32>(),
-                                   nblocks, hist.as<u32>() + (int64_t)256 * nblocks);
+                                   nblocks, hist.as<u32>() + (int64_t)256 * nblocks, nullptr);
             GH_TRY(check_launch(c, "k_rs_scatter"));
             sel ^= 1;
         }
@@ -1432,16 +1438,46 @@ int radix_sort(gh_ctx* c, DevBuf* keys, DevBuf* idx, DevBuf& hist, int64_t n, in
     return 0;
 }
 
+int radix_sort_direct(gh_ctx* c, DevBuf* keys, DevBuf* idx, DevBuf& hist, int64_t n,
+                      int lo_shift, int hi_shift, hipStream_t s, int* sel_out)
+{
+    const int nblocks = (int)((n + kRsChunk - 1) / kRsChunk);
+    if (nblocks > kRsDirectChunks) return fail(c, GH_EARG, "internal: direct sort of %d chunks", nblocks);
+    int sel = 0;
+    u32* h = hist.as<u32>();
+    for (int shift = lo_shift; shift < hi_shift; shift += 8) {
+        u32* hn = shift + 8 < hi_shift ? h + (int64_t)256 * nblocks : nullptr;
+        hipLaunchKernelGGL((k_rs_scatter<true, true>), dim3(nblocks), dim3(kBlock), 0, s,
+                           keys[sel].as<u64>(), keys[sel ^ 1].as<u64>(), idx[sel].as<u32>(),
+                           idx[sel ^ 1].as<u32>(), n, 1, 0, shift, h, nblocks, nullptr, hn);
+        GH_TRY(check_launch(c, "k_rs_scatter"));
+        h += (int64_t)256 * nblocks;
+        sel ^= 1;
+    }
+    *sel_out = sel;
+    return 0;
+}
+
+// Extents of n points per workgroup into part (*nparts of them), clearing zero[0 ..
+// nzero) on the way.
+int bbox_partial_dev(gh_ctx* c, const double* d_pos, int64_t n, DevBuf& part, u32* zero,
+                     int64_t nzero, hipStream_t s, int* nparts)
+{
+    const int np = (int)std::min<int64_t>(1024, (n + kBlock - 1) / kBlock);
+    GH_TRY(ensure(c, part, sizeof(double) * 6 * (size_t)np));
+    hipLaunchKernelGGL(k_bbox_partial, dim3(np), dim3(kBlock), 0, s, d_pos, n,
+                       part.as<double>(), zero, nzero);
+    *nparts = np;
+    return check_launch(c, "k_bbox_partial");
+}
+
 // Root box of the reference (centre box[0..2], size box[3], gravhopper/_jbgrav.c:747-772)
 // of n points, written to the device array box; part holds the partial extents.
 int bbox_dev(gh_ctx* c, const double* d_pos, int64_t n, double eps, DevBuf& part, double* box,
              hipStream_t s)
 {
-    const int nparts = (int)std::min<int64_t>(1024, (n + kBlock - 1) / kBlock);
-    GH_TRY(ensure(c, part, sizeof(double) * 6 * (size_t)nparts));
-    hipLaunchKernelGGL(k_bbox_partial, dim3(nparts), dim3(kBlock), 0, s, d_pos, n,
-                       part.as<double>());
-    GH_TRY(check_launch(c, "k_bbox_partial"));
+    int nparts = 0;
+    GH_TRY(bbox_partial_dev(c, d_pos, n, part, nullptr, 0, s, &nparts));
     hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(kBlock), 0, s, part.as<double>(), nparts,
                        eps, box);
     return check_launch(c, "k_bbox_final");
