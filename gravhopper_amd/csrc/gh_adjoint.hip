@@ -306,16 +306,29 @@ __global__ __launch_bounds__(kBlock) void k_adj_post(const float4* __restrict__ 
                                                      double* __restrict__ lv,
                                                      double* __restrict__ lm)
 {
-    const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (p >= n) return;
-    double g0 = 0.0, g1 = 0.0, g2 = 0.0, gm = 0.0;
-    for (int b = 0; b < nb; b++) {
-        const float4 q = part[(int64_t)b * n_pad + p];
-        g0 += (double)q.x;
-        g1 += (double)q.y;
-        g2 += (double)q.z;
-        gm += (double)q.w;
-    }
+    // 64 particles per workgroup, one per lane; wavefront w adds the slots w, w +
+    // kSlotLanes, ... in order, and wavefront 0 adds the wavefronts' sums in a fixed order
+    // and applies the update
+    static_assert(kBlock == 64 * kSlotLanes, "one slice per wavefront");
+    __shared__ double sh[kSlotLanes][4][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t p = (int64_t)blockIdx.x * 64 + lane;
+    double g4[4] = {0.0, 0.0, 0.0, 0.0};
+    if (p < n)
+        for (int b = w; b < nb; b += kSlotLanes) {
+            const float4 q = part[(int64_t)b * n_pad + p];
+            g4[0] += (double)q.x;
+            g4[1] += (double)q.y;
+            g4[2] += (double)q.z;
+            g4[3] += (double)q.w;
+        }
+    for (int j = 0; j < 4; j++) sh[w][j][lane] = g4[j];
+    __syncthreads();
+    if (w != 0 || p >= n) return;
+    double gs[4];
+    for (int j = 0; j < 4; j++)
+        gs[j] = (sh[0][j][lane] + sh[1][j][lane]) + (sh[2][j][lane] + sh[3][j][lane]);
+    const double g0 = gs[0], g1 = gs[1], g2 = gs[2], gm = gs[3];
     const int64_t i = perm[p];
     const double g[3] = {g0, g1, g2};
     for (int k = 0; k < 3; k++) {
@@ -456,7 +469,7 @@ static int reverse_step(gh_ctx* c, const AdjWork& w, const double* xh, const dou
         }
         GH_TRY(check_launch(c, "k_vjp_sym_f32"));
     }
-    hipLaunchKernelGGL(k_adj_post, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, part, w.nb,
+    hipLaunchKernelGGL(k_adj_post, dim3(blocks_for(n, 64)), dim3(kBlock), 0, s, part, w.nb,
                        w.n_pad, n, o.perm(), mv, cdrift, lx, lv, lm);
     return check_launch(c, "k_adj_post");
 }

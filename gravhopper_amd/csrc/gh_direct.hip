@@ -700,26 +700,35 @@ __global__ __launch_bounds__(kBlock) void k_reduce_sym(const float* __restrict__
                                                        const u32* __restrict__ perm,
                                                        double* __restrict__ acc)
 {
-    const int64_t k0 = ((int64_t)blockIdx.x * kBlock + threadIdx.x) * 4;
-    if (k0 >= n3) return;
-    const int slots = nb_blk ? nb_blk[(k0 / 3) / blk] : nb;
-    const float4* p = (const float4*)(part + k0);
-    const int64_t stride = np_pad * 3 / 4;  // float4 per slab
+    // Each workgroup takes 64 groups of 4 floats, one per lane; its wavefront w adds the
+    // slots w, w + kSlotLanes, ... in order (so each load reads 64 consecutive float4),
+    // and the wavefronts' sums are added in a fixed order.  The order depends on the slot
+    // count only, so an ensemble's particles sum exactly as they would alone.
+    static_assert(kBlock == 64 * kSlotLanes, "one slice per wavefront");
+    __shared__ double sh[kSlotLanes][4][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t k0 = ((int64_t)blockIdx.x * 64 + lane) * 4;
     double s[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int b = 0; b < slots; b++) {
-        const float4 v = p[(int64_t)b * stride];
-        s[0] += (double)v.x;
-        s[1] += (double)v.y;
-        s[2] += (double)v.z;
-        s[3] += (double)v.w;
-    }
-#pragma unroll
-    for (int t = 0; t < 4; t++) {
-        const int64_t k = k0 + t;
-        if (k < n3) {
-            const int64_t q = k / 3;
-            acc[perm ? 3 * (int64_t)perm[q] + (k - 3 * q) : k] = s[t];
+    if (k0 < n3) {
+        const int slots = nb_blk ? nb_blk[(k0 / 3) / blk] : nb;
+        const float4* p = (const float4*)(part + k0);
+        const int64_t stride = np_pad * 3 / 4;  // float4 per slab
+        for (int b = w; b < slots; b += kSlotLanes) {
+            const float4 v = p[(int64_t)b * stride];
+            s[0] += (double)v.x;
+            s[1] += (double)v.y;
+            s[2] += (double)v.z;
+            s[3] += (double)v.w;
         }
+    }
+    for (int j = 0; j < 4; j++) sh[w][j][lane] = s[j];
+    __syncthreads();
+    // wavefront w stores component w of its lanes' groups: (s0 + s1) + (s2 + s3)
+    const int64_t k = k0 + w;
+    if (k0 < n3 && k < n3) {
+        const double t = (sh[0][w][lane] + sh[1][w][lane]) + (sh[2][w][lane] + sh[3][w][lane]);
+        const int64_t q = k / 3;
+        acc[perm ? 3 * (int64_t)perm[q] + (k - 3 * q) : k] = t;
     }
 }
 
@@ -913,8 +922,8 @@ int reduce_sym(gh_ctx* c, const float* part, int nb, const int* nb_blk, int blk,
     if (n <= 0) return 0;
     if ((np_pad * 3) % 4 != 0 || (3 * blk) % 4 != 0)
         return fail(c, GH_EARG, "partial slabs are not 16-byte aligned");
-    hipLaunchKernelGGL(k_reduce_sym, dim3(blocks_for((3 * n + 3) / 4, kBlock)), dim3(kBlock), 0,
-                       s, part, nb, nb_blk, blk, np_pad, 3 * n, perm, acc);
+    hipLaunchKernelGGL(k_reduce_sym, dim3(blocks_for((3 * n + 3) / 4, 64)), dim3(kBlock), 0, s,
+                       part, nb, nb_blk, blk, np_pad, 3 * n, perm, acc);
     return check_launch(c, "k_reduce_sym");
 }
 

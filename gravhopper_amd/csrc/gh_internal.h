@@ -20,6 +20,9 @@ constexpr double kAccelScale = 4.398600412921223e-09;
 constexpr double kDriftScale = 1.022712165045695e-03;
 
 constexpr int kBlock = 256;  // threads per workgroup for every 1-D kernel
+// Wavefronts that share the FP64 sums of 64 particles' partial slots (k_reduce_sym,
+// k_adj_post): each adds every kSlotLanes-th slot, then their sums are combined.
+constexpr int kSlotLanes = 4;
 
 // A device buffer that only grows; reused across calls so the hot loop never allocates.
 struct DevBuf {
