@@ -916,14 +916,14 @@ __device__ __forceinline__ WalkRec load_rec(const NodeW* __restrict__ nodes, int
 {
     const char* p = (const char*)(nodes + n);
     const double2 a = *(const double2*)p;        // cx, cy
-    const int4 b = *(const int4*)(p + 16);       // cz, size, mass
+    const double2 b = *(const double2*)(p + 16); // cz, (size, mass)
     const float4 c = *(const float4*)(p + 32);   // offset of the centre of mass
     WalkRec r;
     r.cx = a.x;
     r.cy = a.y;
-    r.cz = __hiloint2double(b.y, b.x);
-    r.size = __int_as_float(b.z);
-    r.mass = __int_as_float(b.w);
+    r.cz = b.x;
+    r.size = __int_as_float(__double2loint(b.y));
+    r.mass = __int_as_float(__double2hiint(b.y));
     r.ox = c.x;
     r.oy = c.y;
     r.oz = c.z;
