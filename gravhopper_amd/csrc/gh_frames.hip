@@ -123,7 +123,7 @@ int spatial_order(gh_ctx* c, OrderBufs& o, const double* d_pos, int64_t n, hipSt
     GH_TRY(ensure(c, o.box, sizeof(double) * 8));
     GH_TRY(ensure(c, o.nflag, 256));
     const int nb = (int)((n + kRsChunk - 1) / kRsChunk);
-    if (nb <= kRsDirectChunks) {
+    if (nb <= kRsDirectChunks && c->tune[GH_TUNE_SORT_DIRECT] >= 0) {
         // few chunks: six kernels instead of fifteen (the same order, bit for bit)
         const int passes = (63 - kKeyShift + 7) / 8;
         const int64_t nh = (int64_t)256 * nb * passes;

@@ -240,11 +240,14 @@ int gh_timing_read(gh_ctx *ctx, int kernel_class, double *total_ms, int64_t *lau
  *                        partial per particle and block fits 13 GiB, else bands of
  *                        block columns, one pass each, whose partials fit it), n > 0
  *                        bands of n block columns
+ *   GH_TUNE_SORT_DIRECT  spatial order of the FP32 direct sum: 0 automatic (up to 32
+ *                        chunks of 2048 points, one kernel per radix pass), -1 always
+ *                        the histogram, scan and scatter kernels per pass
  * The FP32 direct-sum keys and GH_TUNE_WALK_F64EVAL change results only by rounding;
  * GH_TUNE_WALK_GROUP changes which nodes the FP32 walk accepts (never fewer than the
  * reference's opening test accepts for any target); GH_TUNE_DIRECT_F64_TPW and
- * GH_TUNE_RUN_GRAPH do not change results; GH_TUNE_ALLOC_LIMIT_MB only decides whether a
- * call may allocate. */
+ * GH_TUNE_RUN_GRAPH do not change results, nor does GH_TUNE_SORT_DIRECT (both forms give
+ * the same order); GH_TUNE_ALLOC_LIMIT_MB only decides whether a call may allocate. */
 #define GH_TUNE_DIRECT_SELF 0
 #define GH_TUNE_DIRECT_IPT 1
 #define GH_TUNE_WALK_GROUP 2
@@ -253,7 +256,8 @@ int gh_timing_read(gh_ctx *ctx, int kernel_class, double *total_ms, int64_t *lau
 #define GH_TUNE_ALLOC_LIMIT_MB 5
 #define GH_TUNE_DIRECT_F64_TPW 6
 #define GH_TUNE_SYM_BAND 7
-#define GH_TUNE_COUNT 8
+#define GH_TUNE_SORT_DIRECT 8
+#define GH_TUNE_COUNT 9
 int gh_set_tuning(gh_ctx *ctx, int key, int value);
 
 /* FP64 diagnostics (no reference counterpart): kinetic energy sum 0.5 m v^2

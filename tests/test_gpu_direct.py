@@ -188,3 +188,23 @@ def test_fp64_direct_automatic_shapes_bitwise(ctx, oracle, n):
     fpos = rng.normal(size=(n, 3))
     assert_bitwise(ctx.direct(pos, mass, 1e-3, FP64, force_pos=fpos),
                    oracle.direct_position(pos, mass, fpos, 1e-3), "position n=%d" % n)
+
+
+@pytest.mark.parametrize("n", [2, 3, 255, 2047, 2049, 16384, 65536, 65537])
+def test_fp32_sort_forms_agree(ctx, n):
+    # the spatial order of up to 32 radix chunks (65536 points) takes one kernel per
+    # pass; the classic histogram / scan / scatter passes must give the same order, so
+    # the same FP32 forces bit for bit; both within the FP32 tolerance of the FP64 sum
+    ic = workloads.plummer(n, b=1e-3, mass=1e6, seed=77 + n)
+    pos = ic["pos"] + np.array([3.0, -8.0, 40.0])  # away from the origin
+    got = ctx.direct(pos, ic["mass"], 1e-5, FP32)
+    ctx.set_tuning(_native.TUNE_SORT_DIRECT, -1)
+    try:
+        classic = ctx.direct(pos, ic["mass"], 1e-5, FP32)
+    finally:
+        ctx.set_tuning(_native.TUNE_SORT_DIRECT, 0)
+    assert_bitwise(got, classic, "n=%d" % n)
+    if n >= 16:
+        ref = ctx.direct(pos, ic["mass"], 1e-5, FP64)
+        e = rel_err(got, ref)
+        assert np.median(e) <= FP32_MEDIAN and np.percentile(e, 99) <= FP32_P99
