@@ -438,9 +438,9 @@ __global__ __launch_bounds__(kBlock) void k_direct_f32(const double4* __restrict
 //
 // Results are partial fields part[K][p] (FP32, 3 per particle): particle p of block B
 // receives one partial from every block K (K = B for its diagonal); k_reduce_sym adds
-// them in block order in FP64.  The j-side partials can go to a second array
-// (SymOut): a pass over the block pairs of a band of columns J keeps only the band's
-// slots (see direct_sym).
+// them in FP64, in an order fixed by their count.  The j-side partials can go to a
+// second array (SymOut): a pass over the block pairs of a band of columns J keeps only
+// the band's slots (see direct_sym).
 constexpr int kRotFromNext = 0x134;  // DPP wave_rol:1 -- lane l reads lane l+1 (mod 64)
 
 __device__ __forceinline__ float rot_from_next(float v)
@@ -687,13 +687,12 @@ __global__ __launch_bounds__(kBlock) void k_direct_sym_ds(const double4* __restr
     }
 }
 
-// acc[perm[p]] = sum over the partial slots of p's simulation, in slot order, in FP64
-// (perm == nullptr: acc[p]).  nb_blk[b] (nullable): number of slots of the simulation
-// that block b belongs to; otherwise every particle has nb slots.  Four consecutive
-// components per thread, read as one 16-byte load per partial slab (each slab, 3 np_pad
-// floats with np_pad a multiple of the block size, starts on a 16-byte boundary, and a
-// group of four never straddles two blocks, as 3 x 256 is a multiple of 4); every
-// component is still summed in block order.
+// acc[perm[p]] = sum over the partial slots of p's simulation in FP64 (perm == nullptr:
+// acc[p]).  nb_blk[b] (nullable): number of slots of the simulation that block b belongs
+// to; otherwise every particle has nb slots.  Four consecutive components per lane, read
+// as one 16-byte load per partial slab (each slab, 3 np_pad floats with np_pad a multiple
+// of the block size, starts on a 16-byte boundary, and a group of four never straddles
+// two blocks, as 3 x 256 is a multiple of 4).
 __global__ __launch_bounds__(kBlock) void k_reduce_sym(const float* __restrict__ part, int nb,
                                                        const int* __restrict__ nb_blk,
                                                        int blk, int64_t np_pad, int64_t n3,
