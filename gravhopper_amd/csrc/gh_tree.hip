@@ -762,52 +762,104 @@ __device__ __forceinline__ void emit_node(const FillArgs& a, int64_t id, int lv,
     }
 }
 
+// Exclusive prefix sum of v over the wavefront's lanes, and its total (wave-uniform).
+__device__ __forceinline__ int wave_scan_excl(int v, int& total)
+{
+    const int lane = threadIdx.x & 63;
+    int incl = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(incl, o);
+        if (lane >= o) incl += t;
+    }
+    total = __shfl(incl, 63);
+    return incl - v;
+}
+
+// An internal node waiting in the fill's queue: its centre and size, sorted start
+// position, node id, level and branch.
+struct FillEntry {
+    double cx, cy, cz, size;
+    u32 pos, id;
+    int lv, br;
+};
+constexpr int kFillQueue = 128;  // entries per wavefront
+
 // One thread per sorted particle i: the internal nodes starting at i (levels lprev+1 ..
 // lcur) and the leaf of i (level lcur+1, or lprev+1 when no internal node starts
-// here).  The centre recurrence runs in three phases: down to the first node of i, the
-// internal nodes one level apart, then the leaf, so that the lanes of a wavefront write
-// their k-th node in the same pass rather than one pass per distinct level.
-// one wavefront per workgroup: a thread's work varies with the nodes it writes, and a
-// SIMD slot is refilled as soon as its wavefront ends (6 us less per build at N=2^20
-// than 256-thread workgroups)
+// here).  Each lane runs the FP64 centre recurrence down its own path and queues its
+// internal nodes in LDS; the wavefront then finishes the queued nodes together (end
+// search, moment prefixes, records), a lane per node, so that a lane with many internal
+// nodes does not hold up the others (a particle starts 0.55 internal nodes on average
+// at N=2^20, some start tens); a queue that would overflow is taken in rounds.  Last,
+// every lane writes its leaf.  One wavefront per workgroup.
 constexpr int kFillBlock = 64;
 __global__ __launch_bounds__(kFillBlock) void k_fill_nodes(FillArgs a, int64_t n)
 {
-    const int64_t i = (int64_t)blockIdx.x * kFillBlock + threadIdx.x;
-    if (i >= n) return;
-    const u32 p = a.perm[i];
-    const double4 q = a.spos[i];
-    const double x = q.x, y = q.y, z = q.z;
-    const int lprev = (i == 0) ? -1 : (int)a.dp[i - 1] - 1;
-    const int lcur = (int)a.dp[i] - 1;
-    const int n_int = lcur > lprev ? lcur - lprev : 0;
-    const int first = lprev + 1;  // level of the first node starting at i
-    const int64_t base = a.off.at(i);
+    __shared__ FillEntry queue[kFillQueue];
+    const int lane = threadIdx.x;
+    const int64_t i = (int64_t)blockIdx.x * kFillBlock + lane;
+    const bool live = i < n;  // every lane takes part in the wave-wide steps
+    u32 p = 0;
+    double x = 0.0, y = 0.0, z = 0.0, m_leaf = 0.0;
+    int n_int = 0, first = 0;
+    int64_t base = 0;
     double cx = a.box[0], cy = a.box[1], cz = a.box[2], size = a.box[3];
     int br = -1;
-    for (int lv = 0; lv < first; lv++) {
-        const double h = 0.5 * size;
-        br = descend(x, y, z, cx, cy, cz, h);
-        size = h;
-    }
-    if (n_int > 0) {
-        // what every internal node starting at i shares: the first end-search groups and
-        // the moment prefix at i
-        const EndStart st = end_start(a.es, i);
-        const DD4 lo = a.pref.at(i);
-        for (int k = 0; k < n_int; k++) {
-            const int lv = first + k;
-            const int64_t e = node_end_from(a.es, n, st, lv);
-            const DD4 hi = a.pref.at(e + 1);
-            const double m = dd_diff(hi, lo, 0);
-            emit_node(a, base + k, lv, br, cx, cy, cz, size, e, false, p,
-                      dd_diff(hi, lo, 1) / m, dd_diff(hi, lo, 2) / m, dd_diff(hi, lo, 3) / m, m);
+    if (live) {
+        p = a.perm[i];
+        const double4 q = a.spos[i];
+        x = q.x;
+        y = q.y;
+        z = q.z;
+        m_leaf = q.w;
+        const int lprev = (i == 0) ? -1 : (int)a.dp[i - 1] - 1;
+        const int lcur = (int)a.dp[i] - 1;
+        n_int = lcur > lprev ? lcur - lprev : 0;
+        first = lprev + 1;  // level of the first node starting at i
+        base = a.off.at(i);
+        for (int lv = 0; lv < first; lv++) {
             const double h = 0.5 * size;
             br = descend(x, y, z, cx, cy, cz, h);
             size = h;
         }
     }
-    emit_node(a, base + n_int, first + n_int, br, cx, cy, cz, size, i, true, p, x, y, z, q.w);
+    int k = 0;  // this lane's internal nodes queued so far
+    for (;;) {
+        int total;
+        const int pre = wave_scan_excl(n_int - k, total);
+        if (total == 0) break;
+        const int take = min(n_int - k, max(0, kFillQueue - pre));
+        for (int t = 0; t < take; t++, k++) {
+            FillEntry e;
+            e.cx = cx;
+            e.cy = cy;
+            e.cz = cz;
+            e.size = size;
+            e.pos = (u32)i;
+            e.id = (u32)(base + k);
+            e.lv = first + k;
+            e.br = br;
+            queue[pre + t] = e;
+            const double h = 0.5 * size;
+            br = descend(x, y, z, cx, cy, cz, h);
+            size = h;
+        }
+        __syncthreads();
+        const int cnt = total < kFillQueue ? total : kFillQueue;
+        for (int j = lane; j < cnt; j += kFillBlock) {
+            const FillEntry e = queue[j];
+            const int64_t s0 = e.pos;
+            const int64_t end = node_end_from(a.es, n, end_start(a.es, s0), e.lv);
+            const DD4 lo = a.pref.at(s0), hi = a.pref.at(end + 1);
+            const double m = dd_diff(hi, lo, 0);
+            emit_node(a, e.id, e.lv, e.br, e.cx, e.cy, e.cz, e.size, end, false, 0u,
+                      dd_diff(hi, lo, 1) / m, dd_diff(hi, lo, 2) / m, dd_diff(hi, lo, 3) / m, m);
+        }
+        __syncthreads();  // the queue is read before the next round refills it
+    }
+    if (live)
+        emit_node(a, base + n_int, first + n_int, br, cx, cy, cz, size, i, true, p, x, y, z,
+                  m_leaf);
 }
 
 // ------------------------------------------------------------------------------
