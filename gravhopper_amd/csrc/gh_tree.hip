@@ -13,7 +13,9 @@
 //   2. k_keys         per particle, octant digits of W*21 levels of the recurrence,
 //                     21 levels (63 bits) per 64-bit word, level 0 most significant
 //   3. radix sort     stable LSD sort of (key words, index), 8-bit digits
-//                     (k_rs_hist / k_rs_scan / k_rs_scatter, wave-ballot ranking)
+//                     (k_rs_hist / k_rs_scan / k_rs_scatter, wave-ballot ranking);
+//                     one-word keys are sorted by their bits 24..63 (levels 0-12) and
+//                     the short runs left equal there by the whole key (k_sort_runs)
 //   4. k_delta        delta_i = number of leading levels shared by sorted keys i, i+1;
 //                     keys equal over all W words -> more words (or coincident error);
 //                     the same pass writes the node counts, the end-search levels and
@@ -86,11 +88,12 @@ struct TreeBufs {
     DevBuf spos;  // (x, y, z, m) of the particles in sorted order
     DevBuf nodes64, nodesw, kids, skip, level, branch, leaf;
     DevBuf tkeys[2], tidx[2];  // Morton order of position targets (tree_force_position)
-    long long* readback = nullptr;  // pinned host words: flags[2], node count
+    long long* readback = nullptr;  // pinned host words: flags[0..1], node count, flags[2]
     long long* readback_dev = nullptr;  // the same words' device address
     int64_t n_nodes = 0;
     int words = 0;
     int key_sel = 0;  // which of keys[0]/keys[1] holds the sorted keys
+    int full_sorts = 0;  // builds left that sort one-word keys over all bits
 };
 
 // ------------------------------------------------------------------------------
@@ -406,6 +409,45 @@ __global__ __launch_bounds__(kBlock) void k_rs_scatter(
     }
 }
 
+// One-word tree keys are radix-sorted by their bits kSortLowBits..63 only (the octant
+// digits of levels 0-12), which leaves in input order the runs of positions whose keys
+// agree there.  Such runs are few and short in practice (a galaxy of 2^20 particles has
+// about 10^4, none longer than 4), so the thread at the start of each run sorts it by
+// the whole key in place, stably (insertion sort), which gives the order of a sort over
+// all bits.  A run longer than kRunMax is left alone and counted in *long_runs; the
+// build then sorts again over all bits.  Permuting a run never changes bits 24..63 of
+// any of its entries, so no thread's view of where runs start and end depends on when
+// it reads its neighbours.
+constexpr int kSortLowBits = 24;
+constexpr int kRunMax = 64;
+__global__ __launch_bounds__(kBlock) void k_sort_runs(u64* __restrict__ keys,
+                                                      u32* __restrict__ idx, int64_t n,
+                                                      int* __restrict__ long_runs)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i + 1 >= n) return;
+    const u64 t = keys[i] >> kSortLowBits;
+    if ((keys[i + 1] >> kSortLowBits) != t) return;
+    if (i > 0 && (keys[i - 1] >> kSortLowBits) == t) return;
+    int64_t e = i + 2;  // one past the run's end, or i + kRunMax + 1 for a long run
+    while (e < n && e - i <= kRunMax && (keys[e] >> kSortLowBits) == t) e++;
+    if (e - i > kRunMax) {
+        atomicAdd(long_runs, 1);
+        return;
+    }
+    for (int64_t j = i + 1; j < e; j++) {
+        const u64 k = keys[j];
+        const u32 p = idx[j];
+        int64_t q = j;
+        for (; q > i && keys[q - 1] > k; q--) {
+            keys[q] = keys[q - 1];
+            idx[q] = idx[q - 1];
+        }
+        keys[q] = k;
+        idx[q] = p;
+    }
+}
+
 // ------------------------------------------------------------------------------
 // 4. shared levels of adjacent sorted keys
 // ------------------------------------------------------------------------------
@@ -435,6 +477,7 @@ struct DeltaArgs {
     long long* cnt;         // nodes starting at each sorted position
     double4* spos;          // sorted (x, y, z, m)
     int* flags;             // [0] pairs equal over all words, [1] coincident pairs
+                            // ([2] is k_sort_runs' count of long runs)
 };
 
 // Per sorted position i, after the sort:
@@ -646,7 +689,8 @@ __global__ __launch_bounds__(kBlock) void k_scan_local_fill(
 // Two workgroups after k_scan_local_fill, side by side: workgroup 1 scans the moments'
 // chunk totals (total to pref[n]); workgroup 0 scans the node counts' (total to off[n]),
 // forms the end-search levels 3 and up, and writes the words the host reads back to its
-// pinned memory: the two flag counts of k_delta and the node count.
+// pinned memory: the two flag counts of k_delta, the node count and the long-run count
+// of k_sort_runs.
 __global__ __launch_bounds__(kBlock) void k_scan_sums_fill(
     long long* __restrict__ off_sums, DD4* __restrict__ pref_sums, int64_t nb,
     long long* __restrict__ off_total, DD4* __restrict__ pref_total,
@@ -665,6 +709,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_sums_fill(
         int* hf = (int*)host;
         hf[0] = flags[0];
         hf[1] = flags[1];
+        host[2] = flags[2];
         host[1] = nodes;
     }
 }
@@ -1573,6 +1618,13 @@ static int build_tree(gh_ctx* c, const double* d_pos, const double* d_mass, int6
     ScanView<OpI64> off_view;
     ScanView<OpDD4> pref_view;
     long long total = 0;
+    // Whether one-word keys are sorted over all bits, or by bits kSortLowBits.. and then
+    // run by run (k_sort_runs).  After a build with a run too long for k_sort_runs the
+    // next kFullSortBuilds builds of the context sort over all bits.
+    constexpr int kFullSortBuilds = 64;
+    const int sort_mode = c->tune[GH_TUNE_TREE_SORT];
+    bool all_bits = sort_mode < 0 || (sort_mode == 0 && tb->full_sorts > 0);
+    if (sort_mode == 0 && tb->full_sorts > 0) tb->full_sorts--;
     // Keys with `words` words, sort, shared levels, numbering and moments, then one
     // readback of (flags, node count).  Keys still equal over all words (rare) mean
     // particles closer than words * 21 levels: the pass is redone with one more word.
@@ -1583,14 +1635,23 @@ static int build_tree(gh_ctx* c, const double* d_pos, const double* d_mass, int6
                         kMaxWords * kLevelsPerWord);
         GH_TRY(ensure(c, tb->keys[0], sizeof(u64) * (size_t)n * words));
         GH_TRY(ensure(c, tb->keys[1], sizeof(u64) * (size_t)n * words));
-        // 2. keys (the first thread also clears the flags read after k_delta)
+        // 2. keys (the first thread also clears the flags read after k_delta and
+        // k_sort_runs)
         hipLaunchKernelGGL(k_keys, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, d_pos, n,
                            tb->box.as<double>(), words, tb->keys[0].as<u64>(),
                            tb->idx[0].as<u32>(), tb->flags.as<int>(), d_mass,
                            tb->packed.as<double4>());
         GH_TRY(check_launch(c, "k_keys"));
         // 3. sort
-        GH_TRY(radix_sort(c, tb->keys, tb->idx, tb->hist, n, words, 0, s, &tb->key_sel));
+        const bool by_runs = words == 1 && !all_bits;
+        GH_TRY(radix_sort(c, tb->keys, tb->idx, tb->hist, n, words, by_runs ? kSortLowBits : 0,
+                          s, &tb->key_sel));
+        if (by_runs) {
+            hipLaunchKernelGGL(k_sort_runs, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s,
+                               tb->keys[tb->key_sel].as<u64>(), tb->idx[tb->key_sel].as<u32>(),
+                               n, tb->flags.as<int>() + 2);
+            GH_TRY(check_launch(c, "k_sort_runs"));
+        }
         // 4./5. shared levels, end-search levels, node counts, sorted particles
         DeltaArgs da;
         da.keys = tb->keys[tb->key_sel].as<u64>();
@@ -1629,6 +1690,14 @@ static int build_tree(gh_ctx* c, const double* d_pos, const double* d_mass, int6
         GH_HIP(c, hipStreamSynchronize(s));
         if (tb->readback[1] < 0)
             return fail(c, GH_EHIP, "internal: the tree build's readback was not written");
+        if (tb->readback[2] > 0) {
+            // a run too long for k_sort_runs, so the order (and what k_delta found in it)
+            // is not the keys' order: the same words again, sorted over all bits
+            all_bits = true;
+            if (sort_mode == 0) tb->full_sorts = kFullSortBuilds;
+            words--;
+            continue;
+        }
         const int* got = (const int*)tb->readback;
         if (got[1] > 0)
             return fail(c, GH_ECOINCIDENT,

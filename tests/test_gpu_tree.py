@@ -116,6 +116,53 @@ def test_topology_bitwise_search_levels(ctx, oracle, n):
                      "n=%d" % n)
 
 
+def clustered(n_field, sizes, seed):
+    """Field particles plus tight clusters (spread 1e-6 of a root box of about 10), each
+    inside one cell of octree level 12, so their keys agree over levels 0-12."""
+    rng = np.random.default_rng(seed)
+    parts = [rng.normal(size=(n_field, 3)) * 2.0]
+    for k in sizes:
+        parts.append(rng.normal(size=(1, 3)) + rng.normal(size=(k, 3)) * 1e-6)
+    pos = np.concatenate(parts)
+    return pos, rng.uniform(1, 2, len(pos))
+
+
+@pytest.mark.parametrize("mode", [1, -1, 0])
+def test_topology_bitwise_key_runs(ctx, oracle, mode):
+    # runs of 2..64 keys equal over levels 0-12 are ordered by k_sort_runs after the radix
+    # passes over those levels (mode 1); -1 sorts over all levels; each gives the same tree
+    pos, mass = clustered(20000, [2, 3, 5, 17, 40, 63, 64], seed=11)
+    ctx.set_tuning(_native.TUNE_TREE_SORT, mode)
+    try:
+        print("mode", mode, len(pos), flush=True)
+        compare_topology(ctx.tree_dump(pos, mass, 1e-3), oracle.tree_dump(pos, mass, 1e-3),
+                         "runs, mode %d" % mode)
+    finally:
+        ctx.set_tuning(_native.TUNE_TREE_SORT, 0)
+
+
+@pytest.mark.parametrize("mode", [1, 0])
+def test_topology_bitwise_long_key_run(ctx, oracle, mode):
+    # a run of 65 or 300 keys is longer than k_sort_runs takes: the build sorts again over
+    # all levels; in the automatic mode the next builds start that way (same results)
+    ctx.set_tuning(_native.TUNE_TREE_SORT, mode)
+    try:
+        for sizes in ([65], [300, 4], [3]):
+            pos, mass = clustered(5000, sizes, seed=12)
+            print("mode", mode, "clusters", sizes, flush=True)
+            compare_topology(ctx.tree_dump(pos, mass, 1e-3),
+                             oracle.tree_dump(pos, mass, 1e-3), "clusters %s" % sizes)
+            want, wcnt = oracle.tree(pos, mass, 1e-3, 0.7, count=True)
+            got, gcnt = ctx.tree_interactions(pos, mass, 1e-3, 0.7)
+            assert np.array_equal(gcnt, wcnt), "clusters %s: accepted-node counts" % sizes
+            # a cluster member's accepted nodes lie about 1e-6 away, so the reference's
+            # centres of mass (running sums, rounding about 1e-15 absolute) move its
+            # force by about 1e-8 relative; the device's come from double-double sums
+            np.testing.assert_allclose(got, want, rtol=1e-6, atol=1e-10 * np.abs(want).max())
+    finally:
+        ctx.set_tuning(_native.TUNE_TREE_SORT, 0)
+
+
 @pytest.mark.parametrize("case", tree_cases(), ids=lambda c: c[0])
 def test_fp64_walk_matches_reference(ctx, oracle, case):
     name, pos, mass, eps = case
