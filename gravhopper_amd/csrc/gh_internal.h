@@ -236,8 +236,10 @@ constexpr int kRsItems = 8;
 constexpr int kRsChunk = kBlock * kRsItems;  // elements per radix-sort workgroup
 // Sorts of at most this many chunks may take radix_sort_direct.
 constexpr int kRsDirectChunks = 32;
+// have_hist0: hist already holds the first pass's digit counts (k_rs_hist's layout).
 int radix_sort(gh_ctx* c, DevBuf* keys, DevBuf* idx, DevBuf& hist, int64_t n, int words,
-               int lo_shift, hipStream_t s, int* sel_out, int hi_shift = 64);
+               int lo_shift, hipStream_t s, int* sel_out, int hi_shift = 64,
+               bool have_hist0 = false);
 // One-word stable LSD sort of (keys[0], idx[0]) by bits [lo_shift, hi_shift), 8 per pass,
 // one kernel per pass, for at most kRsDirectChunks chunks.  hist holds, per pass p, the
 // digit counts of each chunk of that pass's input at hist + p * 256 * nb (digit d, chunk

@@ -157,34 +157,52 @@ __device__ __forceinline__ int descend(double x, double y, double z, double& cx,
     return (int)gx | ((int)gy << 1) | ((int)gz << 2);
 }
 
+// One workgroup per radix-sort chunk (kRsChunk particles, kKeysItems per thread).
 // packed (nullable) receives (x, y, z, m) of each particle in input order, so that the
 // gather into sorted order after the sort fetches one 32-byte record per particle
-// rather than two scattered pieces (position and mass).
-__global__ __launch_bounds__(kBlock) void k_keys(const double* __restrict__ pos, int64_t n,
-                                                 const double* __restrict__ box, int words,
-                                                 u64* __restrict__ keys, u32* __restrict__ idx,
-                                                 int* __restrict__ flags,
-                                                 const double* __restrict__ mass,
-                                                 double4* __restrict__ packed)
+// rather than two scattered pieces (position and mass).  hist (nullable) receives the
+// first radix pass's digit counts, those of bits hshift..hshift+7 of key word hword, per
+// chunk: hist[d * gridDim.x + chunk], as k_rs_hist would count them.
+constexpr int kKeysBlock = 1024;
+constexpr int kKeysItems = kRsChunk / kKeysBlock;
+static_assert(kKeysBlock >= 256 && kKeysItems * kKeysBlock == kRsChunk, "k_keys shape");
+__global__ __launch_bounds__(kKeysBlock) void k_keys(
+    const double* __restrict__ pos, int64_t n, const double* __restrict__ box, int words,
+    u64* __restrict__ keys, u32* __restrict__ idx, int* __restrict__ flags,
+    const double* __restrict__ mass, double4* __restrict__ packed, u32* __restrict__ hist,
+    int hword, int hshift)
 {
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (flags && i < 4) flags[i] = 0;
-    if (i >= n) return;
-    const double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
-    if (packed) packed[i] = make_double4(x, y, z, mass[i]);
-    double cx = box[0], cy = box[1], cz = box[2];
-    double size = box[3];
-    for (int w = 0; w < words; w++) {
-        u64 k = 0;
-        for (int l = 0; l < kLevelsPerWord; l++) {
-            const double h = 0.5 * size;
-            const int d = descend(x, y, z, cx, cy, cz, h);
-            k = (k << 3) | (u64)d;
-            size = h;
-        }
-        keys[(int64_t)w * n + i] = k;
+    __shared__ u32 h[256];
+    if (hist) {
+        if (threadIdx.x < 256) h[threadIdx.x] = 0u;
+        __syncthreads();
     }
-    idx[i] = (u32)i;
+    if (flags && blockIdx.x == 0 && threadIdx.x < 4) flags[threadIdx.x] = 0;
+    const double bx = box[0], by = box[1], bz = box[2], bs = box[3];
+    for (int it = 0; it < kKeysItems; it++) {
+        const int64_t i = (int64_t)blockIdx.x * kRsChunk + (int64_t)it * kKeysBlock + threadIdx.x;
+        if (i >= n) break;
+        const double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
+        if (packed) packed[i] = make_double4(x, y, z, mass[i]);
+        double cx = bx, cy = by, cz = bz;
+        double size = bs;
+        for (int w = 0; w < words; w++) {
+            u64 k = 0;
+            for (int l = 0; l < kLevelsPerWord; l++) {
+                const double hh = 0.5 * size;
+                const int d = descend(x, y, z, cx, cy, cz, hh);
+                k = (k << 3) | (u64)d;
+                size = hh;
+            }
+            keys[(int64_t)w * n + i] = k;
+            if (hist && w == hword) atomicAdd(&h[(u32)(k >> hshift) & 255u], 1u);
+        }
+        idx[i] = (u32)i;
+    }
+    if (hist) {
+        __syncthreads();
+        if (threadIdx.x < 256) hist[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] = h[threadIdx.x];
+    }
 }
 
 // ------------------------------------------------------------------------------
@@ -212,27 +230,48 @@ __global__ __launch_bounds__(kBlock) void k_rs_hist(const u64* __restrict__ key,
 }
 
 // Per digit d (one workgroup each): exclusive prefix of hist[d][0..nblocks) in place,
-// and the digit's total count in totals[d].
+// and the digit's total count in totals[d].  kRsScanItems consecutive entries per
+// thread, a shuffle scan of the thread sums in each wavefront and one barrier for the
+// wavefronts' totals per round of kBlock * kRsScanItems entries.
+constexpr int kRsScanItems = 4;
 __global__ __launch_bounds__(kBlock) void k_rs_scan(u32* __restrict__ hist, int nblocks,
                                                     u32* __restrict__ totals)
 {
-    __shared__ u32 sh[kBlock];
+    __shared__ u32 wsum[kBlock / 64];
     u32* row = hist + (int64_t)blockIdx.x * nblocks;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     u32 carry = 0;
-    for (int b0 = 0; b0 < nblocks; b0 += kBlock) {
-        const int b = b0 + threadIdx.x;
-        const u32 v = b < nblocks ? row[b] : 0u;
-        sh[threadIdx.x] = v;
-        __syncthreads();
-        for (int off = 1; off < kBlock; off <<= 1) {
-            const u32 t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0u;
-            __syncthreads();
-            sh[threadIdx.x] += t;
-            __syncthreads();
+    for (int b0 = 0; b0 < nblocks; b0 += kBlock * kRsScanItems) {
+        const int b = b0 + threadIdx.x * kRsScanItems;
+        u32 v[kRsScanItems];
+        u32 t = 0;
+#pragma unroll
+        for (int k = 0; k < kRsScanItems; k++) {
+            v[k] = b + k < nblocks ? row[b + k] : 0u;
+            t += v[k];
         }
-        if (b < nblocks) row[b] = carry + sh[threadIdx.x] - v;
-        carry += sh[kBlock - 1];
+        u32 incl = t;  // inclusive scan of t over the wavefront
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const u32 o = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += o;
+        }
+        if (lane == 63) wsum[wave] = incl;
         __syncthreads();
+        u32 before = carry, round = 0;
+#pragma unroll
+        for (int w = 0; w < kBlock / 64; w++) {
+            if (w < wave) before += wsum[w];
+            round += wsum[w];
+        }
+        before += incl - t;
+#pragma unroll
+        for (int k = 0; k < kRsScanItems; k++) {
+            if (b + k < nblocks) row[b + k] = before;
+            before += v[k];
+        }
+        carry += round;
+        __syncthreads();  // wsum is rewritten by the next round
     }
     if (threadIdx.x == 0) totals[blockIdx.x] = carry;
 }
@@ -1503,7 +1542,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_f64(const NodeD* __restrict__ n
 // word (word-major, `words` words); returns which of the two buffer pairs holds the
 // result.
 int radix_sort(gh_ctx* c, DevBuf* keys, DevBuf* idx, DevBuf& hist, int64_t n, int words,
-               int lo_shift, hipStream_t s, int* sel_out, int hi_shift)
+               int lo_shift, hipStream_t s, int* sel_out, int hi_shift, bool have_hist0)
 {
     const int nblocks = (int)((n + kRsChunk - 1) / kRsChunk);
     GH_TRY(ensure(c, hist, sizeof(u32) * 256 * (size_t)(nblocks + 1)));
@@ -1511,9 +1550,11 @@ int radix_sort(gh_ctx* c, DevBuf* keys, DevBuf* idx, DevBuf& hist, int64_t n, in
     for (int w = words - 1; w >= 0; w--) {
         for (int shift = lo_shift; shift < hi_shift; shift += 8) {
             const u64* kin = keys[sel].as<u64>();
-            hipLaunchKernelGGL(k_rs_hist, dim3(nblocks), dim3(kBlock), 0, s, kin + (int64_t)w * n,
-                               n, shift, hist.as<u32>(), nblocks);
-            GH_TRY(check_launch(c, "k_rs_hist"));
+            if (!(have_hist0 && w == words - 1 && shift == lo_shift)) {
+                hipLaunchKernelGGL(k_rs_hist, dim3(nblocks), dim3(kBlock), 0, s,
+                                   kin + (int64_t)w * n, n, shift, hist.as<u32>(), nblocks);
+                GH_TRY(check_launch(c, "k_rs_hist"));
+            }
             hipLaunchKernelGGL(k_rs_scan, dim3(256), dim3(kBlock), 0, s, hist.as<u32>(), nblocks,
                                hist.as<u32>() + (int64_t)256 * nblocks);
             GH_TRY(check_launch(c, "k_rs_scan"));
@@ -1635,17 +1676,20 @@ static int build_tree(gh_ctx* c, const double* d_pos, const double* d_mass, int6
                         kMaxWords * kLevelsPerWord);
         GH_TRY(ensure(c, tb->keys[0], sizeof(u64) * (size_t)n * words));
         GH_TRY(ensure(c, tb->keys[1], sizeof(u64) * (size_t)n * words));
-        // 2. keys (the first thread also clears the flags read after k_delta and
-        // k_sort_runs)
-        hipLaunchKernelGGL(k_keys, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, d_pos, n,
-                           tb->box.as<double>(), words, tb->keys[0].as<u64>(),
+        // 2. keys (the first threads also clear the flags read after k_delta and
+        // k_sort_runs,
+        // and counts the first radix pass's digits)
+        const bool by_runs = words == 1 && !all_bits;
+        const int lo_bits = by_runs ? kSortLowBits : 0;
+        GH_TRY(ensure(c, tb->hist, sizeof(u32) * 256 * (size_t)(blocks_for(n, kRsChunk) + 1)));
+        hipLaunchKernelGGL(k_keys, dim3(blocks_for(n, kRsChunk)), dim3(kKeysBlock), 0, s, d_pos,
+                           n, tb->box.as<double>(), words, tb->keys[0].as<u64>(),
                            tb->idx[0].as<u32>(), tb->flags.as<int>(), d_mass,
-                           tb->packed.as<double4>());
+                           tb->packed.as<double4>(), tb->hist.as<u32>(), words - 1, lo_bits);
         GH_TRY(check_launch(c, "k_keys"));
         // 3. sort
-        const bool by_runs = words == 1 && !all_bits;
-        GH_TRY(radix_sort(c, tb->keys, tb->idx, tb->hist, n, words, by_runs ? kSortLowBits : 0,
-                          s, &tb->key_sel));
+        GH_TRY(radix_sort(c, tb->keys, tb->idx, tb->hist, n, words, lo_bits, s, &tb->key_sel,
+                          64, true));
         if (by_runs) {
             hipLaunchKernelGGL(k_sort_runs, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s,
                                tb->keys[tb->key_sel].as<u64>(), tb->idx[tb->key_sel].as<u32>(),
@@ -1776,9 +1820,10 @@ static int walk(gh_ctx* c, const double* tgt, bool self, int64_t nt, double eps,
             GH_TRY(ensure(c, tb->tkeys[k], sizeof(u64) * (size_t)nt));
             GH_TRY(ensure(c, tb->tidx[k], sizeof(u32) * (size_t)nt));
         }
-        hipLaunchKernelGGL(k_keys, dim3(blocks_for(nt, kBlock)), dim3(kBlock), 0, s, tgt, nt,
-                           tb->box.as<double>(), 1, tb->tkeys[0].as<u64>(), tb->tidx[0].as<u32>(),
-                           (int*)nullptr, (const double*)nullptr, (double4*)nullptr);
+        hipLaunchKernelGGL(k_keys, dim3(blocks_for(nt, kRsChunk)), dim3(kKeysBlock), 0, s, tgt,
+                           nt, tb->box.as<double>(), 1, tb->tkeys[0].as<u64>(),
+                           tb->tidx[0].as<u32>(), (int*)nullptr, (const double*)nullptr,
+                           (double4*)nullptr, (u32*)nullptr, 0, 0);
         GH_TRY(check_launch(c, "k_keys"));
         int sel = 0;
         GH_TRY(radix_sort(c, tb->tkeys, tb->tidx, tb->hist, nt, 1, 24, s, &sel));
