@@ -72,7 +72,7 @@ SIGNATURES = {
 # gh_set_tuning keys
 TUNE_DIRECT_SELF, TUNE_DIRECT_IPT, TUNE_WALK_GROUP, TUNE_RUN_GRAPH = 0, 1, 2, 3
 TUNE_WALK_F64EVAL, TUNE_ALLOC_LIMIT_MB, TUNE_DIRECT_F64_TPW = 4, 5, 6
-TUNE_SYM_BAND, TUNE_SORT_DIRECT, TUNE_TREE_SORT = 7, 8, 9
+TUNE_SYM_BAND, TUNE_SORT_DIRECT, TUNE_TREE_SORT, TUNE_WALK_PREFIX = 7, 8, 9, 10
 
 # kernel classes of gh_timing_read
 T_DIRECT_F32, T_DIRECT_F64, T_TREE_BUILD, T_TREE_WALK, T_UPDATE, T_VJP = range(6)

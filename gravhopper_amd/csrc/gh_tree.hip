@@ -87,6 +87,7 @@ struct TreeBufs {
     DevBuf hist, bbox_part, box, flags, delta, off, prefix, block_sums, block_counts, packed;
     DevBuf spos;  // (x, y, z, m) of the particles in sorted order
     DevBuf nodes64, nodesw, kids, skip, level, branch, leaf;
+    DevBuf wprefix;  // FP32 walk prefixes per block of groups (k_walk_prefix)
     DevBuf tkeys[2], tidx[2];  // Morton order of position targets (tree_force_position)
     long long* readback = nullptr;  // pinned host words: flags[0..1], node count, flags[2]
     long long* readback_dev = nullptr;  // the same words' device address
@@ -996,6 +997,12 @@ __device__ __forceinline__ int64_t xcd_block(int64_t per_xcd)
 // receives each target's list length.
 constexpr int kStackCap = 768;
 constexpr int kListCap = 128;
+// Walk prefixes (k_walk_prefix): groups per block, list entries and steps at most, and the
+// ints of one block's state (stack height, list length, stack, list).
+constexpr int kPrefixGroups = 64;
+constexpr int kPrefixCap = 1024;
+constexpr int kPrefixSteps = 32;
+constexpr int kPrefixInts = 2 + kStackCap + kPrefixCap;
 // Two wavefronts per workgroup, each walking its own group: a CU holds at most 16
 // workgroups, so one-wavefront workgroups would cap it at 16 walks.
 constexpr int kGroupWaves = 2;
@@ -1077,7 +1084,8 @@ __device__ __forceinline__ void walk_lists(
     int n_nodes, double ox, double oy, double oz, float bcx, float bcy, float bcz, float bhx,
     float bhy, float bhz, double dx0, double dy0, double dz0, int slice, int nslices,
     float inv_theta, double eps2, int* stack, typename GroupEntry<F64>::T* list,
-    u64* fails, int cnt, double& ax, double& ay, double& az, int& listed)
+    u64* fails, int cnt, double& ax, double& ay, double& az, int& listed,
+    const int* __restrict__ prefix)
 {
     typedef typename GroupEntry<F64>::T Entry;
     const int lane = threadIdx.x & 63;
@@ -1204,8 +1212,28 @@ __device__ __forceinline__ void walk_lists(
     };
 
     wave_sync();  // the previous walk of this wavefront is done with stack and list
-    if (lane == 0) stack[0] = 0;
-    int top = n_nodes > 0 ? 1 : 0;
+    int top = 0;
+    if (prefix) {
+        // the state k_walk_prefix left for this block: stack, then the accepted nodes'
+        // entries, flushed as the steps that accepted them would have flushed them
+        top = prefix[0];
+        const int npre = prefix[1];
+        const int* pstack = prefix + 2;
+        const int* plist = pstack + top;
+        for (int j = lane; j < top; j += 64) stack[j] = pstack[j];
+        for (int b = 0; b < npre; b += 64) {
+            const int take = npre - b < 64 ? npre - b : 64;
+            const WalkRec rec = load_rec(nodes, lane < take ? plist[b + lane] : 0);
+            if (lane < take) list[held + lane] = entry_of(rec);
+            held += take;
+            listed += take;
+            wave_sync();
+            while (held >= 64) flush(64);
+        }
+    } else {
+        if (lane == 0) stack[0] = 0;
+        top = n_nodes > 0 ? 1 : 0;
+    }
     wave_sync();
     while (top > 0) {
         int k = top < 64 ? top : 64;
@@ -1298,12 +1326,143 @@ __device__ __forceinline__ int exponent_f(float x)
     return (int)((__float_as_int(x) >> 23) & 0xff) - 127;  // 0 gives -127
 }
 
+// Walk prefixes.  The first steps of the group walk are alike for neighbouring groups:
+// near the root each node the walk pops is either far from all of them (accepted) or
+// near all of them (opened).  One wavefront per block of kPrefixGroups consecutive
+// groups runs those steps once, as walk_lists does them (the same pops, pushes and list
+// order), for as long as every node a step pops is decided alike for every target of
+// the block, and stores the state there: prefix[0] = stack height h, prefix[1] = list
+// length L, then the stack (h ids) and the accepted nodes (L ids).  Each group of the
+// block starts its walk from that state, so it builds the same lists in the same order
+// as from the root, and its forces are the same bits.  A node is decided alike when it
+// is a leaf (always accepted), or when, with R = size / theta and the box of the block's
+// targets, its centre is farther than R + mu from all of the box (every target passes
+// the reference's test there, so every group accepts it) or nearer than R - mu to all
+// of it (no target passes, so every group opens it).  mu = 1e-4 (R + the box's largest
+// half-size) covers the FP32 rounding of the groups' tests (about 1e-7 of the distances)
+// and the widening of their boxes (1e-6 of their half-sizes).  The steps stop also where
+// the walk would turn to its sequential form (stack nearly full), after kPrefixSteps, or
+// before the list would exceed kPrefixCap entries.
+__global__ __launch_bounds__(64) void k_walk_prefix(
+    const NodeW* __restrict__ nodes, const int* __restrict__ kids, int n_nodes,
+    const double* __restrict__ tgt, const double4* __restrict__ stgt,
+    const u32* __restrict__ order, int64_t nt, int G, double inv_theta,
+    int* __restrict__ prefixes)
+{
+    __shared__ int stack[kStackCap];
+    __shared__ int list[kPrefixCap];
+    const int lane = threadIdx.x;
+    int* const out = prefixes + (int64_t)blockIdx.x * kPrefixInts;
+    const int64_t b0 = (int64_t)blockIdx.x * kPrefixGroups * G;
+    const int64_t b1 = b0 + (int64_t)kPrefixGroups * G < nt ? b0 + (int64_t)kPrefixGroups * G : nt;
+    // the block's target box, in FP64
+    double lo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, hi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+    for (int64_t k = b0 + lane; k < b1; k += 64) {
+        double q[3];
+        if (stgt) {
+            const double4 p = stgt[k];
+            q[0] = p.x, q[1] = p.y, q[2] = p.z;
+        } else {
+            const int64_t t = order ? (int64_t)order[k] : k;
+            q[0] = tgt[3 * t], q[1] = tgt[3 * t + 1], q[2] = tgt[3 * t + 2];
+        }
+        for (int d = 0; d < 3; d++) {
+            lo[d] = fmin(lo[d], q[d]);
+            hi[d] = fmax(hi[d], q[d]);
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1)
+        for (int d = 0; d < 3; d++) {
+            lo[d] = fmin(lo[d], __shfl_xor(lo[d], o, 64));
+            hi[d] = fmax(hi[d], __shfl_xor(hi[d], o, 64));
+        }
+    double bc[3], bh[3], bhmax = 0.0;
+    for (int d = 0; d < 3; d++) {
+        bc[d] = 0.5 * (lo[d] + hi[d]);
+        bh[d] = 0.5 * (hi[d] - lo[d]);
+        bhmax = fmax(bhmax, bh[d]);
+    }
+    // a box that is not finite (non-finite targets) decides nothing: the walk from the root
+    const bool finite = isfinite(bhmax) && isfinite(bc[0]) && isfinite(bc[1]) && isfinite(bc[2]);
+    int top = n_nodes > 0 ? 1 : 0, held = 0;
+    if (lane == 0) stack[0] = 0;
+    wave_sync();
+    for (int step = 0; finite && step < kPrefixSteps && top > 0; step++) {
+        int k = top < 64 ? top : 64;
+        const int room = (kStackCap - top) / 7;
+        if (room < 1) break;
+        if (k > room) k = room;
+        const bool active = lane < k;
+        const int n = active ? stack[top - 1 - lane] : 0;
+        const WalkRec rec = load_rec(nodes, n);
+        bool acc = false, open = false;
+        if (active) {
+            if (__float_as_int(rec.size) < 0) {
+                acc = true;
+            } else {
+                const double R = (double)rec.size * inv_theta;
+                const double c[3] = {rec.cx, rec.cy, rec.cz};
+                double dmin2 = 0.0, dmax2 = 0.0;
+                for (int d = 0; d < 3; d++) {
+                    const double a = fabs(c[d] - bc[d]);
+                    const double e = fmax(0.0, a - bh[d]);
+                    dmin2 += e * e;
+                    dmax2 += (a + bh[d]) * (a + bh[d]);
+                }
+                if (isinf(R)) {
+                    open = true;
+                } else {
+                    const double mu = 1e-4 * (R + bhmax);
+                    if (dmin2 > (R + mu) * (R + mu)) acc = true;
+                    else if (R > mu && dmax2 < (R - mu) * (R - mu)) open = true;
+                }
+            }
+        }
+        if (__ballot(active && !acc && !open) != 0) break;  // a node decided differently
+        const u64 am = __ballot(acc);
+        const int added = (int)__popcll(am);
+        if (held + added > kPrefixCap) break;
+        int4 ka = make_int4(-1, -1, -1, -1), kb = ka;
+        if (open) {
+            ka = *(const int4*)(kids + 8 * (int64_t)n);
+            kb = *(const int4*)(kids + 8 * (int64_t)n + 4);
+        }
+        int nk = 8 + (ka.x >> 31) + (ka.y >> 31) + (ka.z >> 31) + (ka.w >> 31) + (kb.x >> 31) +
+                 (kb.y >> 31) + (kb.z >> 31) + (kb.w >> 31);
+        if (!open) nk = 0;
+        int total;
+        const int pre = wave_excl_count(nk, total);
+        wave_sync();  // every lane has read its popped id
+        if (acc) list[held + lanes_below(am)] = n;
+        if (nk > 0) {
+            const int w = top - k + pre;
+            const int kk[8] = {ka.x, ka.y, ka.z, ka.w, kb.x, kb.y, kb.z, kb.w};
+#pragma unroll
+            for (int q = 0; q < 8; q++)
+                if (q < nk) stack[w + q] = kk[q];
+        }
+        held += added;
+        top = top - k + total;
+        wave_sync();
+    }
+    // the stack goes right after the two counts, the list right after the stack
+    int* const pstack = out + 2;
+    int* const plist = pstack + top;
+    for (int j = lane; j < top; j += 64) pstack[j] = stack[j];
+    for (int j = lane; j < held; j += 64) plist[j] = list[j];
+    if (lane == 0) {
+        out[0] = top;
+        out[1] = held;
+    }
+}
+
 template <bool F64, bool GUARD>
 __global__ __launch_bounds__(kGroupBlock) void k_walk_group(
     const NodeW* __restrict__ nodes, const int* __restrict__ kids, const int* __restrict__ skip,
     int n_nodes, const double* __restrict__ tgt, const double4* __restrict__ stgt,
     const u32* __restrict__ order, int64_t nt, int G, float inv_theta, double eps2,
-    double* __restrict__ acc, int* __restrict__ counts, int64_t per_xcd)
+    double* __restrict__ acc, int* __restrict__ counts, int64_t per_xcd,
+    const int* __restrict__ prefixes)
 {
     typedef typename GroupEntry<F64>::T Entry;
     __shared__ int stacks[kGroupWaves][kStackCap];
@@ -1318,6 +1477,8 @@ __global__ __launch_bounds__(kGroupBlock) void k_walk_group(
     if (first >= nt) return;
     const int cnt = (int)(nt - first < G ? nt - first : G);
     const int ti = lane & (G - 1);  // this lane's target within the group
+    const int* const prefix =
+        prefixes ? prefixes + (first / G / kPrefixGroups) * kPrefixInts : nullptr;
     auto target = [&](int64_t k) -> double4 {
         if (stgt) return stgt[k];
         const int64_t t = order ? (int64_t)order[k] : k;
@@ -1370,7 +1531,7 @@ __global__ __launch_bounds__(kGroupBlock) void k_walk_group(
         int listed = 0;
         walk_lists<F64, GUARD>(nodes, kids, skip, n_nodes, o.x, o.y, o.z, bcx, bcy, bcz, bhx,
                                bhy, bhz, dx0, dy0, dz0, lane / G, 64 / G, inv_theta, eps2,
-                               stack, list, fails, cnt, ax, ay, az, listed);
+                               stack, list, fails, cnt, ax, ay, az, listed, prefix);
         // add the slices of each target (lanes ti, ti + G, ...), in FP64
         for (int s = G; s < 64; s <<= 1) {
             ax += __shfl_xor(ax, s);
@@ -1388,7 +1549,7 @@ __global__ __launch_bounds__(kGroupBlock) void k_walk_group(
         int listed = 0;
         walk_lists<F64, GUARD>(nodes, kids, skip, n_nodes, p.x, p.y, p.z, 0.f, 0.f, 0.f, 0.f,
                                0.f, 0.f, 0.0, 0.0, 0.0, lane, 64, inv_theta, eps2, stack, list,
-                               fails, 1, ax, ay, az, listed);
+                               fails, 1, ax, ay, az, listed, prefix);
         for (int s = 1; s < 64; s <<= 1) {
             ax += __shfl_xor(ax, s);
             ay += __shfl_xor(ay, s);
@@ -1872,9 +2033,19 @@ static int walk(gh_ctx* c, const double* tgt, bool self, int64_t nt, double eps,
     const int* sk = tb->skip.as<int>();
     const int T = (int)tb->n_nodes;
     const double eps2 = eps * eps;
+    // the walks' shared first steps, per block of kPrefixGroups groups
+    int* pf = nullptr;
+    if (c->tune[GH_TUNE_WALK_PREFIX] >= 0) {
+        const int64_t nblk = (n_groups + kPrefixGroups - 1) / kPrefixGroups;
+        GH_TRY(ensure(c, tb->wprefix, sizeof(int) * (size_t)kPrefixInts * (size_t)nblk));
+        pf = tb->wprefix.as<int>();
+        hipLaunchKernelGGL(k_walk_prefix, dim3((unsigned)nblk), dim3(64), 0, s, nd, kd, T, tgt,
+                           stgt, order, nt, G, theta > 0.0 ? 1.0 / theta : HUGE_VAL, pf);
+        GH_TRY(check_launch(c, "k_walk_prefix"));
+    }
 #define GH_GWALK(F, GU)                                                                    \
     hipLaunchKernelGGL((k_walk_group<F, GU>), grid, dim3(kGroupBlock), 0, s, nd, kd, sk, T, tgt, \
-                       stgt, order, nt, G, inv_theta, eps2, d_acc, counts, per)
+                       stgt, order, nt, G, inv_theta, eps2, d_acc, counts, per, pf)
     if (f64eval)
         GH_GWALK(true, false);
     else if (guard)
@@ -1908,7 +2079,7 @@ void tree_free(gh_ctx* c)
                       &tb->bbox_part, &tb->box,   &tb->flags,     &tb->delta,   &tb->off,
                       &tb->prefix,  &tb->spos,      &tb->block_sums, &tb->block_counts,
                       &tb->packed,
-                      &tb->nodes64, &tb->nodesw, &tb->kids,
+                      &tb->nodes64, &tb->nodesw, &tb->kids,  &tb->wprefix,
                       &tb->skip,    &tb->level,   &tb->branch,    &tb->leaf,    &tb->tkeys[0],
                       &tb->tkeys[1], &tb->tidx[0], &tb->tidx[1]};
     for (DevBuf* b : bufs) release(c, *b);

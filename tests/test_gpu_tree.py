@@ -284,6 +284,32 @@ def test_fp64_walk_independent_of_target_grouping(ctx):
     assert_bitwise(all_pos, self_acc[pick], "position targets vs self walk")
 
 
+@pytest.mark.parametrize("group", [0, 1, 4, 16])
+def test_fp32_walk_prefix_same_bits(ctx, group):
+    # the walks of a block of groups start from the steps they share (run once per block
+    # by k_walk_prefix); their lists, and so their forces, are those of walks from the root
+    rng = np.random.default_rng(404)
+    g = workloads.galaxy(n_halo=40000, n_disk=40000, seed=405)
+    cases = [("galaxy", g["pos"], g["mass"], 0.05, None),
+             ("cauchy", rng.standard_cauchy(size=(30000, 3)), rng.uniform(1, 2, 30000), 1e-3, None),
+             ("positions", g["pos"], g["mass"], 0.05,
+              np.concatenate([g["pos"][:5000], rng.normal(size=(9000, 3)) * 300.0]))]
+    ctx.set_tuning(_native.TUNE_WALK_GROUP, group)
+    try:
+        for name, pos, mass, eps, fpos in cases:
+            for theta in (0.0, 0.4, 0.7, 1.2):
+                print(name, "G", group, "theta", theta, flush=True)
+                got = ctx.tree(pos, mass, eps, theta, FP32, force_pos=fpos)
+                ctx.set_tuning(_native.TUNE_WALK_PREFIX, -1)
+                try:
+                    want = ctx.tree(pos, mass, eps, theta, FP32, force_pos=fpos)
+                finally:
+                    ctx.set_tuning(_native.TUNE_WALK_PREFIX, 0)
+                assert_bitwise(got, want, "%s G=%d theta=%g" % (name, group, theta))
+    finally:
+        ctx.set_tuning(_native.TUNE_WALK_GROUP, 0)
+
+
 def test_fp32_walk_few_targets(ctx, oracle):
     # few position targets are spread over many short walks (down to one target each)
     rng = np.random.default_rng(88)
