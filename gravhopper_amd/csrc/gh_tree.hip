@@ -629,9 +629,9 @@ struct OpDD4 {
         for (int k = 0; k < 4; k++) dd_add(a.h[k], a.l[k], b.h[k], b.l[k]);
         return a;
     }
-    __device__ static T load(const In* in, int64_t i)
+    __device__ static T load(const In* in, int64_t i) { return of(in[i]); }
+    __device__ static T of(const double4 q)
     {
-        const double4 q = in[i];
         DD4 r;
         r.h[0] = q.w;
         r.h[1] = q.w * q.x;
@@ -715,15 +715,72 @@ __device__ __forceinline__ typename Op::T scan_sums_block(typename Op::T* __rest
 
 // The fill's two scans over the sorted particles, local phase: node counts cnt -> first
 // node ids off, and moments (m, m x, m y, m z) of spos -> double-double prefixes pref.
+// Both scans share their loads and one ladder over the threads' sums.
+struct CountMoments {
+    long long c;
+    DD4 d;
+};
 __global__ __launch_bounds__(kBlock) void k_scan_local_fill(
     const long long* __restrict__ cnt, const double4* __restrict__ spos, int64_t n,
     long long* __restrict__ off, long long* __restrict__ off_sums, DD4* __restrict__ pref,
     DD4* __restrict__ pref_sums)
 {
-    __shared__ long long shi[kBlock];
-    __shared__ DD4 shd[kBlock];
-    scan_local_block<OpI64>(cnt, n, off, off_sums, shi);
-    scan_local_block<OpDD4>(spos, n, pref, pref_sums, shd);
+    __shared__ CountMoments sh[kBlock];
+    const int64_t b = (int64_t)blockIdx.x * kScanChunk + (int64_t)threadIdx.x * kScanItems;
+    long long ci[kScanItems];
+    double4 qi[kScanItems];
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        const bool in = b + k < n;
+        ci[k] = in ? cnt[b + k] : 0;
+        qi[k] = in ? spos[b + k] : make_double4(0.0, 0.0, 0.0, 0.0);
+    }
+    CountMoments run;
+    run.c = 0;
+    run.d = OpDD4::zero();
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        run.c += ci[k];
+        run.d = OpDD4::add(run.d, OpDD4::of(qi[k]));
+    }
+    sh[threadIdx.x] = run;
+    __syncthreads();
+    for (int o = 1; o < kBlock; o <<= 1) {
+        CountMoments t;
+        t.c = 0;
+        t.d = OpDD4::zero();
+        const bool take = threadIdx.x >= o;
+        if (take) t = sh[threadIdx.x - o];
+        __syncthreads();
+        if (take) {
+            sh[threadIdx.x].c += t.c;
+            sh[threadIdx.x].d = OpDD4::add(t.d, sh[threadIdx.x].d);
+        }
+        __syncthreads();
+    }
+    CountMoments before;
+    if (threadIdx.x) {
+        before = sh[threadIdx.x - 1];
+    } else {
+        before.c = 0;
+        before.d = OpDD4::zero();
+    }
+    // element k: the earlier threads' sum plus this thread's sum before k
+    long long cv = 0;
+    DD4 dv = OpDD4::zero();
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        if (b + k < n) {
+            off[b + k] = before.c + cv;
+            pref[b + k] = OpDD4::add(before.d, dv);
+        }
+        cv += ci[k];
+        dv = OpDD4::add(dv, OpDD4::of(qi[k]));
+    }
+    if (threadIdx.x == kBlock - 1) {
+        off_sums[blockIdx.x] = sh[kBlock - 1].c;
+        pref_sums[blockIdx.x] = sh[kBlock - 1].d;
+    }
 }
 
 // Two workgroups after k_scan_local_fill, side by side: workgroup 1 scans the moments'
@@ -1357,19 +1414,28 @@ __global__ __launch_bounds__(64) void k_walk_prefix(
     const int64_t b1 = b0 + (int64_t)kPrefixGroups * G < nt ? b0 + (int64_t)kPrefixGroups * G : nt;
     // the block's target box, in FP64
     double lo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, hi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
-    for (int64_t k = b0 + lane; k < b1; k += 64) {
-        double q[3];
-        if (stgt) {
-            const double4 p = stgt[k];
-            q[0] = p.x, q[1] = p.y, q[2] = p.z;
-        } else {
-            const int64_t t = order ? (int64_t)order[k] : k;
-            q[0] = tgt[3 * t], q[1] = tgt[3 * t + 1], q[2] = tgt[3 * t + 2];
+    constexpr int kBatch = 8;  // loads in flight per lane
+    bool bad = false;          // a coordinate that is not finite
+    for (int64_t k0 = b0 + lane; k0 < b1; k0 += 64 * kBatch) {
+        double q[kBatch][3];
+#pragma unroll
+        for (int u = 0; u < kBatch; u++) {
+            const int64_t k = k0 + 64 * u < b1 ? k0 + 64 * u : k0;  // k0 < b1 repeats
+            if (stgt) {
+                const double4 p = stgt[k];
+                q[u][0] = p.x, q[u][1] = p.y, q[u][2] = p.z;
+            } else {
+                const int64_t t = order ? (int64_t)order[k] : k;
+                q[u][0] = tgt[3 * t], q[u][1] = tgt[3 * t + 1], q[u][2] = tgt[3 * t + 2];
+            }
         }
-        for (int d = 0; d < 3; d++) {
-            lo[d] = fmin(lo[d], q[d]);
-            hi[d] = fmax(hi[d], q[d]);
-        }
+#pragma unroll
+        for (int u = 0; u < kBatch; u++)
+            for (int d = 0; d < 3; d++) {
+                bad = bad || !isfinite(q[u][d]);
+                lo[d] = fmin(lo[d], q[u][d]);
+                hi[d] = fmax(hi[d], q[u][d]);
+            }
     }
     for (int o = 32; o > 0; o >>= 1)
         for (int d = 0; d < 3; d++) {
@@ -1382,8 +1448,9 @@ __global__ __launch_bounds__(64) void k_walk_prefix(
         bh[d] = 0.5 * (hi[d] - lo[d]);
         bhmax = fmax(bhmax, bh[d]);
     }
-    // a box that is not finite (non-finite targets) decides nothing: the walk from the root
-    const bool finite = isfinite(bhmax) && isfinite(bc[0]) && isfinite(bc[1]) && isfinite(bc[2]);
+    // a block with a target that is not finite decides nothing: its walks start at the root
+    const bool finite = __ballot(bad) == 0 && isfinite(bhmax) && isfinite(bc[0]) &&
+                        isfinite(bc[1]) && isfinite(bc[2]);
     int top = n_nodes > 0 ? 1 : 0, held = 0;
     if (lane == 0) stack[0] = 0;
     wave_sync();

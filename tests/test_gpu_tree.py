@@ -290,10 +290,11 @@ def test_fp32_walk_prefix_same_bits(ctx, group):
     # by k_walk_prefix); their lists, and so their forces, are those of walks from the root
     rng = np.random.default_rng(404)
     g = workloads.galaxy(n_halo=40000, n_disk=40000, seed=405)
+    fpos = np.concatenate([g["pos"][:5000], rng.normal(size=(9000, 3)) * 300.0])
+    fpos[7000, 1] = np.nan  # its block of groups starts from the root
     cases = [("galaxy", g["pos"], g["mass"], 0.05, None),
              ("cauchy", rng.standard_cauchy(size=(30000, 3)), rng.uniform(1, 2, 30000), 1e-3, None),
-             ("positions", g["pos"], g["mass"], 0.05,
-              np.concatenate([g["pos"][:5000], rng.normal(size=(9000, 3)) * 300.0]))]
+             ("positions", g["pos"], g["mass"], 0.05, fpos)]
     ctx.set_tuning(_native.TUNE_WALK_GROUP, group)
     try:
         for name, pos, mass, eps, fpos in cases:
