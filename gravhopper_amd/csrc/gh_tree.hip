@@ -1904,9 +1904,8 @@ static int build_tree(gh_ctx* c, const double* d_pos, const double* d_mass, int6
                         kMaxWords * kLevelsPerWord);
         GH_TRY(ensure(c, tb->keys[0], sizeof(u64) * (size_t)n * words));
         GH_TRY(ensure(c, tb->keys[1], sizeof(u64) * (size_t)n * words));
-        // 2. keys (the first threads also clear the flags read after k_delta and
-        // k_sort_runs,
-        // and counts the first radix pass's digits)
+        // 2. keys; the kernel also counts the first radix pass's digits, and its first
+        // threads clear the flags read after k_delta and k_sort_runs
         const bool by_runs = words == 1 && !all_bits;
         const int lo_bits = by_runs ? kSortLowBits : 0;
         GH_TRY(ensure(c, tb->hist, sizeof(u32) * 256 * (size_t)(blocks_for(n, kRsChunk) + 1)));
