@@ -141,6 +141,22 @@ def test_topology_bitwise_key_runs(ctx, oracle, mode):
         ctx.set_tuning(_native.TUNE_TREE_SORT, 0)
 
 
+@pytest.mark.parametrize("n", [2, 3, 5])
+def test_topology_bitwise_key_runs_small(ctx, oracle, n):
+    # n particles 1e-7 apart in a root box of about 10 (set by one far particle): their
+    # keys agree over levels 0-12, one run of n
+    rng = np.random.default_rng(n)
+    pos = np.concatenate([rng.normal(size=(1, 3)) + rng.normal(size=(n, 3)) * 1e-7,
+                          [[10.0, -3.0, 2.0]]])
+    mass = rng.uniform(1, 2, n + 1)
+    ctx.set_tuning(_native.TUNE_TREE_SORT, 1)
+    try:
+        compare_topology(ctx.tree_dump(pos, mass, 0.0), oracle.tree_dump(pos, mass, 0.0),
+                         "one run of %d" % n)
+    finally:
+        ctx.set_tuning(_native.TUNE_TREE_SORT, 0)
+
+
 @pytest.mark.parametrize("mode", [1, 0])
 def test_topology_bitwise_long_key_run(ctx, oracle, mode):
     # a run of 65 or 300 keys is longer than k_sort_runs takes: the build sorts again over
@@ -284,7 +300,7 @@ def test_fp64_walk_independent_of_target_grouping(ctx):
     assert_bitwise(all_pos, self_acc[pick], "position targets vs self walk")
 
 
-@pytest.mark.parametrize("group", [0, 1, 4, 16])
+@pytest.mark.parametrize("group", [0, 1, 4, 16, 64])
 def test_fp32_walk_prefix_same_bits(ctx, group):
     # the walks of a block of groups start from the steps they share (run once per block
     # by k_walk_prefix); their lists, and so their forces, are those of walks from the root
@@ -294,7 +310,9 @@ def test_fp32_walk_prefix_same_bits(ctx, group):
     fpos[7000, 1] = np.nan  # its block of groups starts from the root
     cases = [("galaxy", g["pos"], g["mass"], 0.05, None),
              ("cauchy", rng.standard_cauchy(size=(30000, 3)), rng.uniform(1, 2, 30000), 1e-3, None),
-             ("positions", g["pos"], g["mass"], 0.05, fpos)]
+             ("positions", g["pos"], g["mass"], 0.05, fpos),
+             ("tiny", rng.normal(size=(50, 3)), rng.uniform(1, 2, 50), 1e-2,
+              rng.normal(size=(7, 3)))]
     ctx.set_tuning(_native.TUNE_WALK_GROUP, group)
     try:
         for name, pos, mass, eps, fpos in cases:
