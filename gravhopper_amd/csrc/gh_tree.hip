@@ -1058,7 +1058,7 @@ constexpr int kListCap = 128;
 // ints of one block's state (stack height, list length, stack, list).
 constexpr int kPrefixGroups = 64;
 constexpr int kPrefixCap = 1024;
-constexpr int kPrefixSteps = 32;
+constexpr int kPrefixSteps = 256;
 constexpr int kPrefixInts = 2 + kStackCap + kPrefixCap;
 // Two wavefronts per workgroup, each walking its own group: a CU holds at most 16
 // workgroups, so one-wavefront workgroups would cap it at 16 walks.
@@ -1383,23 +1383,25 @@ __device__ __forceinline__ int exponent_f(float x)
     return (int)((__float_as_int(x) >> 23) & 0xff) - 127;  // 0 gives -127
 }
 
-// Walk prefixes.  The first steps of the group walk are alike for neighbouring groups:
-// near the root each node the walk pops is either far from all of them (accepted) or
-// near all of them (opened).  One wavefront per block of kPrefixGroups consecutive
-// groups runs those steps once, as walk_lists does them (the same pops, pushes and list
-// order), for as long as every node a step pops is decided alike for every target of
-// the block, and stores the state there: prefix[0] = stack height h, prefix[1] = list
-// length L, then the stack (h ids) and the accepted nodes (L ids).  Each group of the
-// block starts its walk from that state, so it builds the same lists in the same order
-// as from the root, and its forces are the same bits.  A node is decided alike when it
-// is a leaf (always accepted), or when, with R = size / theta and the box of the block's
-// targets, its centre is farther than R + mu from all of the box (every target passes
+// Walk prefixes.  Much of a group's walk is the same for its neighbours: far from all
+// of them, nodes are accepted; near all of them, nodes are opened.  One wavefront per
+// block of kPrefixGroups consecutive groups walks the tree once for the block's target
+// box, deciding every node that is decided alike for all of the block's targets and
+// deferring the others, and stores prefix[0] = h, prefix[1] = L, then the h deferred
+// nodes and the L accepted ones.  Each group of the block starts its walk with the
+// accepted nodes in its list and the deferred ones on its stack, so it accepts exactly
+// the nodes its walk from the root accepts (whether a node is visited and accepted does
+// not depend on the order of the walk), only in another order.  A node is decided alike
+// when it is a leaf (always accepted), or when, with R = size / theta and the block's
+// target box, its centre is farther than R + mu from all of the box (every target passes
 // the reference's test there, so every group accepts it) or nearer than R - mu to all
 // of it (no target passes, so every group opens it).  mu = 1e-4 (R + the box's largest
 // half-size) covers the FP32 rounding of the groups' tests (about 1e-7 of the distances)
-// and the widening of their boxes (1e-6 of their half-sizes).  The steps stop also where
-// the walk would turn to its sequential form (stack nearly full), after kPrefixSteps, or
-// before the list would exceed kPrefixCap entries.
+// and the widening of their boxes (1e-6 of their half-sizes).  The walk stops where its
+// stack is nearly full, after kPrefixSteps steps, or before its list or deferred nodes
+// would overflow; what is left on its stack is deferred too.  A block whose deferred
+// nodes would not fit a walk's stack (or with a target that is not finite) starts its
+// walks at the root.
 __global__ __launch_bounds__(64) void k_walk_prefix(
     const NodeW* __restrict__ nodes, const int* __restrict__ kids, int n_nodes,
     const double* __restrict__ tgt, const double4* __restrict__ stgt,
@@ -1408,6 +1410,7 @@ __global__ __launch_bounds__(64) void k_walk_prefix(
 {
     __shared__ int stack[kStackCap];
     __shared__ int list[kPrefixCap];
+    __shared__ int defer[kStackCap];
     const int lane = threadIdx.x;
     int* const out = prefixes + (int64_t)blockIdx.x * kPrefixInts;
     const int64_t b0 = (int64_t)blockIdx.x * kPrefixGroups * G;
@@ -1451,7 +1454,7 @@ __global__ __launch_bounds__(64) void k_walk_prefix(
     // a block with a target that is not finite decides nothing: its walks start at the root
     const bool finite = __ballot(bad) == 0 && isfinite(bhmax) && isfinite(bc[0]) &&
                         isfinite(bc[1]) && isfinite(bc[2]);
-    int top = n_nodes > 0 ? 1 : 0, held = 0;
+    int top = n_nodes > 0 ? 1 : 0, held = 0, nd = 0;
     if (lane == 0) stack[0] = 0;
     wave_sync();
     for (int step = 0; finite && step < kPrefixSteps && top > 0; step++) {
@@ -1485,10 +1488,12 @@ __global__ __launch_bounds__(64) void k_walk_prefix(
                 }
             }
         }
-        if (__ballot(active && !acc && !open) != 0) break;  // a node decided differently
-        const u64 am = __ballot(acc);
-        const int added = (int)__popcll(am);
-        if (held + added > kPrefixCap) break;
+        const bool undecided = active && !acc && !open;
+        const u64 am = __ballot(acc), um = __ballot(undecided);
+        const int added = (int)__popcll(am), ndef = (int)__popcll(um);
+        // a step whose results would not fit is not taken: its popped ids stay on the
+        // stack and are deferred with the rest
+        if (held + added > kPrefixCap || nd + ndef > kStackCap) break;
         int4 ka = make_int4(-1, -1, -1, -1), kb = ka;
         if (open) {
             ka = *(const int4*)(kids + 8 * (int64_t)n);
@@ -1501,6 +1506,7 @@ __global__ __launch_bounds__(64) void k_walk_prefix(
         const int pre = wave_excl_count(nk, total);
         wave_sync();  // every lane has read its popped id
         if (acc) list[held + lanes_below(am)] = n;
+        if (undecided) defer[nd + lanes_below(um)] = n;
         if (nk > 0) {
             const int w = top - k + pre;
             const int kk[8] = {ka.x, ka.y, ka.z, ka.w, kb.x, kb.y, kb.z, kb.w};
@@ -1509,16 +1515,28 @@ __global__ __launch_bounds__(64) void k_walk_prefix(
                 if (q < nk) stack[w + q] = kk[q];
         }
         held += added;
+        nd += ndef;
         top = top - k + total;
         wave_sync();
     }
-    // the stack goes right after the two counts, the list right after the stack
+    // the deferred nodes, then what is left on the stack, go right after the two counts,
+    // the list right after them
     int* const pstack = out + 2;
-    int* const plist = pstack + top;
-    for (int j = lane; j < top; j += 64) pstack[j] = stack[j];
+    if (nd + top > kStackCap) {
+        // too many for a walk's stack: the block's walks start at the root
+        if (lane == 0) {
+            pstack[0] = 0;
+            out[0] = n_nodes > 0 ? 1 : 0;
+            out[1] = 0;
+        }
+        return;
+    }
+    for (int j = lane; j < nd; j += 64) pstack[j] = defer[j];
+    for (int j = lane; j < top; j += 64) pstack[nd + j] = stack[j];
+    int* const plist = pstack + nd + top;
     for (int j = lane; j < held; j += 64) plist[j] = list[j];
     if (lane == 0) {
-        out[0] = top;
+        out[0] = nd + top;
         out[1] = held;
     }
 }

@@ -248,15 +248,16 @@ int gh_timing_read(gh_ctx *ctx, int kernel_class, double *total_ms, int64_t *lau
  *                        own; a run of more than 64 keys makes the build sort over all
  *                        digits, and so do the context's next 64 builds), 1 the same
  *                        without the 64 builds, -1 always over all digits
- *   GH_TUNE_WALK_PREFIX  FP32 tree walk: 0 each block of 64 groups first runs the steps
- *                        that decide alike for all of its groups once (k_walk_prefix),
- *                        and the groups start from there; -1 every group from the root
+ *   GH_TUNE_WALK_PREFIX  FP32 tree walk: 0 each block of 64 groups first decides once the
+ *                        nodes decided alike for all of its groups (k_walk_prefix), and
+ *                        the groups start from there; -1 every group from the root
  * The FP32 direct-sum keys and GH_TUNE_WALK_F64EVAL change results only by rounding;
  * GH_TUNE_WALK_GROUP changes which nodes the FP32 walk accepts (never fewer than the
  * reference's opening test accepts for any target); GH_TUNE_DIRECT_F64_TPW and
  * GH_TUNE_RUN_GRAPH do not change results, nor do GH_TUNE_SORT_DIRECT and
- * GH_TUNE_TREE_SORT (each of their forms gives the same order) and GH_TUNE_WALK_PREFIX
- * (the same lists in the same order); GH_TUNE_ALLOC_LIMIT_MB only decides whether a call may allocate. */
+ * GH_TUNE_TREE_SORT (each of their forms gives the same order); GH_TUNE_WALK_PREFIX keeps
+ * every target's accepted nodes and changes only their order, so the FP32 rounding of
+ * the sums; GH_TUNE_ALLOC_LIMIT_MB only decides whether a call may allocate. */
 #define GH_TUNE_DIRECT_SELF 0
 #define GH_TUNE_DIRECT_IPT 1
 #define GH_TUNE_WALK_GROUP 2

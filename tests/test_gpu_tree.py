@@ -301,9 +301,11 @@ def test_fp64_walk_independent_of_target_grouping(ctx):
 
 
 @pytest.mark.parametrize("group", [0, 1, 4, 16, 64])
-def test_fp32_walk_prefix_same_bits(ctx, group):
-    # the walks of a block of groups start from the steps they share (run once per block
-    # by k_walk_prefix); their lists, and so their forces, are those of walks from the root
+def test_fp32_walk_prefix_same_lists(ctx, group):
+    # the walks of a block of groups start from the nodes k_walk_prefix decided for all of
+    # them (accepted into the list, or opened) and the nodes it deferred; every target's
+    # list must hold as many nodes as from the root (the same nodes, in another order),
+    # and the forces differ only by FP32 summation order
     rng = np.random.default_rng(404)
     g = workloads.galaxy(n_halo=40000, n_disk=40000, seed=405)
     fpos = np.concatenate([g["pos"][:5000], rng.normal(size=(9000, 3)) * 300.0])
@@ -317,14 +319,21 @@ def test_fp32_walk_prefix_same_bits(ctx, group):
     try:
         for name, pos, mass, eps, fpos in cases:
             for theta in (0.0, 0.4, 0.7, 1.2):
-                print(name, "G", group, "theta", theta, flush=True)
-                got = ctx.tree(pos, mass, eps, theta, FP32, force_pos=fpos)
+                what = "%s G=%d theta=%g" % (name, group, theta)
+                print(what, flush=True)
+                got, gcnt = ctx.tree_interactions(pos, mass, eps, theta, force_pos=fpos,
+                                                  precision=FP32)
                 ctx.set_tuning(_native.TUNE_WALK_PREFIX, -1)
                 try:
-                    want = ctx.tree(pos, mass, eps, theta, FP32, force_pos=fpos)
+                    want, wcnt = ctx.tree_interactions(pos, mass, eps, theta, force_pos=fpos,
+                                                       precision=FP32)
                 finally:
                     ctx.set_tuning(_native.TUNE_WALK_PREFIX, 0)
-                assert_bitwise(got, want, "%s G=%d theta=%g" % (name, group, theta))
+                assert np.array_equal(gcnt, wcnt), what + ": list lengths"
+                fin = np.isfinite(want).all(axis=1)
+                assert np.array_equal(fin, np.isfinite(got).all(axis=1)), what
+                e = rel_err(got[fin], want[fin])
+                assert np.median(e) <= FP32_LIST_MEDIAN and np.percentile(e, 99) <= FP32_LIST_P99, what
     finally:
         ctx.set_tuning(_native.TUNE_WALK_GROUP, 0)
 
