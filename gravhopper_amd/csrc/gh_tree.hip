@@ -1059,6 +1059,7 @@ constexpr int kListCap = 128;
 constexpr int kPrefixGroups = 64;
 constexpr int kPrefixCap = 1024;
 constexpr int kPrefixSteps = 256;
+constexpr int kPrefixStart = kStackCap - 7 * 64;  // stack ids a walk may start with
 constexpr int kPrefixInts = 2 + kStackCap + kPrefixCap;
 // Two wavefronts per workgroup, each walking its own group: a CU holds at most 16
 // workgroups, so one-wavefront workgroups would cap it at 16 walks.
@@ -1398,10 +1399,10 @@ __device__ __forceinline__ int exponent_f(float x)
 // of it (no target passes, so every group opens it).  mu = 1e-4 (R + the box's largest
 // half-size) covers the FP32 rounding of the groups' tests (about 1e-7 of the distances)
 // and the widening of their boxes (1e-6 of their half-sizes).  The walk stops where its
-// stack is nearly full, after kPrefixSteps steps, or before its list or deferred nodes
-// would overflow; what is left on its stack is deferred too.  A block whose deferred
-// nodes would not fit a walk's stack (or with a target that is not finite) starts its
-// walks at the root.
+// stack is nearly full, after kPrefixSteps steps, or before its list would overflow or
+// its deferred nodes and stack together would exceed kPrefixStart ids (so that a group's
+// walk starting with them still pops 64 per step); what is left on its stack is deferred
+// too.  A block with a target that is not finite starts its walks at the root.
 __global__ __launch_bounds__(64) void k_walk_prefix(
     const NodeW* __restrict__ nodes, const int* __restrict__ kids, int n_nodes,
     const double* __restrict__ tgt, const double4* __restrict__ stgt,
@@ -1491,9 +1492,6 @@ __global__ __launch_bounds__(64) void k_walk_prefix(
         const bool undecided = active && !acc && !open;
         const u64 am = __ballot(acc), um = __ballot(undecided);
         const int added = (int)__popcll(am), ndef = (int)__popcll(um);
-        // a step whose results would not fit is not taken: its popped ids stay on the
-        // stack and are deferred with the rest
-        if (held + added > kPrefixCap || nd + ndef > kStackCap) break;
         int4 ka = make_int4(-1, -1, -1, -1), kb = ka;
         if (open) {
             ka = *(const int4*)(kids + 8 * (int64_t)n);
@@ -1504,6 +1502,10 @@ __global__ __launch_bounds__(64) void k_walk_prefix(
         if (!open) nk = 0;
         int total;
         const int pre = wave_excl_count(nk, total);
+        // a step whose results would not fit is not taken: its popped ids stay on the
+        // stack and are deferred with the rest (so a walk starts with at most
+        // kPrefixStart ids on its stack, and its first steps can still pop 64)
+        if (held + added > kPrefixCap || nd + ndef + (top - k + total) > kPrefixStart) break;
         wave_sync();  // every lane has read its popped id
         if (acc) list[held + lanes_below(am)] = n;
         if (undecided) defer[nd + lanes_below(um)] = n;
