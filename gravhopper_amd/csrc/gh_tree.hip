@@ -1054,12 +1054,13 @@ __device__ __forceinline__ int64_t xcd_block(int64_t per_xcd)
 // receives each target's list length.
 constexpr int kStackCap = 768;
 constexpr int kListCap = 128;
-// Walk prefixes (k_walk_prefix): groups per block, list entries and steps at most, and the
-// ints of one block's state (stack height, list length, stack, list).
+// Walk prefixes (k_walk_prefix): groups per block, accepted nodes and steps at most, the
+// stack ids a group's walk may start with, and the ints of one block's state (stack
+// height, list length, stack, list).
 constexpr int kPrefixGroups = 64;
 constexpr int kPrefixCap = 1024;
 constexpr int kPrefixSteps = 256;
-constexpr int kPrefixStart = kStackCap - 7 * 64;  // stack ids a walk may start with
+constexpr int kPrefixStart = kStackCap - 7 * 64;
 constexpr int kPrefixInts = 2 + kStackCap + kPrefixCap;
 // Two wavefronts per workgroup, each walking its own group: a CU holds at most 16
 // workgroups, so one-wavefront workgroups would cap it at 16 walks.
@@ -1135,7 +1136,9 @@ __device__ __forceinline__ WalkRec load_rec(const NodeW* __restrict__ nodes, int
 // and evaluation as described above.  This lane's target is (dx0, dy0, dz0) = x - o
 // (FP64; t its FP32 rounding), its slice of each flush `slice` of `nslices`, and the
 // group has cnt targets.  list holds kListCap entries; fails is LDS scratch of 64
-// entries.  Adds the lane's partial field to (ax, ay, az); listed = the list length.
+// entries.  prefix (nullable): the state k_walk_prefix left for the group's block, to
+// start from instead of the root.  Adds the lane's partial field to (ax, ay, az);
+// listed = the list length.
 template <bool F64, bool GUARD>
 __device__ __forceinline__ void walk_lists(
     const NodeW* __restrict__ nodes, const int* __restrict__ kids, const int* __restrict__ skip,
@@ -1272,8 +1275,8 @@ __device__ __forceinline__ void walk_lists(
     wave_sync();  // the previous walk of this wavefront is done with stack and list
     int top = 0;
     if (prefix) {
-        // the state k_walk_prefix left for this block: stack, then the accepted nodes'
-        // entries, flushed as the steps that accepted them would have flushed them
+        // the block's deferred nodes onto the stack, and its accepted nodes' entries into
+        // the list, evaluated by 64 as they come in
         top = prefix[0];
         const int npre = prefix[1];
         const int* pstack = prefix + 2;
