@@ -29,6 +29,19 @@ def test_library_exports_every_declared_symbol():
     assert sorted(_native.SIGNATURES) == syms
 
 
+def test_python_constants_match_header():
+    # every GH_TUNE_*, GH_T_* and precision / algorithm constant the Python layer names
+    # has the header's value
+    txt = open(os.path.join(ROOT, "include", "gravhop.h")).read()
+    defs = {k: int(v) for k, v in re.findall(r"#define\s+(GH_[A-Z0-9_]+)\s+(-?\d+)\b", txt)}
+    tune = {k: v for k, v in defs.items() if k.startswith("GH_TUNE_") and k != "GH_TUNE_COUNT"}
+    assert len(tune) == defs["GH_TUNE_COUNT"]
+    for k, v in tune.items():
+        assert getattr(_native, k[3:]) == v, k
+    for k in ("GH_FP64", "GH_FP32", "GH_DIRECT", "GH_TREE"):
+        assert getattr(_native, k) == defs[k], k
+
+
 def test_library_built_for_gfx950():
     data = open(_native.LIB_PATH, "rb").read()
     assert b"gfx950" in data
